@@ -1,0 +1,972 @@
+/*
+ * ORACLE -- test infrastructure only, never part of the product.
+ *
+ * CPU/OpenMP restatement of the program that opesci-fd's StaggeredGrid code
+ * generator emits for the 3-D isotropic elastic velocity-stress scheme
+ * (opesci/staggeredgrid.py, opesci/regulargrid.py, opesci/fields.py,
+ * opesci/templates/staggered3d_tmpl.py, driven by tests/eigenwave3d.py).
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load this library, as the checker or as the timed CPU baseline.  The
+ * product (opesci-fd_amd/) never links, imports or calls it.
+ *
+ * Two conventions are restated:
+ *   current  -- the generator as it stands: constants folded into float
+ *               literals (regulargrid.py:329-342 + codeprinter.py:42-63),
+ *               Levander normal-stress recompute at 4th order
+ *               (fields.py:313-353), L2 accumulated serially in real_t
+ *               (staggeredgrid.py:892-945).
+ *   snapshot -- the older generator that wrote benchmarks/test3d.cpp:
+ *               unfolded real_t constant arithmetic in the printed term
+ *               order (test3d.cpp:473-486,633-643), no normal-stress
+ *               recompute, Levander velocity ghosts (test3d.cpp:645-769).
+ *
+ * Memory layout follows the reference: F[tp=2][dim1][dim2][dim3], z fastest,
+ * every buffer zero-initialised (the reference relies on fresh zero pages).
+ */
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+#include <omp.h>
+
+extern "C" {
+
+/* Parameters shared with oracle/oracle.py (ctypes mirror, keep in sync). */
+typedef struct OracleParams {
+    int32_t order;        /* spatial order 2M: 2,4,6,8 */
+    int32_t n[3];         /* cells per axis = reference grid_size; dim = n+1+2M */
+    int32_t ntsteps;
+    int32_t real_bytes;   /* 4: float, 8: double (the reference 'double' switch) */
+    int32_t media_arrays; /* 0: constant media; 1: rho/vp/vs arrays of dim1*dim2*dim3 floats */
+    int32_t snapshot;     /* 1: benchmarks/test3d.cpp conventions */
+    int32_t nthreads;     /* OpenMP threads (0: runtime default) */
+    int32_t bc_algo;      /* -1: reference rule (levander iff order==4); 0: robertsson; 1: levander */
+    int32_t ic_mode;      /* 0: analytic eigenmode (tests/eigenwave3d.py:87-108); 1: all zero */
+    int32_t src_on;       /* 1: inject src_wavelet at src_idx into Txx/Tyy/Tzz */
+    int32_t src_idx[3];
+    int32_t nrec;         /* receivers recorded each step (U,V,W,P) */
+    int32_t pad0;
+    double  L[3];         /* domain size */
+    double  dt;
+    double  rho, lam, mu; /* constant media (reference Variables, declared float) */
+    const float*   rho_arr;
+    const float*   vp_arr;
+    const float*   vs_arr;
+    const float*   src_wavelet; /* ntsteps values */
+    const int32_t* rec_idx;     /* 3*nrec */
+    float*         rec_out;     /* 4*nrec*ntsteps, [step][rec][U,V,W,P] */
+} OracleParams;
+
+int oracle_run(const OracleParams* p, void* fields_out[9], double l2_f64[9],
+               double l2_ref[9], double timing[2]);
+int oracle_fd_weights(int half_order, double* w);
+}
+
+namespace {
+
+enum { FU = 0, FV, FW, TXX, TYY, TZZ, TXY, TYZ, TXZ, NF = 9 };
+
+/* staggering per field and axis (fields.py:175-190 VField.set, :270-292 SField.set) */
+const int STAG[NF][3] = {
+    {1, 0, 0}, {0, 1, 0}, {0, 0, 1},
+    {0, 0, 0}, {0, 0, 0}, {0, 0, 0},
+    {1, 1, 0}, {0, 1, 1}, {1, 0, 1}};
+const int TIME_STAG[NF] = {1, 1, 1, 0, 0, 0, 0, 0, 0};
+
+inline int shear_of(int i, int j) {
+    if (i > j) std::swap(i, j);
+    if (i == 0 && j == 1) return TXY;
+    if (i == 1 && j == 2) return TYZ;
+    return TXZ;
+}
+
+/* Staggered first-derivative weights (closed forms of util.Deriv_half,
+ * util.py:195-236): d/dx f ~ sum_k w_k (f(x+(k+1/2)h) - f(x-(k+1/2)h))/h. */
+const double FDW[5][4] = {
+    {0, 0, 0, 0},
+    {1.0, 0, 0, 0},
+    {9.0 / 8.0, -1.0 / 24.0, 0, 0},
+    {75.0 / 64.0, -25.0 / 384.0, 3.0 / 640.0, 0},
+    {1225.0 / 1024.0, -245.0 / 3072.0, 49.0 / 5120.0, -5.0 / 7168.0}};
+
+inline float f32(double v) { return (float)v; }
+
+template <typename R>
+struct Sim {
+    const OracleParams& P;
+    int M;                 /* margin m = order/2 */
+    int d[3];              /* dim1..dim3 */
+    size_t vol;
+    bool levander;
+    R dx[3];               /* const real_t dx1..dx3 */
+    R dt;
+    double dxd[3], dtd;
+    std::vector<R> F[NF];
+    /* media (read mode) */
+    std::vector<R> beta1, beta2, beta3, lam_a, mu_a, mu12, mu13, mu23, beta_a;
+    /* constant media as declared by the reference: const float */
+    float lam_f, mu_f, rho_f, beta_f;
+    double lamd, mud, rhod, betad;
+    /* folded interior coefficients (current convention) */
+    R cN[3][3][4];        /* normal stress i, source axis j, k */
+    R cNmu[3][4];         /* read mode: 2*c for the mu part of the own axis */
+    R cS[3][2][4];        /* shear pair p, group g, k */
+    R cV[3][3][4];        /* velocity i, axis j, k */
+
+    explicit Sim(const OracleParams& p) : P(p) {
+        M = p.order / 2;
+        for (int a = 0; a < 3; ++a) d[a] = p.n[a] + 1 + 2 * M;
+        vol = (size_t)d[0] * d[1] * d[2];
+        levander = p.bc_algo < 0 ? (p.order == 4) : (p.bc_algo == 1);
+        for (int a = 0; a < 3; ++a) {
+            dxd[a] = p.L[a] / (double)p.n[a];  /* regulargrid.py:194-196 */
+            dx[a] = (R)dxd[a];
+        }
+        dtd = p.dt;
+        dt = (R)dtd;
+        rhod = p.rho; lamd = p.lam; mud = p.mu; betad = 1.0 / p.rho;
+        rho_f = f32(rhod); lam_f = f32(lamd); mu_f = f32(mud); beta_f = f32(betad);
+        for (int f = 0; f < NF; ++f) F[f].assign(2 * vol, (R)0);
+        fold_coefficients();
+        if (p.media_arrays) build_media();
+    }
+
+    inline size_t ix(int x, int y, int z) const { return ((size_t)x * d[1] + y) * d[2] + z; }
+    inline R& at(int f, int l, int x, int y, int z) { return F[f][(size_t)l * vol + ix(x, y, z)]; }
+    inline R& atp(int f, int l, const int q[3]) { return F[f][(size_t)l * vol + ix(q[0], q[1], q[2])]; }
+
+    /* Constants folded in double then printed as float literals
+     * (regulargrid.py:329-342, codeprinter.py:42-63: every Float gets an F
+     * suffix, so even the double build multiplies by float-rounded values). */
+    void fold_coefficients() {
+        const double* w = FDW[M];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                for (int k = 0; k < M; ++k) {
+                    double base = dtd * w[k] / dxd[j];
+                    double media = P.media_arrays ? 1.0 : (i == j ? lamd + 2.0 * mud : lamd);
+                    cN[i][j][k] = (R)f32(base * media);
+                    cV[i][j][k] = (R)f32(base * (P.media_arrays ? 1.0 : betad));
+                }
+        for (int i = 0; i < 3; ++i)
+            for (int k = 0; k < M; ++k) cNmu[i][k] = (R)f32(2.0 * dtd * w[k] / dxd[i]);
+        /* shear pairs in field order Txy(0,1), Tyz(1,2), Txz(0,2):
+         * group 0 = d(V_i)/d(x_j) (coefficient of dx_i in the printed form),
+         * group 1 = d(V_j)/d(x_i) */
+        const int pr[3][2] = {{0, 1}, {1, 2}, {0, 2}};
+        for (int s = 0; s < 3; ++s)
+            for (int k = 0; k < M; ++k) {
+                double m_ = P.media_arrays ? 1.0 : mud;
+                cS[s][0][k] = (R)f32(dtd * w[k] / dxd[pr[s][1]] * m_);
+                cS[s][1][k] = (R)f32(dtd * w[k] / dxd[pr[s][0]] * m_);
+            }
+    }
+
+    /* Effective media (staggeredgrid.py:571-597).  The reference fills the
+     * averages over [m, dim-m) only, so the Levander ghost formulas divide by
+     * never-written (zero) cells and turn NaN; here every cell whose
+     * neighbours exist is filled (last index clamps), which reproduces the
+     * reference wherever the reference is finite. */
+    void build_media() {
+        std::vector<R> rho(vol), vp(vol), vs(vol);
+        for (size_t q = 0; q < vol; ++q) {
+            rho[q] = (R)P.rho_arr[q]; vp[q] = (R)P.vp_arr[q]; vs[q] = (R)P.vs_arr[q];
+        }
+        beta_a.assign(vol, 0); lam_a.assign(vol, 0); mu_a.assign(vol, 0);
+        beta1.assign(vol, 0); beta2.assign(vol, 0); beta3.assign(vol, 0);
+        mu12.assign(vol, 0); mu13.assign(vol, 0); mu23.assign(vol, 0);
+        for (size_t q = 0; q < vol; ++q) {
+            double r = rho[q], a = vp[q], b = vs[q];
+            beta_a[q] = (R)(1.0 / r);
+            lam_a[q] = (R)(r * (a * a - 2.0 * b * b));
+            mu_a[q] = (R)(r * (b * b));
+        }
+        for (int x = 0; x < d[0]; ++x)
+            for (int y = 0; y < d[1]; ++y)
+                for (int z = 0; z < d[2]; ++z) {
+                    int x1 = std::min(x + 1, d[0] - 1), y1 = std::min(y + 1, d[1] - 1), z1 = std::min(z + 1, d[2] - 1);
+                    size_t q = ix(x, y, z);
+                    beta1[q] = (R)(((double)beta_a[q] + beta_a[ix(x1, y, z)]) / 2.0);
+                    beta2[q] = (R)(((double)beta_a[q] + beta_a[ix(x, y1, z)]) / 2.0);
+                    beta3[q] = (R)(((double)beta_a[q] + beta_a[ix(x, y, z1)]) / 2.0);
+                    auto hm = [&](size_t a, size_t b, size_t c, size_t e) {
+                        return (R)(1.0 / (0.25 * (1.0 / (double)mu_a[a] + 1.0 / (double)mu_a[b] +
+                                                  1.0 / (double)mu_a[c] + 1.0 / (double)mu_a[e])));
+                    };
+                    mu12[q] = hm(q, ix(x1, y, z), ix(x, y1, z), ix(x1, y1, z));
+                    mu13[q] = hm(q, ix(x1, y, z), ix(x, y, z1), ix(x1, y, z1));
+                    mu23[q] = hm(q, ix(x, y1, z), ix(x, y, z1), ix(x, y1, z1));
+                }
+    }
+
+    const std::vector<R>& shear_mu(int f) const { return f == TXY ? mu12 : (f == TYZ ? mu23 : mu13); }
+    const std::vector<R>& beta_of(int i) const { return i == 0 ? beta1 : (i == 1 ? beta2 : beta3); }
+
+    /* ---------------- coordinates and analytic eigenmode ---------------- */
+    /* real_t x = dx1*(_x - m) or dx1*(_x - m + 0.5) (staggeredgrid.py:632-641):
+     * the product is exact in double and rounds once to real_t. */
+    inline R coord(int a, int i, int stag) const {
+        return (R)((double)dx[a] * ((double)(i - M) + (stag ? 0.5 : 0.0)));
+    }
+
+    /* Eigenmode of tests/eigenwave3d.py:87-98 with Omega = pi*sqrt(2 mu beta),
+     * A = sqrt(2 mu / beta), evaluated in double from real_t coordinates and
+     * float media (the reference declares media Variables as float). */
+    static double eigen(int f, double x, double y, double z, double t, double mu, double beta) {
+        const double PI = 3.14159265358979323846;
+        double Om = PI * std::sqrt(2.0 * mu * beta), A = std::sqrt(2.0 * mu / beta);
+        double sx = std::sin(PI * x), sy = std::sin(PI * y), sz = std::sin(PI * z);
+        switch (f) {
+            case FU: return std::cos(PI * x) * (sy - sz) * std::cos(Om * t);
+            case FV: return std::cos(PI * y) * (sz - sx) * std::cos(Om * t);
+            case FW: return std::cos(PI * z) * (sx - sy) * std::cos(Om * t);
+            case TXX: return -A * sx * (sy - sz) * std::sin(Om * t);
+            case TYY: return -A * sy * (sz - sx) * std::sin(Om * t);
+            case TZZ: return -A * sz * (sx - sy) * std::sin(Om * t);
+            default: return 0.0;
+        }
+    }
+
+    /* Snapshot (benchmarks/test3d.cpp:90-190 / :778-888) spelling of the same
+     * mode: time literal in double, sqrt(mu/rho) and sqrt(mu*rho) through the
+     * double ::sqrt of a float quotient/product, evaluated left to right. */
+    double eigen_snapshot(int f, R x, R y, R z, double t) const {
+        const double PI = 3.14159265358979323846;
+        double sq2 = std::sqrt(2.0);
+        /* unqualified sqrt on a float argument resolves to ::sqrt(double) */
+        double smr = std::sqrt((double)(float)(mu_f / rho_f));
+        double smp = std::sqrt((double)(float)(mu_f * rho_f));
+        double X = x, Y = y, Z = z;
+        switch (f) {
+            case FU: return (std::sin(PI * Y) - std::sin(PI * Z)) * std::cos(PI * X) * std::cos(t * sq2 * PI * smr);
+            case FV: return (-std::sin(PI * X) + std::sin(PI * Z)) * std::cos(PI * Y) * std::cos(t * sq2 * PI * smr);
+            case FW: return (std::sin(PI * X) - std::sin(PI * Y)) * std::cos(PI * Z) * std::cos(t * sq2 * PI * smr);
+            case TXX: return -(sq2 * smp * (std::sin(PI * Y) - std::sin(PI * Z)) * std::sin(PI * X) * std::sin(t * sq2 * PI * smr));
+            case TYY: return -(sq2 * smp * (-std::sin(PI * X) + std::sin(PI * Z)) * std::sin(PI * Y) * std::sin(t * sq2 * PI * smr));
+            case TZZ: return -(sq2 * smp * (std::sin(PI * X) - std::sin(PI * Y)) * std::sin(PI * Z) * std::sin(t * sq2 * PI * smr));
+            default: return 0.0;
+        }
+    }
+
+    /* Current-generator spelling of the analytic solution in the L2 loop:
+     * float time literal, ::sqrt(double) of the float products beta*mu and
+     * mu/beta (staggeredgrid.py:931-935 + codeprinter.py:42-63). */
+    double eigen_current_ref(int f, R x, R y, R z, double t) const {
+        const double PI = 3.14159265358979323846;
+        double tf = (double)f32(t);
+        double sbm = std::sqrt((double)(float)(beta_f * mu_f));
+        double smb = std::sqrt((double)(float)(mu_f / beta_f));
+        double sq2 = std::sqrt(2.0);
+        double Om = sq2 * PI * sbm, A = sq2 * smb;
+        double X = x, Y = y, Z = z;
+        double sx = std::sin(PI * X), sy = std::sin(PI * Y), sz = std::sin(PI * Z);
+        switch (f) {
+            case FU: return std::cos(PI * X) * (sy - sz) * std::cos(tf * Om);
+            case FV: return std::cos(PI * Y) * (sz - sx) * std::cos(tf * Om);
+            case FW: return std::cos(PI * Z) * (sx - sy) * std::cos(tf * Om);
+            case TXX: return -A * sx * (sy - sz) * std::sin(tf * Om);
+            case TYY: return -A * sy * (sz - sx) * std::sin(tf * Om);
+            case TZZ: return -A * sz * (sx - sy) * std::sin(tf * Om);
+            default: return 0.0;
+        }
+    }
+
+    inline void media_at(size_t q, double& mu, double& beta) const {
+        if (P.media_arrays) { mu = (double)mu_a[q]; beta = (double)beta_a[q]; }
+        else { mu = (double)mu_f; beta = (double)beta_f; }
+    }
+
+    /* StaggeredGrid.initialise (staggeredgrid.py:612-659): level 0, interior
+     * ranges [m, dim-m) or [m, dim-m-1) when staggered, t=dt/2 for velocities. */
+    void initialise() {
+        if (P.ic_mode != 0) return;
+        for (int f = 0; f < NF; ++f) {
+            int hi[3];
+            for (int a = 0; a < 3; ++a) hi[a] = d[a] - M - (STAG[f][a] ? 1 : 0);
+            double t0 = TIME_STAG[f] ? dtd / 2.0 : 0.0;
+#pragma omp for schedule(static, 1)
+            for (int x = M; x < hi[0]; ++x)
+                for (int y = M; y < hi[1]; ++y)
+                    for (int z = M; z < hi[2]; ++z) {
+                        R X = coord(0, x, STAG[f][0]), Y = coord(1, y, STAG[f][1]), Z = coord(2, z, STAG[f][2]);
+                        double v;
+                        if (P.snapshot) {
+                            v = eigen_snapshot(f, X, Y, Z, t0);
+                            if (v == 0.0) v = 0.0; /* the snapshot prints a literal 0 */
+                        } else {
+                            double mu, beta;
+                            media_at(ix(x, y, z), mu, beta);
+                            v = eigen(f, X, Y, Z, t0, mu, beta);
+                        }
+                        at(f, 0, x, y, z) = (R)v;
+                    }
+        }
+    }
+
+    /* ---------------- interior kernels ---------------- */
+    /* D(F, axis, a, k) terms: F[p + (a+k) e] - F[p + (a-1-k) e]; a is 1
+     * when the output is staggered on the derivative axis (fields.py:98-138
+     * Field.align shifts +1/2 / -1/2). */
+
+    /* Pointer-stride form of the folded kernels, one instantiation per half
+     * order so every offset is a compile-time constant like the generated
+     * code's array subscripts. */
+    template <int MM, bool HET>
+    void stress_current_t(int t0, int t1) {
+        const ptrdiff_t S[3] = {(ptrdiff_t)d[1] * d[2], (ptrdiff_t)d[2], 1};
+        const R* v0[3] = {F[FU].data() + t0 * vol, F[FV].data() + t0 * vol, F[FW].data() + t0 * vol};
+        const R* s0[6];
+        R* s1[6];
+        for (int f = 0; f < 6; ++f) { s0[f] = F[TXX + f].data() + t0 * vol; s1[f] = F[TXX + f].data() + t1 * vol; }
+        R cn[3][3][MM], cm[3][MM], cs[3][2][MM];
+        for (int i = 0; i < 3; ++i)
+            for (int k = 0; k < MM; ++k) {
+                cm[i][k] = cNmu[i][k];
+                for (int j = 0; j < 3; ++j) cn[i][j][k] = cN[i][j][k];
+                if (i < 3) { cs[i][0][k] = cS[i][0][k]; cs[i][1][k] = cS[i][1][k]; }
+            }
+        const int pr[3][2] = {{0, 1}, {1, 2}, {0, 2}};
+        const R* lam = HET ? lam_a.data() : nullptr;
+        const R* mu = HET ? mu_a.data() : nullptr;
+        const R* msh[3] = {HET ? mu12.data() : nullptr, HET ? mu23.data() : nullptr, HET ? mu13.data() : nullptr};
+#pragma omp for schedule(static, 1)
+        for (int x = MM; x < d[0] - MM; ++x)
+            for (int y = MM; y < d[1] - MM; ++y) {
+                const size_t base = ix(x, y, 0);
+#pragma GCC ivdep
+                for (int z = MM; z < d[2] - MM; ++z) {
+                    const size_t q = base + z;
+                    for (int i = 0; i < 3; ++i) {
+                        R acc = s0[i][q];
+                        for (int j = 0; j < 3; ++j) {
+                            const R* v = v0[j] + q;
+                            for (int k = 0; k < MM; ++k) {
+                                R fp = v[k * S[j]], fm = v[-(k + 1) * S[j]];
+                                R c = HET ? cn[i][j][k] * lam[q] : cn[i][j][k];
+                                acc = acc + c * fp;
+                                acc = acc - c * fm;
+                                if (HET && j == i) {
+                                    R c2 = cm[i][k] * mu[q];
+                                    acc = acc + c2 * fp;
+                                    acc = acc - c2 * fm;
+                                }
+                            }
+                        }
+                        s1[i][q] = acc;
+                    }
+                    for (int sp = 0; sp < 3; ++sp) {
+                        const int ia = pr[sp][0], ja = pr[sp][1];
+                        R acc = s0[3 + sp][q];
+                        for (int g = 0; g < 2; ++g) {
+                            const int src = g == 0 ? ia : ja, ax = g == 0 ? ja : ia;
+                            const R* v = v0[src] + q;
+                            for (int k = 0; k < MM; ++k) {
+                                R c = HET ? cs[sp][g][k] * msh[sp][q] : cs[sp][g][k];
+                                acc = acc + c * v[(1 + k) * S[ax]];
+                                acc = acc - c * v[-k * S[ax]];
+                            }
+                        }
+                        s1[3 + sp][q] = acc;
+                    }
+                }
+            }
+    }
+
+    template <int MM, bool HET>
+    void velocity_current_t(int t0, int t1) {
+        const ptrdiff_t S[3] = {(ptrdiff_t)d[1] * d[2], (ptrdiff_t)d[2], 1};
+        const R* st[6];
+        for (int f = 0; f < 6; ++f) st[f] = F[TXX + f].data() + t1 * vol;
+        const R* v0[3];
+        R* v1[3];
+        for (int i = 0; i < 3; ++i) { v0[i] = F[FU + i].data() + t0 * vol; v1[i] = F[FU + i].data() + t1 * vol; }
+        R cv[3][3][MM];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                for (int k = 0; k < MM; ++k) cv[i][j][k] = cV[i][j][k];
+        const R* bt[3] = {HET ? beta1.data() : nullptr, HET ? beta2.data() : nullptr, HET ? beta3.data() : nullptr};
+#pragma omp for schedule(static, 1)
+        for (int x = MM; x < d[0] - MM; ++x)
+            for (int y = MM; y < d[1] - MM; ++y) {
+                const size_t base = ix(x, y, 0);
+#pragma GCC ivdep
+                for (int z = MM; z < d[2] - MM; ++z) {
+                    const size_t q = base + z;
+                    for (int i = 0; i < 3; ++i) {
+                        R acc = v0[i][q];
+                        for (int j = 0; j < 3; ++j) {
+                            const int src = (i == j) ? i : (shear_of(i, j) - TXX);
+                            const int a = (i == j) ? 1 : 0;
+                            const R* t = st[src] + q;
+                            for (int k = 0; k < MM; ++k) {
+                                R c = HET ? cv[i][j][k] * bt[i][q] : cv[i][j][k];
+                                acc = acc + c * t[(a + k) * S[j]];
+                                acc = acc - c * t[(a - 1 - k) * S[j]];
+                            }
+                        }
+                        v1[i][q] = acc;
+                    }
+                }
+            }
+    }
+
+    void stress_current(int t0, int t1) {
+        const bool h = P.media_arrays != 0;
+        switch (M) {
+            case 1: h ? stress_current_t<1, true>(t0, t1) : stress_current_t<1, false>(t0, t1); break;
+            case 2: h ? stress_current_t<2, true>(t0, t1) : stress_current_t<2, false>(t0, t1); break;
+            case 3: h ? stress_current_t<3, true>(t0, t1) : stress_current_t<3, false>(t0, t1); break;
+            default: h ? stress_current_t<4, true>(t0, t1) : stress_current_t<4, false>(t0, t1); break;
+        }
+    }
+
+    void velocity_current(int t0, int t1) {
+        const bool h = P.media_arrays != 0;
+        switch (M) {
+            case 1: h ? velocity_current_t<1, true>(t0, t1) : velocity_current_t<1, false>(t0, t1); break;
+            case 2: h ? velocity_current_t<2, true>(t0, t1) : velocity_current_t<2, false>(t0, t1); break;
+            case 3: h ? velocity_current_t<3, true>(t0, t1) : velocity_current_t<3, false>(t0, t1); break;
+            default: h ? velocity_current_t<4, true>(t0, t1) : velocity_current_t<4, false>(t0, t1); break;
+        }
+    }
+
+    /* snapshot kernels: test3d.cpp:473-486 and :633-643 term order.
+     * S0: 27*F[0] + F[-2] - 27*F[-1] - F[+1];  S1: -27*F[0] + F[-1] + 27*F[+1] - F[+2] */
+    inline R S0(int f, int l, const int p[3], int ax) {
+        int a[3] = {p[0], p[1], p[2]}, b[3] = {p[0], p[1], p[2]}, c[3] = {p[0], p[1], p[2]};
+        a[ax] -= 2; b[ax] -= 1; c[ax] += 1;
+        R r = (R)27 * atp(f, l, p) + atp(f, l, a);
+        r = r - (R)27 * atp(f, l, b);
+        r = r - atp(f, l, c);
+        return r;
+    }
+    inline R S1(int f, int l, const int p[3], int ax) {
+        int a[3] = {p[0], p[1], p[2]}, b[3] = {p[0], p[1], p[2]}, c[3] = {p[0], p[1], p[2]};
+        a[ax] -= 1; b[ax] += 1; c[ax] += 2;
+        R r = (R)(-27) * atp(f, l, p) + atp(f, l, a);
+        r = r + (R)27 * atp(f, l, b);
+        r = r - atp(f, l, c);
+        return r;
+    }
+    /* product of the two spacings other than axis j, in increasing index order */
+    inline R Pexcl(int j) const {
+        if (j == 0) return dx[1] * dx[2];
+        if (j == 1) return dx[0] * dx[2];
+        return dx[0] * dx[1];
+    }
+
+    void stress_snapshot(int t0, int t1) {
+        const R lam = (R)lam_f, mu = (R)mu_f;
+        const R vol3 = dx[0] * dx[1] * dx[2];
+        const R c24 = (R)(1.0f / 24.0f);
+#pragma omp for schedule(static, 1)
+        for (int x = M; x < d[0] - M; ++x)
+            for (int y = M; y < d[1] - M; ++y)
+                for (int z = M; z < d[2] - M; ++z) {
+                    const int p[3] = {x, y, z};
+                    for (int i = 0; i < 3; ++i) {
+                        R sum = (R)0;
+                        bool first = true;
+                        for (int j = 2; j >= 0; --j) {   /* lambda groups, dx1*dx2 < dx1*dx3 < dx2*dx3 */
+                            if (j == i) continue;
+                            R g = lam * dt;
+                            if (j == 0) { g = g * dx[1]; g = g * dx[2]; }
+                            if (j == 1) { g = g * dx[0]; g = g * dx[2]; }
+                            if (j == 2) { g = g * dx[0]; g = g * dx[1]; }
+                            g = g * S0(FU + j, t0, p, j);
+                            sum = first ? g : sum + g;
+                            first = false;
+                        }
+                        /* own axis: 27*lambda*F0 + lambda*F-2 - 27*lambda*F-1 - lambda*F+1 + 54*mu*F0 + 2*mu*F-2 - 54*mu*F-1 - 2*mu*F+1 */
+                        int fi = FU + i;
+                        int a[3] = {p[0], p[1], p[2]}, b[3] = {p[0], p[1], p[2]}, c[3] = {p[0], p[1], p[2]};
+                        a[i] -= 2; b[i] -= 1; c[i] += 1;
+                        R F0 = atp(fi, t0, p), Fm2 = atp(fi, t0, a), Fm1 = atp(fi, t0, b), Fp1 = atp(fi, t0, c);
+                        R o = (R)27 * lam * F0 + lam * Fm2;
+                        o = o - (R)27 * lam * Fm1;
+                        o = o - lam * Fp1;
+                        o = o + (R)54 * mu * F0;
+                        o = o + (R)2 * mu * Fm2;
+                        o = o - (R)54 * mu * Fm1;
+                        o = o - (R)2 * mu * Fp1;
+                        R g = dt;
+                        if (i == 0) { g = g * dx[1]; g = g * dx[2]; }
+                        if (i == 1) { g = g * dx[0]; g = g * dx[2]; }
+                        if (i == 2) { g = g * dx[0]; g = g * dx[1]; }
+                        sum = sum + g * o;
+                        R tt = (R)24 * dx[0];
+                        tt = tt * dx[1]; tt = tt * dx[2];
+                        sum = sum + tt * at(TXX + i, t0, x, y, z);
+                        at(TXX + i, t1, x, y, z) = c24 * sum / vol3;
+                    }
+                    const int pr[3][2] = {{0, 1}, {1, 2}, {0, 2}};
+                    for (int s = 0; s < 3; ++s) {
+                        int f = TXY + s, ia = pr[s][0], ja = pr[s][1];
+                        R g1 = mu * dt; g1 = g1 * dx[ia]; g1 = g1 * S1(FU + ia, t0, p, ja);
+                        R g2 = mu * dt; g2 = g2 * dx[ja]; g2 = g2 * S1(FU + ja, t0, p, ia);
+                        R tt = (R)24 * dx[ia]; tt = tt * dx[ja];
+                        R sum = g1 + g2;
+                        sum = sum + tt * at(f, t0, x, y, z);
+                        at(f, t1, x, y, z) = c24 * sum / (dx[ia] * dx[ja]);
+                    }
+                }
+    }
+
+    void velocity_snapshot(int t0, int t1) {
+        const R beta = (R)beta_f;
+        const R vol3 = dx[0] * dx[1] * dx[2];
+        const R c24 = (R)(1.0f / 24.0f);
+#pragma omp for schedule(static, 1)
+        for (int x = M; x < d[0] - M; ++x)
+            for (int y = M; y < d[1] - M; ++y)
+                for (int z = M; z < d[2] - M; ++z) {
+                    const int p[3] = {x, y, z};
+                    for (int i = 0; i < 3; ++i) {
+                        R sum = (R)0;
+                        bool first = true;
+                        for (int j = 2; j >= 0; --j) {
+                            int src = (i == j) ? TXX + i : shear_of(i, j);
+                            R g = beta * dt;
+                            if (j == 0) { g = g * dx[1]; g = g * dx[2]; }
+                            if (j == 1) { g = g * dx[0]; g = g * dx[2]; }
+                            if (j == 2) { g = g * dx[0]; g = g * dx[1]; }
+                            g = g * ((i == j) ? S1(src, t1, p, j) : S0(src, t1, p, j));
+                            sum = first ? g : sum + g;
+                            first = false;
+                        }
+                        R tt = (R)24 * dx[0];
+                        tt = tt * dx[1]; tt = tt * dx[2];
+                        sum = sum + tt * at(FU + i, t0, x, y, z);
+                        at(FU + i, t1, x, y, z) = c24 * sum / vol3;
+                    }
+                }
+    }
+
+    /* ---------------- boundary conditions ---------------- */
+    /* plane loop over the two axes other than d, with the reference nesting
+     * (inner loop = highest remaining axis, staggeredgrid.py:781-811) */
+    template <typename Fn>
+    void plane_loop(int dax, int lo, int hi_off, Fn fn) {
+        int o0 = dax == 0 ? 1 : 0, o1 = dax == 2 ? 1 : 2;
+        int hi0 = d[o0] - hi_off, hi1 = d[o1] - hi_off;
+#pragma omp for schedule(static, 1)
+        for (int a = lo; a < hi0; ++a)
+            for (int b = lo; b < hi1; ++b) {
+                int q[3];
+                q[dax] = 0; q[o0] = a; q[o1] = b;
+                fn(q);
+            }
+    }
+
+    inline int bplane(int dax, int side) const { return side == 0 ? M : d[dax] - M - 1; }
+
+    /* SField.set_free_surface antisymmetry (fields.py:355-381) */
+    void stress_antisym(int f, int dax, int side, int lvl) {
+        int b = bplane(dax, side);
+        int sg = side == 0 ? 1 : -1;   /* (-1)**side */
+        bool st = STAG[f][dax];
+        plane_loop(dax, 0, 0, [&](int q[3]) {
+            int g[3] = {q[0], q[1], q[2]}, s[3] = {q[0], q[1], q[2]};
+            if (!st) {
+                g[dax] = b;
+                atp(f, lvl, g) = (R)0;
+                for (int k = 1; k < M; ++k) {
+                    g[dax] = b - sg * k; s[dax] = b + sg * k;
+                    atp(f, lvl, g) = -atp(f, lvl, s);
+                }
+            } else {
+                int gi = b - (1 - side), si = gi + sg;
+                for (int k = 0; k < M; ++k) {
+                    g[dax] = gi - sg * k; s[dax] = si + sg * k;
+                    atp(f, lvl, g) = -atp(f, lvl, s);
+                }
+            }
+        });
+    }
+
+    /* Levander normal-stress recompute (fields.py:318-353): T_ii at the plane
+     * of a free surface normal to dax != i, with d(V_dax)/d(x_dax) eliminated
+     * through T_dd' = 0, 4th-order FD in the tangential directions. */
+    void stress_tnn(int f, int dax, int side, int t0, int t1) {
+        int i = f - TXX;
+        int j = 3 - i - dax;
+        int b = bplane(dax, side);
+        const double* w = FDW[2];
+        R ci[2], cj[2];
+        if (!P.media_arrays) {
+            double l2m = lamd + 2.0 * mud;
+            double ki = l2m - lamd * lamd / l2m, kj = lamd - lamd * lamd / l2m;
+            for (int k = 0; k < 2; ++k) {
+                ci[k] = (R)f32(dtd * w[k] / dxd[i] * ki);
+                cj[k] = (R)f32(dtd * w[k] / dxd[j] * kj);
+            }
+        }
+        plane_loop(dax, M + 1, M + 1, [&](int q[3]) {
+            q[dax] = b;
+            R c_i[2] = {ci[0], ci[1]}, c_j[2] = {cj[0], cj[1]};
+            if (P.media_arrays) {
+                size_t qq = ix(q[0], q[1], q[2]);
+                double l = lam_a[qq], m = mu_a[qq], l2m = l + 2.0 * m;
+                R ki = (R)(l2m - l * l / l2m), kj = (R)(l - l * l / l2m);
+                for (int k = 0; k < 2; ++k) {
+                    c_i[k] = (R)f32(dtd * w[k] / dxd[i]) * ki;
+                    c_j[k] = (R)f32(dtd * w[k] / dxd[j]) * kj;
+                }
+            }
+            R acc = atp(f, t0, q);
+            for (int g = 0; g < 2; ++g) {
+                int ax = g == 0 ? i : j;
+                for (int k = 0; k < 2; ++k) {
+                    int qp[3] = {q[0], q[1], q[2]}, qm[3] = {q[0], q[1], q[2]};
+                    qp[ax] += k; qm[ax] -= k + 1;
+                    R c = g == 0 ? c_i[k] : c_j[k];
+                    acc = acc + c * atp(FU + ax, t0, qp);
+                    acc = acc - c * atp(FU + ax, t0, qm);
+                }
+            }
+            atp(f, t1, q) = acc;
+        });
+    }
+
+    /* StaggeredGrid.stress_bc_getter order (staggeredgrid.py:754-813) */
+    void stress_bc(bool init, int t0, int t1) {
+        for (int f = TXX; f <= TXZ; ++f) {
+            bool normal = f <= TZZ;
+            for (int dax = 0; dax < 3; ++dax) {
+                bool own = normal ? (f - TXX == dax) : (STAG[f][dax] != 0);
+                if (normal && !own) {
+                    if (init || !levander || P.snapshot) continue;
+                    for (int side = 0; side < 2; ++side) stress_tnn(f, dax, side, t0, t1);
+                    continue;
+                }
+                if (!own) continue; /* shear not involving dax: no BC */
+                for (int side = 0; side < 2; ++side) stress_antisym(f, dax, side, t1);
+            }
+        }
+    }
+
+    /* VField.set_free_surface, Robertsson branch (fields.py:243-259) */
+    void vel_robertsson(int f, int dax, int side, int lvl) {
+        int b = bplane(dax, side);
+        int sg = side == 0 ? 1 : -1;
+        int g0 = STAG[f][dax] ? b - (1 - side) : b - sg;
+        plane_loop(dax, 1, 1, [&](int q[3]) {
+            for (int k = 0; k < M; ++k) {
+                q[dax] = g0 - sg * k;
+                atp(f, lvl, q) = (R)0;
+            }
+        });
+    }
+
+    /* VField.set_free_surface, Levander branch (fields.py:208-242):
+     * staggered V_d from T_dd' = 0, others from T_id'(b-1/2) = T_id'(b+1/2),
+     * both with 2nd-order differences. */
+    void vel_levander(int f, int dax, int side, int lvl) {
+        int i = f - FU;
+        int b = bplane(dax, side);
+        if (i == dax) {
+            /* V_d[b-1] = V_d[b] + r * sum_j dx_d/dx_j (V_j[b][x_j] - V_j[b][x_j-1])   (side 0)
+             * V_d[b]   = V_d[b-1] - r * sum_j ...                                     (side 1) */
+            int gi = side == 0 ? b - 1 : b, si = side == 0 ? b : b - 1;
+            double sgn = side == 0 ? 1.0 : -1.0;
+            plane_loop(dax, 1, 1, [&](int q[3]) {
+                if (P.snapshot) { vel_levander_snapshot_stag(i, dax, side, b, q, lvl); return; }
+                int s[3] = {q[0], q[1], q[2]};
+                s[dax] = si;
+                R acc = atp(FU + i, lvl, s);
+                double ratio;
+                if (P.media_arrays) {
+                    int qb[3] = {q[0], q[1], q[2]};
+                    qb[dax] = b;
+                    size_t qq = ix(qb[0], qb[1], qb[2]);
+                    ratio = (double)lam_a[qq] / ((double)lam_a[qq] + 2.0 * (double)mu_a[qq]);
+                } else {
+                    ratio = lamd / (lamd + 2.0 * mud);
+                }
+                for (int j = 0; j < 3; ++j) {
+                    if (j == dax) continue;
+                    int qp[3] = {q[0], q[1], q[2]}, qm[3] = {q[0], q[1], q[2]};
+                    qp[dax] = b; qm[dax] = b; qm[j] -= 1;
+                    R c = P.media_arrays ? (R)(f32(sgn * dxd[dax] / dxd[j]) * ratio)
+                                         : (R)f32(sgn * ratio * dxd[dax] / dxd[j]);
+                    acc = acc + c * atp(FU + j, lvl, qp);
+                    acc = acc - c * atp(FU + j, lvl, qm);
+                }
+                q[dax] = gi;
+                atp(FU + i, lvl, q) = acc;
+            });
+        } else {
+            /* V_i[b-1] = 2V_i[b] - V_i[b+1] + dx_d/dx_i (V_d[b-1][x_i+1] - V_d[b-1][x_i] - V_d[b][x_i+1] + V_d[b][x_i]) */
+            int gi = side == 0 ? b - 1 : b + 1, fi = side == 0 ? b + 1 : b - 1;
+            plane_loop(dax, 1, 1, [&](int q[3]) {
+                if (P.snapshot) { vel_levander_snapshot_other(i, dax, side, b, q, lvl); return; }
+                int qb[3] = {q[0], q[1], q[2]}, qf[3] = {q[0], q[1], q[2]};
+                qb[dax] = b; qf[dax] = fi;
+                int a0[3] = {q[0], q[1], q[2]}, a1[3] = {q[0], q[1], q[2]}, a2[3] = {q[0], q[1], q[2]}, a3[3] = {q[0], q[1], q[2]};
+                a0[dax] = b - 1; a0[i] += 1;   /* V_d[b-1][x_i+1] */
+                a1[dax] = b - 1;               /* V_d[b-1][x_i]   */
+                a2[dax] = b; a2[i] += 1;       /* V_d[b][x_i+1]   */
+                a3[dax] = b;                   /* V_d[b][x_i]     */
+                R acc;
+                if (!P.media_arrays) {
+                    R c = (R)f32(dxd[dax] / dxd[i]);
+                    acc = (R)2 * atp(FU + i, lvl, qb);
+                    acc = acc - atp(FU + i, lvl, qf);
+                    acc = acc + c * atp(FU + dax, lvl, a0);
+                    acc = acc - c * atp(FU + dax, lvl, a1);
+                    acc = acc - c * atp(FU + dax, lvl, a2);
+                    acc = acc + c * atp(FU + dax, lvl, a3);
+                } else {
+                    /* mu_L (A_L + (V[b]-V[b-1])/h) = mu_R (A_R + (V[b+1]-V[b])/h) with
+                     * mu_L = mu_id[b-1], mu_R = mu_id[b] (effective shear modulus of T_id) */
+                    const std::vector<R>& mid = shear_mu(shear_of(i, dax));
+                    int ql[3] = {q[0], q[1], q[2]}, qr[3] = {q[0], q[1], q[2]};
+                    ql[dax] = b - 1; qr[dax] = b;
+                    double muL = mid[ix(ql[0], ql[1], ql[2])], muR = mid[ix(qr[0], qr[1], qr[2])];
+                    double h = dxd[dax], hi_ = dxd[i];
+                    double AL = ((double)atp(FU + dax, lvl, a0) - (double)atp(FU + dax, lvl, a1)) / hi_;
+                    double AR = ((double)atp(FU + dax, lvl, a2) - (double)atp(FU + dax, lvl, a3)) / hi_;
+                    double Vb = atp(FU + i, lvl, qb), Vf = atp(FU + i, lvl, qf);
+                    double v;
+                    if (side == 0) v = Vb + h * AL - (muR / muL) * (h * AR + Vf - Vb);
+                    else v = Vb - h * AR + (muL / muR) * (h * AL + Vb - Vf);
+                    acc = (R)v;
+                }
+                q[dax] = gi;
+                atp(FU + i, lvl, q) = acc;
+            });
+        }
+    }
+
+    /* snapshot spellings (test3d.cpp:645-769): staggered V_d ghost */
+    void vel_levander_snapshot_stag(int i, int dax, int side, int b, int q[3], int lvl) {
+        const R lam = (R)lam_f, mu = (R)mu_f;
+        /* terms in printed order: by coefficient monomial dx1*dx2 < dx1*dx3 < dx2*dx3, i.e. axis z, y, x;
+         * own axis lambda-term in its slot, 2*mu term last */
+        int gi = side == 0 ? b - 1 : b, si = side == 0 ? b : b - 1;
+        R sum = (R)0;
+        bool first = true;
+        for (int j = 2; j >= 0; --j) {
+            R P2 = (j == 0) ? dx[1] * dx[2] : (j == 1 ? dx[0] * dx[2] : dx[0] * dx[1]);
+            (void)P2;
+            if (j == dax) {
+                int s[3] = {q[0], q[1], q[2]};
+                s[dax] = si;
+                R t = lam;
+                if (j == 0) { t = t * dx[1]; t = t * dx[2]; }
+                if (j == 1) { t = t * dx[0]; t = t * dx[2]; }
+                if (j == 2) { t = t * dx[0]; t = t * dx[1]; }
+                t = t * atp(FU + i, lvl, s);
+                sum = first ? t : sum + t;
+                first = false;
+                continue;
+            }
+            int qp[3] = {q[0], q[1], q[2]}, qm[3] = {q[0], q[1], q[2]};
+            qp[dax] = b; qm[dax] = b; qm[j] -= 1;
+            R c = lam;
+            if (j == 0) { c = c * dx[1]; c = c * dx[2]; }
+            if (j == 1) { c = c * dx[0]; c = c * dx[2]; }
+            if (j == 2) { c = c * dx[0]; c = c * dx[1]; }
+            /* side 0: +c*V[x_j] - c*V[x_j-1]; side 1: -c*V[x_j] + c*V[x_j-1] */
+            R tp = c * atp(FU + j, lvl, qp), tm = c * atp(FU + j, lvl, qm);
+            if (side == 0) {
+                sum = first ? tp : sum + tp;
+                sum = sum - tm;
+            } else {
+                sum = first ? -tp : sum - tp;
+                sum = sum + tm;
+            }
+            first = false;
+        }
+        int s[3] = {q[0], q[1], q[2]};
+        s[dax] = si;
+        R t = (R)2 * mu;
+        if (dax == 0) { t = t * dx[1]; t = t * dx[2]; }
+        if (dax == 1) { t = t * dx[0]; t = t * dx[2]; }
+        if (dax == 2) { t = t * dx[0]; t = t * dx[1]; }
+        sum = sum + t * atp(FU + i, lvl, s);
+        R den = Pexcl(dax) * (lam + (R)2 * mu);
+        q[dax] = gi;
+        atp(FU + i, lvl, q) = sum / den;
+    }
+
+    /* snapshot spellings: non-staggered V_i ghost, terms sorted by index with
+     * the dx_i*(...) group placed by its coefficient symbol */
+    void vel_levander_snapshot_other(int i, int dax, int side, int b, int q[3], int lvl) {
+        int gi = side == 0 ? b - 1 : b + 1, fi = side == 0 ? b + 1 : b - 1;
+        int qb[3] = {q[0], q[1], q[2]}, qf[3] = {q[0], q[1], q[2]};
+        qb[dax] = b; qf[dax] = fi;
+        R grp;
+        if (side == 0) { grp = (R)2 * atp(FU + i, lvl, qb); grp = grp - atp(FU + i, lvl, qf); }
+        else { grp = -atp(FU + i, lvl, qf); grp = grp + (R)2 * atp(FU + i, lvl, qb); }
+        grp = dx[i] * grp;
+        /* V_d terms in lexicographic array-index order (x, y, z) */
+        struct T { int idx[3]; int sign; };
+        T t[4];
+        int k = 0;
+        for (int pb = 0; pb < 2; ++pb)
+            for (int pi = 0; pi < 2; ++pi) {
+                T& e = t[k++];
+                e.idx[0] = q[0]; e.idx[1] = q[1]; e.idx[2] = q[2];
+                e.idx[dax] = b - 1 + pb; e.idx[i] += pi;
+                /* (b-1,x_i): -, (b-1,x_i+1): +, (b,x_i): +, (b,x_i+1): - */
+                e.sign = (pb == pi) ? -1 : 1;
+            }
+        std::sort(t, t + 4, [](const T& a, const T& c) {
+            for (int r = 0; r < 3; ++r) if (a.idx[r] != c.idx[r]) return a.idx[r] < c.idx[r];
+            return false;
+        });
+        R sum;
+        bool first = true;
+        if (i < dax) { sum = grp; first = false; }
+        for (int r = 0; r < 4; ++r) {
+            R v = dx[dax] * atp(FU + dax, lvl, t[r].idx);
+            if (first) { sum = t[r].sign > 0 ? v : -v; first = false; }
+            else sum = t[r].sign > 0 ? sum + v : sum - v;
+        }
+        if (i > dax) sum = sum + grp;
+        q[dax] = gi;
+        atp(FU + i, lvl, q) = sum / dx[i];
+    }
+
+    /* StaggeredGrid.velocity_bc order (staggeredgrid.py:815-864): per axis,
+     * the field staggered along it first, then the others in field order. */
+    void velocity_bc(int lvl) {
+        for (int dax = 0; dax < 3; ++dax) {
+            int seq[3] = {dax, -1, -1}, n = 1;
+            for (int f = 0; f < 3; ++f) if (f != dax) seq[n++] = f;
+            for (int s = 0; s < 3; ++s)
+                for (int side = 0; side < 2; ++side) {
+                    if (levander) vel_levander(FU + seq[s], dax, side, lvl);
+                    else vel_robertsson(FU + seq[s], dax, side, lvl);
+                }
+        }
+    }
+
+    /* point source + receivers (tests/src/test_ref_iso_elastic.cpp:227-290):
+     * record after the velocity update, then Tnn -= src/3 */
+    void receivers_and_source(int ti, int t1) {
+#pragma omp single
+        {
+            for (int r = 0; r < P.nrec; ++r) {
+                const int* q = P.rec_idx + 3 * r;
+                float* o = P.rec_out + ((size_t)ti * P.nrec + r) * 4;
+                o[0] = (float)atp(FU, t1, q);
+                o[1] = (float)atp(FV, t1, q);
+                o[2] = (float)atp(FW, t1, q);
+                o[3] = (float)((atp(TXX, t1, q) + atp(TYY, t1, q) + atp(TZZ, t1, q)) / (R)3);
+            }
+            if (P.src_on) {
+                R s = (R)P.src_wavelet[ti] / (R)3;
+                atp(TXX, t1, P.src_idx) -= s;
+                atp(TYY, t1, P.src_idx) -= s;
+                atp(TZZ, t1, P.src_idx) -= s;
+            }
+        }
+    }
+
+    double run_loop() {
+        double t_start = 0, t_end = 0;
+        int t0 = 0, t1 = 0;
+#pragma omp parallel
+        {
+            initialise();
+            stress_bc(true, 0, 0);
+            velocity_bc(0);
+#pragma omp barrier
+#pragma omp single
+            t_start = omp_get_wtime();
+            for (int ti = 0; ti < P.ntsteps; ++ti) {
+#pragma omp single
+                {
+                    t0 = ti % 2;
+                    t1 = (t0 + 1) % 2;
+                }
+                if (P.snapshot) stress_snapshot(t0, t1);
+                else stress_current(t0, t1);
+                stress_bc(false, t0, t1);
+                if (P.snapshot) velocity_snapshot(t0, t1);
+                else velocity_current(t0, t1);
+                velocity_bc(t1);
+                if (P.nrec > 0 || P.src_on) receivers_and_source(ti, t1);
+            }
+#pragma omp single
+            t_end = omp_get_wtime();
+        }
+        return t_end - t_start;
+    }
+
+    /* L2 norms (staggeredgrid.py:892-945): reference style accumulates
+     * pow(F - sol, 2.0) serially into real_t; the f64 variant accumulates in
+     * double with the analytic solution evaluated in double. */
+    void norms(double l2_f64[9], double l2_ref[9]) {
+        int lvl = P.ntsteps % 2;
+        double volume = dxd[0] * dxd[1] * dxd[2];
+        for (int f = 0; f < NF; ++f) {
+            int hi[3];
+            for (int a = 0; a < 3; ++a) hi[a] = d[a] - M - (STAG[f][a] ? 1 : 0);
+            double tn = dtd * P.ntsteps + (TIME_STAG[f] ? dtd / 2.0 : 0.0);
+            R acc_ref = (R)0;
+            double acc = 0.0;
+            for (int x = M; x < hi[0]; ++x)
+                for (int y = M; y < hi[1]; ++y)
+                    for (int z = M; z < hi[2]; ++z) {
+                        R X = coord(0, x, STAG[f][0]), Y = coord(1, y, STAG[f][1]), Z = coord(2, z, STAG[f][2]);
+                        double v = (double)at(f, lvl, x, y, z);
+                        double sref = P.snapshot ? eigen_snapshot(f, X, Y, Z, tn) : eigen_current_ref(f, X, Y, Z, tn);
+                        double e = v - sref;
+                        acc_ref = (R)((double)acc_ref + std::pow(e, 2.0));
+                        double mu, beta;
+                        media_at(ix(x, y, z), mu, beta);
+                        double s = eigen(f, X, Y, Z, tn, mu, beta);
+                        acc += (v - s) * (v - s);
+                    }
+            double vol_lit = P.snapshot ? volume : (double)f32(volume);
+            if (P.snapshot) l2_ref[f] = std::pow((double)acc_ref * vol_lit, 0.5);
+            else l2_ref[f] = std::pow((double)(R)((double)acc_ref * vol_lit), 0.5);
+            l2_f64[f] = std::sqrt(acc * volume);
+        }
+    }
+
+    void copy_out(void* out[9]) {
+        int lvl = P.ntsteps % 2;
+        for (int f = 0; f < NF; ++f)
+            if (out[f]) std::memcpy(out[f], F[f].data() + (size_t)lvl * vol, vol * sizeof(R));
+    }
+};
+
+template <typename R>
+int run_typed(const OracleParams* p, void* out[9], double l2_f64[9], double l2_ref[9], double timing[2]) {
+    double t_all = omp_get_wtime();
+    Sim<R> s(*p);
+    double loop = s.run_loop();
+    if (l2_f64 && l2_ref) s.norms(l2_f64, l2_ref);
+    if (out) s.copy_out(out);
+    if (timing) {
+        timing[0] = loop;
+        timing[1] = omp_get_wtime() - t_all;
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int oracle_run(const OracleParams* p, void* fields_out[9], double l2_f64[9],
+                          double l2_ref[9], double timing[2]) {
+    if (!p) return 1;
+    if (p->order < 2 || p->order > 8 || (p->order % 2)) return 2;
+    if (p->snapshot && p->order != 4) return 3;
+    if (p->media_arrays && (!p->rho_arr || !p->vp_arr || !p->vs_arr)) return 4;
+    if (p->nthreads > 0) omp_set_num_threads(p->nthreads);
+    if (p->real_bytes == 8) return run_typed<double>(p, fields_out, l2_f64, l2_ref, timing);
+    return run_typed<float>(p, fields_out, l2_f64, l2_ref, timing);
+}
+
+extern "C" int oracle_fd_weights(int half_order, double* w) {
+    if (half_order < 1 || half_order > 4) return 1;
+    for (int k = 0; k < half_order; ++k) w[k] = FDW[half_order][k];
+    return 0;
+}

@@ -1,0 +1,131 @@
+"""Pins the oracle (oracle/elastic_oracle.cpp) before it is trusted.
+
+* known answer: the snapshot convention reproduces the L2 vector printed by
+  the reference's own generated propagator benchmarks/test3d.cpp
+* FD weights: brute-force Taylor/Vandermonde solve and the notebook output
+* invariants: cyclic x->y->z symmetry, linearity in the initial state,
+  shear stresses stay at discretisation-noise level, convergence rates
+"""
+import fractions
+
+import numpy as np
+import pytest
+
+import oracle
+
+
+def vandermonde_weights(M):
+    """First-derivative weights on the half-integer stencil +-(k+1/2), exact rationals."""
+    pts = [fractions.Fraction(2 * k + 1, 2) for k in range(M)]
+    # odd moments: sum_k w_k * 2 * p_k^(2r+1)/(2r+1)! = delta_{r0}
+    A = [[2 * p ** (2 * r + 1) for p in pts] for r in range(M)]
+    b = [fractions.Fraction(1)] + [fractions.Fraction(0)] * (M - 1)
+    n = M
+    for c in range(n):
+        piv = next(r for r in range(c, n) if A[r][c] != 0)
+        A[c], A[piv] = A[piv], A[c]
+        b[c], b[piv] = b[piv], b[c]
+        for r in range(n):
+            if r != c and A[r][c] != 0:
+                f = A[r][c] / A[c][c]
+                A[r] = [a - f * e for a, e in zip(A[r], A[c])]
+                b[r] -= f * b[c]
+    return [b[i] / A[i][i] for i in range(n)]
+
+
+@pytest.mark.parametrize("M", [1, 2, 3, 4])
+def test_fd_weights_brute_force(M):
+    w = oracle.fd_weights(M)
+    exact = vandermonde_weights(M)
+    assert np.allclose(w, [float(v) for v in exact], rtol=0, atol=1e-15)
+
+
+def test_fd_weights_notebook(fixtures):
+    # tests/eigenwave3d.ipynb cell 11: [1/24, -9/8, 9/8, -1/24]/dx1 on x-3/2..x+3/2
+    assert "9*Txx[t, x + 1/2, y, z]/(8*dx1)" in fixtures["fd_weights_4th"]["text"]
+    assert "Txx[t, x + 3/2, y, z]/(24*dx1)" in fixtures["fd_weights_4th"]["text"]
+    assert oracle.fd_weights(2) == [9 / 8, -1 / 24]
+
+
+def test_fd_weights_8th_notebook(fixtures):
+    # tests/eigenwave3d-higher_order.ipynb cell 31: U[t1] coefficients of Txx at
+    # dt/dx = 0.2, beta = 1 -> 0.2 * w_k on Txx[x+1+k] - Txx[x-k]
+    terms = fixtures["velocity_8th_U_terms"]["terms"]
+    txx = {t[2]: t[0] for t in terms if t[1] == "Txx"}
+    w = oracle.fd_weights(4)
+    for k in range(4):
+        plus = "x" if k == -1 else ("x + %d" % (k + 1))
+        minus = "x" if k == 0 else "x - %d" % k
+        assert txx[plus] == pytest.approx(0.2 * w[k], rel=1e-13)
+        assert txx[minus] == pytest.approx(-0.2 * w[k], rel=1e-13)
+
+
+@pytest.mark.slow
+def test_known_answer_test3d(fixtures):
+    """Snapshot convention == benchmarks/test3d.cpp's printed L2 vector."""
+    ref = fixtures["test3d_known_answer"]["l2"]
+    r = oracle.snapshot_test3d()
+    for name, got in zip(oracle.FIELDS, r["l2_ref"]):
+        # printed with %.10f: agreement to the last printed digit
+        assert abs(got - ref[name]) <= 6e-11, (name, got, ref[name])
+
+
+def _small(order=4, n=12, steps=20, double=True, **kw):
+    dx = 1.0 / n
+    return oracle.run(order=order, n=(n, n, n), ntsteps=steps, dt=dx / 6.0, double=double,
+                      rho=1.0, lam=1.0, mu=1.0, **kw)
+
+
+@pytest.mark.parametrize("order,snapshot", [(2, False), (8, False), (4, True)])
+def test_cyclic_symmetry(order, snapshot):
+    """A cube with equal spacing: rotating x->y->z maps U->V->W, Txx->Tyy->Tzz.
+    The BC sequence is not symmetric (later planes overwrite earlier edges),
+    so norms agree only to the level the reference's own known answer shows."""
+    r = _small(order=order, n=24 if snapshot else 12, steps=40, snapshot=snapshot)
+    l2 = r["l2"]
+    vel, nrm = l2[0:3], l2[3:6]
+    assert max(vel) / min(vel) - 1 < 5e-2
+    assert max(nrm) / min(nrm) - 1 < 5e-2
+
+
+def test_levander_recompute_asymmetry_is_an_edge_effect():
+    """With the Levander normal-stress recompute the x-plane antisymmetry of
+    Txx reads values the y/z recompute later overwrites, while Tzz's reads
+    already-recomputed ones (staggeredgrid.py:767-812 order).  The resulting
+    U/V/W asymmetry lives on edges: it must shrink as the grid is refined."""
+    asym = []
+    for n in (12, 24):
+        r = _small(order=4, n=n, steps=n * 3)
+        v = r["l2"][0:3]
+        asym.append(max(v) / min(v) - 1)
+    assert asym[1] < 0.75 * asym[0], asym
+
+
+def test_linearity_in_initial_state():
+    """The scheme is linear: zero initial state with zero source stays exactly zero."""
+    r = _small(order=4, ic_mode=1)
+    for f in r["fields"]:
+        assert not np.any(f)
+
+
+@pytest.mark.parametrize("order,expect", [(2, 1.8), (4, 1.8)])
+def test_convergence_rate(order, expect):
+    """Halving dx with dt = dx/6 (2nd order in time) must cut the velocity L2
+    error by ~2^2 at least: the leapfrog time error dominates at these sizes."""
+    errs = []
+    for n in (8, 16):
+        dx = 1.0 / n
+        steps = n // 2
+        r = oracle.run(order=order, n=(n, n, n), ntsteps=steps * 6, dt=dx / 6.0, double=True,
+                       rho=1.0, lam=1.0, mu=1.0, want_fields=False)
+        errs.append(r["l2"][0])
+    rate = np.log2(errs[0] / errs[1])
+    assert rate > expect, (errs, rate)
+
+
+def test_float_vs_double_close():
+    a = _small(order=4, double=True)
+    b = _small(order=4, double=False)
+    for fa, fb in zip(a["fields"], b["fields"]):
+        scale = np.sqrt(np.mean(fa.astype(np.float64) ** 2)) + 1e-30
+        assert np.sqrt(np.mean((fa - fb.astype(np.float64)) ** 2)) / scale < 1e-5
