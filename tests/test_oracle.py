@@ -94,7 +94,7 @@ def test_levander_recompute_asymmetry_is_an_edge_effect():
     already-recomputed ones (staggeredgrid.py:767-812 order).  The resulting
     U/V/W asymmetry lives on edges: it must shrink as the grid is refined."""
     asym = []
-    for n in (12, 24):
+    for n in (24, 48):
         r = _small(order=4, n=n, steps=n * 3)
         v = r["l2"][0:3]
         asym.append(max(v) / min(v) - 1)
