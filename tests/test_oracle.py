@@ -129,6 +129,8 @@ def test_convergence_rate(order, expect):
 def test_float_vs_double_close():
     a = _small(order=4, double=True)
     b = _small(order=4, double=False)
+    # shear stresses are discretisation noise (analytic value 0): scale every
+    # field by the velocity magnitude, as the L2 error itself is
+    scale = np.sqrt(np.mean(a["fields"][0] ** 2))
     for fa, fb in zip(a["fields"], b["fields"]):
-        scale = np.sqrt(np.mean(fa.astype(np.float64) ** 2)) + 1e-30
         assert np.sqrt(np.mean((fa - fb.astype(np.float64)) ** 2)) / scale < 1e-5
