@@ -1,0 +1,221 @@
+/*
+ * opesci-fd MI355X backend -- public C ABI of libopesci_hip.so.
+ *
+ * The reference compiles one C++ translation unit per model and loads it with
+ * ctypes (opesci/grid.py:85-156); that TU exports opesci_execute,
+ * opesci_convergence and opesci_free over the structs OpesciGrid,
+ * OpesciProfiling and OpesciConvergence (opesci/templates/regular3d_tmpl.py:
+ * 19-23, 41-59, 99-134).  Here the compute lives in one prebuilt gfx950
+ * library and the per-model part the generator used to bake into the TU
+ * (constants, folded stencil coefficients, the boundary sequence, the
+ * analytic initial state) arrives as data: an OpesciModelDesc produced by the
+ * Python generator (opesci/plan.py).  Plain pointers and sizes only.
+ *
+ * Field order everywhere: U, V, W, Txx, Tyy, Tzz, Txy, Tyz, Txz
+ * (StaggeredGrid.fields = vfields + sfields, opesci/staggeredgrid.py:69-71).
+ * Host arrays use the reference layout real_t[tp=2][dim1][dim2][dim3],
+ * dim3 contiguous (opesci/regulargrid.py:466-496).
+ *
+ * Every entry point returns 0 on success and a nonzero OPESCI_E* code
+ * otherwise (the reference always returns 0); opesci_last_error() gives the
+ * message.  Calls on one context must not overlap; different contexts are
+ * independent.
+ */
+#ifndef OPESCI_HIP_H
+#define OPESCI_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OPESCI_ABI_VERSION 1
+#define OPESCI_NFIELDS 9
+#define OPESCI_MAX_HALF_ORDER 4
+
+enum {
+    OPESCI_OK = 0,
+    OPESCI_EINVAL = 1,      /* malformed model description or argument */
+    OPESCI_EHIP = 2,        /* HIP runtime error */
+    OPESCI_ENOMEM = 3,      /* device or host allocation failed */
+    OPESCI_ESTATE = 4,      /* call out of order (e.g. convergence before execute) */
+    OPESCI_EIO = 5,         /* media file missing or short */
+    OPESCI_EUNSUPPORTED = 6 /* configuration without a kernel instantiation */
+};
+
+enum { OPESCI_REAL_F32 = 4, OPESCI_REAL_F64 = 8 };
+enum { OPESCI_STORE_NATIVE = 0, OPESCI_STORE_F16 = 1, OPESCI_STORE_BF16 = 2 };
+
+/* Media identifiers (constant values or arrays; opesci/staggeredgrid.py:234-282) */
+enum {
+    OPESCI_M_RHO = 0, OPESCI_M_VP, OPESCI_M_VS, OPESCI_M_BETA, OPESCI_M_LAMBDA, OPESCI_M_MU,
+    OPESCI_M_BETA1, OPESCI_M_BETA2, OPESCI_M_BETA3, OPESCI_M_MU12, OPESCI_M_MU13, OPESCI_M_MU23,
+    OPESCI_M_COUNT
+};
+
+/* Expression bytecode (analytic solutions, media-dependent boundary
+ * coefficients).  Evaluated in double on the device. */
+enum {
+    OPESCI_OP_CONST = 0, /* push val */
+    OPESCI_OP_COORD,     /* push coordinate of axis a[0] (real_t-rounded, field staggering) */
+    OPESCI_OP_TIME,      /* push t */
+    OPESCI_OP_MEDIA,     /* push media a[0] at anchor + (a[1],a[2],a[3]) */
+    OPESCI_OP_ADD, OPESCI_OP_SUB, OPESCI_OP_MUL, OPESCI_OP_DIV, OPESCI_OP_NEG,
+    OPESCI_OP_POWC,      /* top = pow(top, val) */
+    OPESCI_OP_SQRT, OPESCI_OP_SIN, OPESCI_OP_COS, OPESCI_OP_EXP, OPESCI_OP_LOG,
+    OPESCI_OP_ROUNDF     /* top = (double)(float)top */
+};
+typedef struct OpesciExprOp {
+    int32_t op;
+    int32_t a[4];
+    int32_t pad;
+    double val;
+} OpesciExprOp;
+
+/* One boundary assignment dst[plane][..][..] = sum_i coef_i * src_i over a
+ * plane loop (StaggeredGrid.stress_bc_getter / velocity_bc,
+ * opesci/staggeredgrid.py:754-864).  Statements carry the launch index the
+ * generator's hazard analysis gave them: statements with the same sched in
+ * one phase touch disjoint cells and run in one launch; ascending sched
+ * reproduces the reference's sequential order exactly. */
+typedef struct OpesciBcTerm {
+    int32_t field;   /* source field */
+    int32_t level;   /* 0: t0, 1: t1 */
+    int32_t plane;   /* absolute index along the statement's axis */
+    int32_t off[2];  /* offsets along the two other axes (ascending axis order) */
+    int32_t prog;    /* -1: coefficient is coef; else offset of a media program in bc_code */
+    int32_t prog_len;
+    int32_t pad;
+    double coef;
+} OpesciBcTerm;
+
+typedef struct OpesciBcStmt {
+    int32_t field;   /* destination field */
+    int32_t level;   /* 0: t0, 1: t1 */
+    int32_t axis;    /* plane normal: 0 x, 1 y, 2 z */
+    int32_t plane;   /* absolute destination index along axis */
+    int32_t lo[2];   /* loop range over the other two axes */
+    int32_t hi[2];
+    int32_t term0;   /* first term in the term table */
+    int32_t nterms;
+    int32_t phase;   /* 0: init stress, 1: init velocity, 2: step stress, 3: step velocity */
+    int32_t sched;   /* launch index within the phase */
+} OpesciBcStmt;
+
+/* Everything the generator bakes into the reference's emitted TU. */
+typedef struct OpesciModelDesc {
+    int32_t abi_version;
+    int32_t dims[3];        /* dim1..dim3 incl. ghosts: grid_size + 1 + 2m */
+    int32_t space_order;    /* 2M */
+    int32_t ntsteps;
+    int32_t real_bytes;     /* OPESCI_REAL_F32 / OPESCI_REAL_F64 */
+    int32_t storage;        /* OPESCI_STORE_* (16-bit storage needs real_bytes 4) */
+    int32_t media_arrays;   /* 0 constant media, 1 arrays from rho/vp/vs */
+    int32_t flags;          /* reserved, 0 */
+    double dt;
+    double spacing[3];
+    double media_const[OPESCI_M_COUNT];  /* constant media values (float-rounded as the reference declares them) */
+    /* interior update coefficients, float-rounded like the reference's F literals:
+     *   normal stress i:  T_ii += sum_j sum_k cn[i][j][k] * m * (V_j[+k] - V_j[-1-k]) along j,
+     *                      m = 1 (constant media) or lambda(+2mu for j == i) (arrays)
+     *   shear pair s (xy,yz,xz), group g: T_s += cs[s][g][k] * m * (V[+1+k] - V[-k])
+     *   velocity i:       V_i += sum_j cv[i][j][k] * m * (T[a+k] - T[a-1-k]) along j */
+    double cn[3][3][OPESCI_MAX_HALF_ORDER];
+    double cs[3][2][OPESCI_MAX_HALF_ORDER];
+    double cv[3][3][OPESCI_MAX_HALF_ORDER];
+    /* media arrays (media_arrays = 1): dim1*dim2*dim3 float32 each */
+    const float* rho;
+    const float* vp;
+    const float* vs;
+    /* boundary program */
+    int32_t nstmts;
+    int32_t nterms;
+    int32_t ncode;
+    int32_t pad0;
+    const OpesciBcStmt* stmts;
+    const OpesciBcTerm* terms;
+    const OpesciExprOp* bc_code;
+    /* analytic solution per field: code[sol_off[f] .. sol_off[f]+sol_len[f]) */
+    int32_t sol_off[OPESCI_NFIELDS];
+    int32_t sol_len[OPESCI_NFIELDS];
+    int32_t nsol;
+    int32_t pad1;
+    const OpesciExprOp* sol_code;
+    /* point source (Ricker etc.) into Txx/Tyy/Tzz and receivers
+     * (tests/src/test_ref_iso_elastic.cpp:227-290) */
+    int32_t src_on;
+    int32_t src_idx[3];
+    const float* src_wavelet;   /* ntsteps values; Tnn -= w[t]/3 after the velocity step */
+    int32_t nrec;
+    int32_t pad2;
+    const int32_t* rec_idx;     /* 3*nrec grid indices */
+    /* launch tuning (0 = library default) */
+    int32_t tile_y;
+    int32_t x_chunk;
+    int32_t kernel_variant;     /* 0 default (LDS/register-queue), 1 plain gather */
+    int32_t use_graph;          /* capture the time loop in a hipGraph */
+} OpesciModelDesc;
+
+/* Reference ABI structs.  Field pointers are real_t* (float or double). */
+typedef struct OpesciGrid {
+    void* U; void* V; void* W;
+    void* Txx; void* Tyy; void* Tzz;
+    void* Txy; void* Tyz; void* Txz;
+} OpesciGrid;
+
+typedef struct OpesciProfiling {
+    float g_rtime;   /* seconds of the device time loop */
+    float g_ptime;   /* same (there is no separate process time on the device) */
+    float g_mflops;  /* algorithmic MFlop/s (reference op count) */
+} OpesciProfiling;
+
+typedef struct OpesciContext OpesciContext;
+
+/* --- reference-named entry points ---------------------------------------
+ * opesci_model_load binds a model to the process-default context, the way
+ * loading a generated .so did (opesci/grid.py:35-42).  Then:
+ * opesci_execute      <- generated TU, opesci/templates/regular3d_tmpl.py:102-117
+ *   allocates the fields on the device, applies the initial condition and the
+ *   initial boundary conditions, runs ntsteps on the GPU, and hands back host
+ *   arrays real_t[2][dim1][dim2][dim3] (both time levels) through grid, which
+ *   the caller releases with opesci_free.
+ * opesci_convergence  <- regular3d_tmpl.py:164-170, staggeredgrid.py:892-945
+ *   conv points to 9 real_t: sqrt(sum (F - F_exact)^2 dx dy dz) per field,
+ *   accumulated in double on the device.
+ * opesci_free         <- regular3d_tmpl.py:172-176 */
+int opesci_model_load(const OpesciModelDesc* desc);
+int opesci_execute(OpesciGrid* grid, OpesciProfiling* profiling);
+int opesci_convergence(OpesciGrid* grid, void* conv);
+int opesci_free(OpesciGrid* grid);
+
+/* --- context API (no reference counterpart: the reference has one model per
+ * generated library; these expose the same steps per handle) ------------- */
+int opesci_ctx_create(const OpesciModelDesc* desc, OpesciContext** out);
+int opesci_ctx_destroy(OpesciContext* ctx);
+/* initial condition + initial boundary conditions (staggered3d_tmpl.py:21-22) */
+int opesci_ctx_init(OpesciContext* ctx);
+/* advance nsteps on the device (staggered3d_tmpl.py:40-58); seconds = device time */
+int opesci_ctx_run(OpesciContext* ctx, int32_t nsteps, double* seconds);
+/* per-field L2 vs the analytic solution at the current time (double) */
+int opesci_ctx_l2(OpesciContext* ctx, double out[OPESCI_NFIELDS]);
+/* copy one field, one time level (0/1, or -1 = latest) to/from a dense host array real_t[dim1][dim2][dim3] */
+int opesci_ctx_download(OpesciContext* ctx, int32_t field, int32_t level, void* host);
+int opesci_ctx_upload(OpesciContext* ctx, int32_t field, int32_t level, const void* host);
+/* receiver traces recorded so far: float[nsteps_done][nrec][4] (U, V, W, pressure) */
+int opesci_ctx_receivers(OpesciContext* ctx, float* host, int64_t capacity_floats);
+/* steps completed since init */
+int opesci_ctx_steps(OpesciContext* ctx, int32_t* steps);
+/* average device time per launch of each hot kernel over the last run
+ * (HIP events on the context stream): [0] stress, [1] velocity, [2] all BC launches of a step */
+int opesci_ctx_kernel_times(OpesciContext* ctx, int32_t nsteps, double out_ms[3]);
+
+const char* opesci_last_error(void);
+int opesci_abi_version(void);
+/* device name and architecture of device 0 ("" if none) */
+int opesci_device_info(char* name, int32_t name_len, char* arch, int32_t arch_len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OPESCI_HIP_H */

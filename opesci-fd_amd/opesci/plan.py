@@ -1,0 +1,478 @@
+"""Lowering of a StaggeredGrid model to the data the gfx950 library runs.
+
+The reference prints C++ loop nests (opesci/regulargrid.py:566-619,
+opesci/staggeredgrid.py:728-864); here the same derived expressions are
+lowered to:
+
+* folded stencil coefficients for the hand-written interior kernels, after
+  checking that every derived kernel has exactly the velocity-stress stencil
+  structure those kernels implement (anything else raises);
+* a boundary program: one statement per ghost-cell equation, in the
+  reference's emission order (stress_bc_getter / velocity_bc /
+  initialise_bc), each tagged with a launch index from a read/write hazard
+  analysis so that statements touching disjoint cells share a launch while
+  the result stays identical to the sequential loops;
+* device bytecode for the analytic solutions (initial state, L2 norms).
+
+Coefficients follow the reference's printed literals: every folded Float is
+printed with 15 significant digits and an F suffix (codeprinter.py:42-63),
+so the values are float-rounded even in the double build.
+"""
+import os
+
+import numpy as np
+from sympy import Add, Float, Indexed, Integer, Mul, Symbol, expand
+
+from . import exprvm
+
+SLOT_NAMES = ('U', 'V', 'W', 'Txx', 'Tyy', 'Tzz', 'Txy', 'Tyz', 'Txz')
+STAG = ((1, 0, 0), (0, 1, 0), (0, 0, 1), (0, 0, 0), (0, 0, 0), (0, 0, 0),
+        (1, 1, 0), (0, 1, 1), (1, 0, 1))
+SHEAR_PAIRS = ((0, 1), (1, 2), (0, 2))
+SHEAR_MEDIA = ('mu12', 'mu23', 'mu13')
+STORAGE_CODE = {'native': 0, 'fp32': 0, 'fp64': 0, 'fp16': 1, 'bf16': 2}
+
+STMT_DTYPE = np.dtype([('field', '<i4'), ('level', '<i4'), ('axis', '<i4'), ('plane', '<i4'),
+                       ('lo', '<i4', (2,)), ('hi', '<i4', (2,)), ('term0', '<i4'),
+                       ('nterms', '<i4'), ('phase', '<i4'), ('sched', '<i4')])
+TERM_DTYPE = np.dtype([('field', '<i4'), ('level', '<i4'), ('plane', '<i4'), ('off', '<i4', (2,)),
+                       ('prog', '<i4'), ('prog_len', '<i4'), ('pad', '<i4'), ('coef', '<f8')])
+
+
+class PlanError(NotImplementedError):
+    pass
+
+
+def f_lit(v):
+    """Value of a folded constant as the reference prints it: %.15g + F suffix."""
+    return float(np.float32(float('%.15g' % float(v))))
+
+
+def shear_slot(i, j):
+    i, j = min(i, j), max(i, j)
+    return {(0, 1): 6, (1, 2): 7, (0, 2): 8}[(i, j)]
+
+
+class ModelPlan(object):
+    """Everything libopesci_hip needs for one model (see OpesciModelDesc)."""
+
+    def __repr__(self):
+        return '<ModelPlan order=%d dims=%s steps=%d real=%d storage=%s media=%s bc=%d stmts/%d launches>' % (
+            self.order, self.dims, self.ntsteps, self.real_bytes, self.storage,
+            'arrays' if self.media_arrays else 'const', len(self.stmts), self.nlaunches_step)
+
+
+def _canonical_slots(grid):
+    """Map the grid's fields onto U,V,W,Txx,Tyy,Tzz,Txy,Tyz,Txz by direction."""
+    slots = [None] * 9
+    for v in grid.vfields:
+        slots[v.direction - 1] = v
+    for s in grid.sfields:
+        a, b = sorted(s.direction)
+        slots[3 + a - 1 if a == b else shear_slot(a - 1, b - 1)] = s
+    if any(s is None for s in slots):
+        raise PlanError('the elastic backend needs 3 velocity and 6 stress fields')
+    if [f for f in slots] != list(grid.fields):
+        raise PlanError('fields must be given as velocity (x,y,z) then stress '
+                        '(xx,yy,zz,xy,yz,xz) components, the order of StaggeredGrid.fields')
+    return slots
+
+
+def _time_level(ind, grid):
+    s = str(ind)
+    if s in ('_t0', str(grid.t)):
+        return 0
+    if s in ('_t1', str(grid.t + 1)):
+        return 1
+    raise PlanError('unexpected time index %s' % ind)
+
+
+def _split(term, field_names):
+    """term = number * factors * field -> (number, [other factors], field Indexed)."""
+    num = Integer(1)
+    other = []
+    fld = None
+    for f in Mul.make_args(term):
+        if f.is_Number:
+            num *= f
+        elif isinstance(f, Indexed) and str(f.base.label) in field_names:
+            if fld is not None:
+                raise PlanError('non-linear term %s' % term)
+            fld = f
+        else:
+            other.append(f)
+    return num, other, fld
+
+
+# ---------------------------------------------------------------- interior
+def _extract_interior(grid, slots):
+    names = {str(f.label): k for k, f in enumerate(slots)}
+    M = grid.order[1] // 2
+    idx = grid.index
+    cn = np.zeros((3, 3, 4))
+    cs = np.zeros((3, 2, 4))
+    cv = np.zeros((3, 3, 4))
+    for slot, field in enumerate(slots):
+        if field.kernel_aligned is None:
+            raise PlanError('solve_fd() has not been called')
+        k = grid.kernel_sympy(field)
+        table = {}
+        for term in Add.make_args(expand(k)):
+            num, other, fld = _split(term, names)
+            if fld is None:
+                raise PlanError('%s kernel has a term without a field: %s' % (field, term))
+            src = names[str(fld.base.label)]
+            lvl = _time_level(fld.indices[0], grid)
+            off = tuple(int(fld.indices[1 + a] - idx[a]) for a in range(3))
+            media = None
+            if other:
+                if len(other) != 1 or not isinstance(other[0], Indexed):
+                    raise PlanError('unsupported coefficient %s in %s kernel' % (other, field))
+                m = other[0]
+                if any(m.indices[a] != idx[a] for a in range(3)):
+                    raise PlanError('media must be read at the update point: %s' % m)
+                media = str(m.base.label)
+            table.setdefault((src, lvl, off), {})
+            table[(src, lvl, off)][media] = table[(src, lvl, off)].get(media, 0) + num
+        if table.pop((slot, 0, (0, 0, 0)), None) != {None: 1}:
+            raise PlanError('%s kernel must be F[t1] = F[t0] + ...' % field)
+
+        def take(src, lvl, axis, a, kk, want):
+            op, om = [0, 0, 0], [0, 0, 0]
+            op[axis] = a + kk
+            om[axis] = a - 1 - kk
+            cp = table.pop((src, lvl, tuple(op)), None)
+            cm = table.pop((src, lvl, tuple(om)), None)
+            if cp is None or cm is None:
+                raise PlanError('%s kernel misses the %s stencil pair %d along axis %d'
+                                % (field, SLOT_NAMES[src], kk, axis))
+            if set(cp) != set(want) or set(cm) != set(want):
+                raise PlanError('%s kernel: media factors %s, expected %s' % (field, list(cp), want))
+            for w in want:
+                if f_lit(cp[w]) != -f_lit(cm[w]):
+                    raise PlanError('%s kernel stencil is not antisymmetric' % field)
+            return cp
+
+        if slot < 3:
+            i = slot
+            for j in range(3):
+                src = 3 + i if j == i else shear_slot(i, j)
+                a = 1 if j == i else 0
+                for kk in range(M):
+                    want = [None] if not grid.read else ['beta%d' % (i + 1)]
+                    c = take(src, 1, j, a, kk, want)
+                    cv[i, j, kk] = f_lit(c[want[0]])
+        elif slot < 6:
+            i = slot - 3
+            for j in range(3):
+                for kk in range(M):
+                    if not grid.read:
+                        c = take(j, 0, j, 0, kk, [None])
+                        cn[i, j, kk] = f_lit(c[None])
+                    else:
+                        want = ['lambda', 'mu'] if j == i else ['lambda']
+                        c = take(j, 0, j, 0, kk, want)
+                        if j == i and f_lit(c['mu']) != 2 * f_lit(c['lambda']):
+                            raise PlanError('%s kernel: expected (lambda + 2 mu) on its own axis' % field)
+                        cn[i, j, kk] = f_lit(c['lambda'])
+        else:
+            s = slot - 6
+            ia, ja = SHEAR_PAIRS[s]
+            for g, (src, axis) in enumerate(((ia, ja), (ja, ia))):
+                for kk in range(M):
+                    want = [None] if not grid.read else [SHEAR_MEDIA[s]]
+                    c = take(src, 0, axis, 1, kk, want)
+                    cs[s, g, kk] = f_lit(c[want[0]])
+        if table:
+            raise PlanError('%s kernel has terms outside the elastic stencil: %s'
+                            % (field, sorted(table)[:4]))
+    return cn, cs, cv
+
+
+# ---------------------------------------------------------------- boundaries
+def _bc_ops(grid, slots, init):
+    """(phase, field, axis, side, ranges) in the reference's emission order."""
+    m = grid.margin.value
+    dims = [d.value for d in grid.dim]
+    ops = []
+    stress = [s for s in grid.sfields]
+    for f in stress:
+        normal = f.direction[0] == f.direction[1]
+        for d in range(3):
+            if normal and f.direction[0] != d + 1 and (init or grid.legacy_levander):
+                continue
+            for side in range(2):
+                bc = f.bc[d + 1][side]
+                if not bc:
+                    continue
+                rng = []
+                for d2 in range(3):
+                    if d2 == d:
+                        continue
+                    if normal and f.direction[0] != d + 1:
+                        rng.append((m + 1, dims[d2] - m - 1))
+                    else:
+                        rng.append((0, dims[d2]))
+                ops.append((0 if init else 2, f, d, side, rng))
+    for d in range(3):
+        seq = [f for f in grid.vfields if f.staggered[d + 1]] + \
+              [f for f in grid.vfields if not f.staggered[d + 1]]
+        for f in seq:
+            for side in range(2):
+                if not f.bc[d + 1][side]:
+                    continue
+                rng = [(1, dims[d2] - 1) for d2 in range(3) if d2 != d]
+                ops.append((1 if init else 3, f, d, side, rng))
+    return ops
+
+
+class _BcBuilder(object):
+    def __init__(self, grid, slots):
+        self.grid = grid
+        self.slots = slots
+        self.names = {str(f.label): k for k, f in enumerate(slots)}
+        self.stmts = []
+        self.terms = []
+        self.code = []
+        self.op_of_stmt = []
+
+    def media_anchor_for(self, axis):
+        idx = self.grid.index
+
+        def anchor(e):
+            name = str(e.base.label)
+            if name not in exprvm.MEDIA_IDS:
+                raise PlanError('unknown media array %s' % name)
+            offs = []
+            for a in range(3):
+                v = e.indices[a]
+                offs.append(int(v) if a == axis else int(v - idx[a]))
+            return exprvm.MEDIA_IDS[name], offs
+        return anchor
+
+    def add_op(self, op_id, phase, field, axis, side, ranges):
+        g = self.grid
+        idx = g.index
+        others = [a for a in range(3) if a != axis]
+        eqs = g.transform_bc(field, axis + 1, side)
+        if g.read:
+            eqs = [g.resolve_media_params(e) for e in eqs]
+        for eq in eqs:
+            lhs = eq.lhs
+            if not isinstance(lhs, Indexed) or str(lhs.base.label) not in self.names:
+                raise PlanError('boundary equation with lhs %s' % lhs)
+            dst = self.names[str(lhs.base.label)]
+            for a in others:
+                if lhs.indices[1 + a] != idx[a]:
+                    raise PlanError('boundary lhs must loop over the tangential axes: %s' % lhs)
+            plane = int(lhs.indices[1 + axis])
+            t0 = len(self.terms)
+            rhs = expand(eq.rhs)
+            if rhs != 0:
+                for term in Add.make_args(rhs):
+                    num, other, fld = _split(term, self.names)
+                    if fld is None:
+                        raise PlanError('boundary term without a field: %s' % term)
+                    src = self.names[str(fld.base.label)]
+                    lvl = _time_level(fld.indices[0], g)
+                    p = int(fld.indices[1 + axis])
+                    offs = [int(fld.indices[1 + a] - idx[a]) for a in others]
+                    prog, plen, coef = -1, 0, f_lit(num)
+                    if other:
+                        expr = Mul(*other)
+                        code = exprvm.compile_expr(expr, coords={}, consts={},
+                                                   media_anchor=self.media_anchor_for(axis))
+                        code = [(op, a, f_lit(v) if op == exprvm.OP_CONST else v) for op, a, v in code]
+                        prog, plen = len(self.code), len(code)
+                        self.code += code
+                    self.terms.append((src, lvl, p, offs, prog, plen, coef))
+            self.stmts.append(dict(field=dst, level=_time_level(lhs.indices[0], g), axis=axis,
+                                   plane=plane, lo=[r[0] for r in ranges], hi=[r[1] for r in ranges],
+                                   term0=t0, nterms=len(self.terms) - t0, phase=phase, sched=0))
+            self.op_of_stmt.append(op_id)
+
+    # -------- hazard analysis
+    def _boxes(self, s):
+        """write box and read boxes of a statement: (field, level, ((lo,hi) x3))."""
+        ax = s['axis']
+        others = [a for a in range(3) if a != ax]
+        w = [None] * 3
+        w[ax] = (s['plane'], s['plane'] + 1)
+        for k, a in enumerate(others):
+            w[a] = (s['lo'][k], s['hi'][k])
+        wbox = (s['field'], s['level'], tuple(w))
+        rboxes = []
+        for t in self.terms[s['term0']:s['term0'] + s['nterms']]:
+            src, lvl, p, offs = t[0], t[1], t[2], t[3]
+            r = [None] * 3
+            r[ax] = (p, p + 1)
+            for k, a in enumerate(others):
+                r[a] = (s['lo'][k] + offs[k], s['hi'][k] + offs[k])
+            rboxes.append((src, lvl, tuple(r)))
+        return wbox, rboxes
+
+    @staticmethod
+    def _hit(a, b):
+        if a[0] != b[0] or a[1] != b[1]:
+            return False
+        return all(x[0] < y[1] and y[0] < x[1] for x, y in zip(a[2], b[2]))
+
+    def schedule(self):
+        boxes = [self._boxes(s) for s in self.stmts]
+        for j, sj in enumerate(self.stmts):
+            wj, rj = boxes[j]
+            lvl = 0
+            for i in range(j):
+                si = self.stmts[i]
+                if si['phase'] != sj['phase']:
+                    continue
+                wi, ri = boxes[i]
+                conflict = self._hit(wi, wj) or any(self._hit(wi, r) for r in rj) or \
+                    any(self._hit(r, wj) for r in ri)
+                if conflict:
+                    if self.op_of_stmt[i] == self.op_of_stmt[j]:
+                        raise PlanError('boundary statements of one loop depend on each other; '
+                                        'the sequential per-point semantics cannot be scheduled')
+                    lvl = max(lvl, si['sched'] + 1)
+            sj['sched'] = lvl
+
+    def arrays(self):
+        st = np.zeros(len(self.stmts), dtype=STMT_DTYPE)
+        for i, s in enumerate(self.stmts):
+            for k in ('field', 'level', 'axis', 'plane', 'term0', 'nterms', 'phase', 'sched'):
+                st[i][k] = s[k]
+            st[i]['lo'] = s['lo']
+            st[i]['hi'] = s['hi']
+        te = np.zeros(len(self.terms), dtype=TERM_DTYPE)
+        for i, (src, lvl, p, offs, prog, plen, coef) in enumerate(self.terms):
+            te[i]['field'] = src
+            te[i]['level'] = lvl
+            te[i]['plane'] = p
+            te[i]['off'] = offs
+            te[i]['prog'] = prog
+            te[i]['prog_len'] = plen
+            te[i]['coef'] = coef
+        return st, te, exprvm.to_array(self.code)
+
+
+def _build_bc(grid, slots):
+    b = _BcBuilder(grid, slots)
+    op_id = 0
+    for init in (True, False):
+        for phase, f, d, side, rng in _bc_ops(grid, slots, init):
+            b.add_op(op_id, phase, f, d, side, rng)
+            op_id += 1
+    # stable order by phase keeps the reference order inside each phase
+    order = sorted(range(len(b.stmts)), key=lambda i: b.stmts[i]['phase'])
+    b.stmts = [b.stmts[i] for i in order]
+    b.op_of_stmt = [b.op_of_stmt[i] for i in order]
+    b.schedule()
+    return b
+
+
+# ---------------------------------------------------------------- analytic
+def _media_consts(grid):
+    out = {}
+    if not grid.read:
+        for n in ('rho', 'beta', 'lambda', 'mu'):
+            v = grid.defined_variable[n]
+            out[Symbol(n)] = float(np.float32(v.value)) if v.type == 'float' else float(v.value)
+    grid.create_const_dict()
+    for s, v in grid.const_dict.items():
+        if s not in out and isinstance(v, (int, float)) and s.name not in ('dim1', 'dim2', 'dim3'):
+            out[s] = float(v)
+    return out
+
+
+def _solutions(grid, slots):
+    consts = _media_consts(grid)
+    coords = {s: a for a, s in enumerate(grid.index)}
+    code, off, ln = [], [], []
+    for f in slots:
+        sol = f.sol
+        if sol is None:
+            sol = Integer(0)
+        if grid.read:
+            sol = grid.resolve_media_params(sol.subs(grid.media_dict))
+
+        def anchor(e):
+            name = str(e.base.label)
+            offs = [int(e.indices[a] - grid.index[a]) for a in range(3)]
+            return exprvm.MEDIA_IDS[name], offs
+        c = exprvm.compile_expr(sol, coords=coords, time=grid.t, consts=consts, media_anchor=anchor)
+        off.append(len(code))
+        ln.append(len(c))
+        code += c
+    return exprvm.to_array(code), off, ln
+
+
+def _media_arrays(grid, dims):
+    shape = tuple(dims)
+    n = dims[0] * dims[1] * dims[2]
+    if grid.media_arrays is not None:
+        arrs = [np.ascontiguousarray(a, dtype=np.float32).reshape(shape) for a in grid.media_arrays]
+    else:
+        arrs = []
+        for fn in (grid.rho_file, grid.vp_file, grid.vs_file):
+            if not fn or not os.path.exists(fn):
+                raise IOError('media file %r not found' % fn)
+            a = np.fromfile(fn, dtype=np.float32)
+            if a.size < n:
+                raise IOError('media file %s holds %d values, %d needed' % (fn, a.size, n))
+            arrs.append(a[:n].reshape(shape))
+    return arrs
+
+
+# ---------------------------------------------------------------- entry
+def build_plan(grid):
+    if grid.dimension != 3:
+        raise PlanError('the elastic backend is 3-D')
+    if grid.order[0] != 2:
+        raise PlanError('leapfrog: time order must be 2')
+    orders = set(grid.order[1:])
+    if len(orders) != 1 or grid.order[1] not in (2, 4, 6, 8):
+        raise PlanError('equal spatial orders 2, 4, 6 or 8 are supported')
+    slots = _canonical_slots(grid)
+    p = ModelPlan()
+    p.order = grid.order[1]
+    p.M = p.order // 2
+    p.dims = tuple(d.value for d in grid.dim)
+    p.grid_size = tuple(grid.grid_size)
+    p.ntsteps = int(grid.ntsteps.value)
+    p.dt = float(grid.dt.value)
+    p.spacing = tuple(float(s.value) for s in grid.spacing)
+    p.real_bytes = 8 if grid.double else 4
+    p.storage = grid.storage
+    st = STORAGE_CODE.get(grid.storage)
+    if st is None:
+        raise PlanError('unknown storage %s' % grid.storage)
+    if st and grid.double:
+        raise PlanError('16-bit storage needs the float (FP32 compute) build')
+    p.storage_code = st
+    p.media_arrays = bool(grid.read)
+    p.media_const = np.zeros(12)
+    if not grid.read:
+        dv = grid.defined_variable
+        for n, i in (('rho', 0), ('beta', 3), ('lambda', 4), ('mu', 5)):
+            p.media_const[i] = float(np.float32(dv[n].value))
+        rho, lam, mu = dv['rho'].value, dv['lambda'].value, dv['mu'].value
+        p.media_const[1] = float(np.float32(((lam + 2 * mu) / rho) ** 0.5))
+        p.media_const[2] = float(np.float32((mu / rho) ** 0.5))
+        p.media = None
+    else:
+        p.media = _media_arrays(grid, p.dims)
+    p.cn, p.cs, p.cv = _extract_interior(grid, slots)
+    b = _build_bc(grid, slots)
+    p.stmts, p.terms, p.bc_code = b.arrays()
+    p.nlaunches_step = int(sum(len(set(p.stmts['sched'][p.stmts['phase'] == ph])) for ph in (2, 3)))
+    p.sol_code, p.sol_off, p.sol_len = _solutions(grid, slots)
+    p.src_wavelet = getattr(grid, 'src_wavelet', None)
+    p.src_idx = getattr(grid, 'src_idx', (0, 0, 0))
+    p.rec_idx = getattr(grid, 'rec_idx', None)
+    p.tile_y = getattr(grid, 'tile_y', 0)
+    p.x_chunk = getattr(grid, 'x_chunk', 0)
+    p.kernel_variant = getattr(grid, 'kernel_variant', 0)
+    p.use_graph = 1 if grid.use_graph else 0
+    p.flops_per_point = None
+    p.field_names = tuple(str(f.label) for f in slots)
+    return p
