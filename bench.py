@@ -1,0 +1,135 @@
+"""Benchmark: elastic grid-point updates/s on one MI355X.
+
+Workload: the eigenwave of tests/eigenwave3d.py at 256^3 interior points
+(grid_size 255, dims 260^3), 4th order in space / 2nd in time, FP32,
+constant media rho = lambda = mu = 1, dt = 0.5 dx / sqrt(3 (lambda+2mu)/rho),
+Levander free surfaces on all faces.  A step is one full leapfrog step
+(stress update, stress boundary launches, velocity update, velocity boundary
+launches) over the whole grid, run by libopesci_hip through its C ABI with
+the fields resident in HBM.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Prints one JSON line.  The roofline entry is for the stress kernel (the
+dominant launch: 9 field loads + 6 stores per point); its duration is taken
+with HIP events on the context stream.  cpu_baseline times the oracle (the
+CPU/OpenMP restatement of the reference's generated code, reference GNU
+flags) on a bounded number of steps of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (os.path.join(ROOT, 'opesci-fd_amd'), os.path.join(ROOT, 'tests'), ROOT):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = 'elastic grid-pt updates/s and % of HBM roofline, 1 MI355X, at stated L2 error'
+HBM_PEAK = 8.0e12
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=500)
+    ap.add_argument('--warmup', type=int, default=20)
+    ap.add_argument('--size', type=int, default=256, help='interior points per axis')
+    ap.add_argument('--order', type=int, default=4)
+    ap.add_argument('--storage', default='native')
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--cpu-steps', type=int, default=40)
+    ap.add_argument('--variant', type=int, default=None, help='interior kernel variant (0 sweep, 1 gather)')
+    ap.add_argument('--x-chunk', type=int, default=0)
+    ap.add_argument('--tile-y', type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py runs on exactly one GPU: this backend has no multi-GPU path '
+              '(the slab decomposition is a design note only); use --gpus 1', file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from opesci.backend import Context
+    from workloads import eigen_model
+
+    N, order, K, W = args.size, args.order, args.steps, args.warmup
+    g, kw = eigen_model(N, order, K + W, storage=args.storage)
+    plan = g.lower()
+    if args.variant is not None:
+        plan.kernel_variant = args.variant
+    plan.x_chunk = args.x_chunk
+    plan.tile_y = args.tile_y
+    pts = float(N) ** 3
+    elem = 2 if args.storage in ('fp16', 'bf16') else 4
+    with Context(plan) as ctx:
+        ctx.init()
+        ctx.run(W)
+        t0 = time.perf_counter()
+        secs = ctx.run(K)   # HIP events on the context stream, synchronised on return
+        wall = time.perf_counter() - t0
+        l2 = ctx.l2()
+        kt = ctx.kernel_times(10)   # per-launch averages, after the timed region
+    ms_step = secs * 1e3 / K
+    value = pts * K / secs
+    stress_bytes = pts * 15 * elem            # 9 loads + 6 stores per point
+    achieved = stress_bytes / (kt[0] * 1e-3)
+    step_bytes = pts * 27 * elem              # reference LOAD/STORE model, both kernels
+    out = {
+        'metric': METRIC,
+        'value': value,
+        'unit': 'grid-pt updates/s',
+        'n_gpus': 1,
+        'steps': K,
+        'warmup': W,
+        'ms_per_step': ms_step,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f32' if elem == 4 else 'f32 compute, %s storage' % args.storage,
+        'data': 'synthetic',
+        'config': {
+            'workload': 'eigenwave3d %d^3 interior (dims %d^3), order %d space / 2 time, %s, rho=lambda=mu=1, '
+                        'dt=dx/6, Levander surfaces' % (N, N + order, order, args.storage),
+            'grid_points': int(pts),
+            'l2_error_vs_analytic': dict(zip(('U', 'V', 'W', 'Txx', 'Tyy', 'Tzz', 'Txy', 'Tyz', 'Txz'),
+                                             [float('%.6e' % v) for v in l2])),
+            'step_hbm_roofline_frac': value * step_bytes / pts / HBM_PEAK,
+            'wall_s': wall,
+            'kernel_ms': {'stress': kt[0], 'velocity': kt[1], 'boundary': kt[2]},
+            'boundary_launches_per_step': int(plan.nlaunches_step),
+        },
+        'roofline': {
+            'bound': 'hbm',
+            'achieved': achieved / 1e9,
+            'peak': HBM_PEAK / 1e9,
+            'unit': 'GB/s',
+            'frac': achieved / HBM_PEAK,
+            'traffic': None,
+            'kernel': 'stress update (6 stresses at t1 from U,V,W and stresses at t0)',
+            'algorithmic_bytes_per_launch': stress_bytes,
+        },
+    }
+    if not args.no_cpu:
+        out['cpu_baseline'] = cpu_baseline(N, order, args.cpu_steps)
+    print(json.dumps(out))
+
+
+def cpu_baseline(N, order, steps):
+    sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+    import oracle
+    from workloads import eigen_model
+    _, kw = eigen_model(N, order, steps)
+    cores = min(16, os.cpu_count() or 1)
+    kw['nthreads'] = cores
+    kw['want_fields'] = False
+    r = oracle.run(**kw)
+    pts = float(N) ** 3 * steps
+    return {'value': pts / r['seconds'], 'unit': 'grid-pt updates/s', 'cores': cores, 'kind': 'port',
+            'sample': '%d steps of the same %d^3 order-%d FP32 eigenwave, oracle/elastic_oracle.cpp built with the '
+                      'reference GNU flags (-O3 -fno-tree-vectorize -fopenmp), time loop only' % (steps, N, order)}
+
+
+if __name__ == '__main__':
+    main()

@@ -1,0 +1,416 @@
+// Boundary program, analytic initial state, L2 norms, effective media,
+// layout conversion and point source / receivers.
+#include "common.h"
+
+namespace opesci {
+
+// ------------------------------------------------------------------ bytecode
+// Stack machine over OpesciExprOp (opcodes in include/opesci_hip.h),
+// evaluated in double.  MEDIA reads media a[0] at anchor + (a[1],a[2],a[3]).
+template <typename R>
+__device__ double vm_eval(const OpesciExprOp* code, int n, const double coord[3], double t,
+                          const MediaPtrs<R>& md, const Geom& g, int ax, int ay, int az) {
+    double st[16];
+    int sp = 0;
+    for (int i = 0; i < n; ++i) {
+        const OpesciExprOp o = code[i];
+        switch (o.op) {
+            case OPESCI_OP_CONST: st[sp++] = o.val; break;
+            case OPESCI_OP_COORD: st[sp++] = coord[o.a[0]]; break;
+            case OPESCI_OP_TIME: st[sp++] = t; break;
+            case OPESCI_OP_MEDIA: {
+                const R* p = md.m[o.a[0]];
+                st[sp++] = p ? (double)p[g.at(ax + o.a[1], ay + o.a[2], az + o.a[3])] : 0.0;
+                break;
+            }
+            case OPESCI_OP_ADD: --sp; st[sp - 1] = st[sp - 1] + st[sp]; break;
+            case OPESCI_OP_SUB: --sp; st[sp - 1] = st[sp - 1] - st[sp]; break;
+            case OPESCI_OP_MUL: --sp; st[sp - 1] = st[sp - 1] * st[sp]; break;
+            case OPESCI_OP_DIV: --sp; st[sp - 1] = st[sp - 1] / st[sp]; break;
+            case OPESCI_OP_NEG: st[sp - 1] = -st[sp - 1]; break;
+            case OPESCI_OP_POWC: st[sp - 1] = pow(st[sp - 1], o.val); break;
+            case OPESCI_OP_SQRT: st[sp - 1] = sqrt(st[sp - 1]); break;
+            case OPESCI_OP_SIN: st[sp - 1] = sin(st[sp - 1]); break;
+            case OPESCI_OP_COS: st[sp - 1] = cos(st[sp - 1]); break;
+            case OPESCI_OP_EXP: st[sp - 1] = exp(st[sp - 1]); break;
+            case OPESCI_OP_LOG: st[sp - 1] = log(st[sp - 1]); break;
+            case OPESCI_OP_ROUNDF: st[sp - 1] = (double)(float)st[sp - 1]; break;
+            default: break;
+        }
+    }
+    return sp > 0 ? st[0] : 0.0;
+}
+
+// real_t x = dx*(i - m [+ 0.5]) of the reference loops, rounded once to R
+template <typename R>
+__device__ __forceinline__ double coord_of(double dx, int i, int m, int stagger) {
+    const R h = (R)dx;
+    return (double)(R)((double)h * ((double)(i - m) + (stagger ? 0.5 : 0.0)));
+}
+
+// ------------------------------------------------------------------ boundary program
+// One launch per hazard level: blockIdx.y picks the statement, threads cover
+// its plane loop; statements of one level write disjoint cells and never
+// read what another statement of the level writes.
+template <typename R, typename S>
+__global__ __launch_bounds__(256) void bc_kernel(Geom g, FieldPtrs<S> F, MediaPtrs<R> md,
+                                                 const OpesciBcStmt* __restrict__ stmts,
+                                                 const OpesciBcTerm* __restrict__ terms,
+                                                 const OpesciExprOp* __restrict__ code, int first, int t0, int t1) {
+    using C = Conv<S, R>;
+    const OpesciBcStmt s = stmts[first + blockIdx.y];
+    const int n0 = s.hi[0] - s.lo[0], n1 = s.hi[1] - s.lo[1];
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n0 <= 0 || n1 <= 0 || p >= n0 * n1) return;
+    const int i0 = s.lo[0] + p / n1, i1 = s.lo[1] + p % n1;
+    const int o0 = s.axis == 0 ? 1 : 0, o1 = s.axis == 2 ? 1 : 2;
+    R acc = (R)0;
+    for (int k = 0; k < s.nterms; ++k) {
+        const OpesciBcTerm t = terms[s.term0 + k];
+        int r[3];
+        r[s.axis] = t.plane;
+        r[o0] = i0 + t.off[0];
+        r[o1] = i1 + t.off[1];
+        const R v = C::ld(F.f[t.field][t.level ? t1 : t0] + g.at(r[0], r[1], r[2]));
+        R coef = (R)t.coef;
+        if (t.prog >= 0) {
+            int a[3];
+            a[s.axis] = 0;
+            a[o0] = i0;
+            a[o1] = i1;
+            const double c0[3] = {0, 0, 0};
+            coef = (R)(t.coef * vm_eval<R>(code + t.prog, t.prog_len, c0, 0.0, md, g, a[0], a[1], a[2]));
+        }
+        acc = fma(coef, v, acc);
+    }
+    int q[3];
+    q[s.axis] = s.plane;
+    q[o0] = i0;
+    q[o1] = i1;
+    C::st(F.f[s.field][s.level ? t1 : t0] + g.at(q[0], q[1], q[2]), acc);
+}
+
+struct SolTable {
+    int off[NF];
+    int len[NF];
+    double dx[3];
+    double t[NF];
+};
+
+// ------------------------------------------------------------------ initial state
+// StaggeredGrid.initialise (opesci/staggeredgrid.py:612-659): level 0,
+// ranges [m, dim-m) or [m, dim-m-1) when staggered, t = dt/2 for velocities.
+template <typename R, typename S>
+__global__ __launch_bounds__(256) void initial_kernel(Geom g, FieldPtrs<S> F, MediaPtrs<R> md,
+                                                      const OpesciExprOp* __restrict__ sol, SolTable tab) {
+    using C = Conv<S, R>;
+    const int f = blockIdx.y;
+    const int m = g.m;
+    const int nx = g.nx - 2 * m - stag(f, 0), ny = g.ny - 2 * m - stag(f, 1), nz = g.nz - 2 * m - stag(f, 2);
+    const int64_t n = (int64_t)nx * ny * nz;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+        const int z = m + (int)(p % nz), y = m + (int)((p / nz) % ny), x = m + (int)(p / ((int64_t)nz * ny));
+        const double c[3] = {coord_of<R>(tab.dx[0], x, m, stag(f, 0)), coord_of<R>(tab.dx[1], y, m, stag(f, 1)),
+                             coord_of<R>(tab.dx[2], z, m, stag(f, 2))};
+        const double v = vm_eval<R>(sol + tab.off[f], tab.len[f], c, tab.t[f], md, g, x, y, z);
+        C::st(F.f[f][0] + g.at(x, y, z), (R)v);
+    }
+}
+
+// ------------------------------------------------------------------ L2 norms
+// sum over the interior of (F - F_exact(t_n))^2 in double
+// (StaggeredGrid.converge_test, opesci/staggeredgrid.py:892-945); wavefront
+// reduction by shuffles, one partial per block.
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <typename R, typename S>
+__global__ __launch_bounds__(256) void l2_kernel(Geom g, FieldPtrs<S> F, MediaPtrs<R> md,
+                                                 const OpesciExprOp* __restrict__ sol, SolTable tab, int level,
+                                                 double* __restrict__ partial) {
+    using C = Conv<S, R>;
+    __shared__ double red[4];
+    const int f = blockIdx.y;
+    const int m = g.m;
+    const int nx = g.nx - 2 * m - stag(f, 0), ny = g.ny - 2 * m - stag(f, 1), nz = g.nz - 2 * m - stag(f, 2);
+    const int64_t n = (int64_t)nx * ny * nz;
+    double acc = 0.0;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+        const int z = m + (int)(p % nz), y = m + (int)((p / nz) % ny), x = m + (int)(p / ((int64_t)nz * ny));
+        const double c[3] = {coord_of<R>(tab.dx[0], x, m, stag(f, 0)), coord_of<R>(tab.dx[1], y, m, stag(f, 1)),
+                             coord_of<R>(tab.dx[2], z, m, stag(f, 2))};
+        const double e = (double)C::ld(F.f[f][level] + g.at(x, y, z)) -
+                         vm_eval<R>(sol + tab.off[f], tab.len[f], c, tab.t[f], md, g, x, y, z);
+        acc += e * e;
+    }
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[(size_t)f * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// ------------------------------------------------------------------ media
+// StaggeredGrid.read_data (opesci/staggeredgrid.py:522-598): beta = 1/rho,
+// lambda = rho (vp^2 - 2 vs^2), mu = rho vs^2 on every cell; beta1..3 =
+// arithmetic means over the +x/+y/+z neighbour, mu12/mu13/mu23 = harmonic
+// means over the 4-cell faces (the last index clamps to itself).
+template <typename R>
+__global__ void media_base_kernel(Geom g, const float* __restrict__ rho, const float* __restrict__ vp,
+                                  const float* __restrict__ vs, MediaPtrs<R> out) {
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+        const int z = (int)(p % g.nz), y = (int)((p / g.nz) % g.ny), x = (int)(p / ((int64_t)g.nz * g.ny));
+        const double r = rho[p], a = vp[p], b = vs[p];
+        const int64_t q = g.at(x, y, z);
+        R* const* o = (R* const*)out.m;
+        o[OPESCI_M_RHO][q] = (R)r;
+        o[OPESCI_M_VP][q] = (R)a;
+        o[OPESCI_M_VS][q] = (R)b;
+        o[OPESCI_M_BETA][q] = (R)(1.0 / r);
+        o[OPESCI_M_LAMBDA][q] = (R)(r * (a * a - 2.0 * b * b));
+        o[OPESCI_M_MU][q] = (R)(r * (b * b));
+    }
+}
+
+template <typename R>
+__global__ void media_eff_kernel(Geom g, MediaPtrs<R> md) {
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    R* const* o = (R* const*)md.m;
+    const R* beta = md.m[OPESCI_M_BETA];
+    const R* mu = md.m[OPESCI_M_MU];
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+        const int z = (int)(p % g.nz), y = (int)((p / g.nz) % g.ny), x = (int)(p / ((int64_t)g.nz * g.ny));
+        const int x1 = min(x + 1, g.nx - 1), y1 = min(y + 1, g.ny - 1), z1 = min(z + 1, g.nz - 1);
+        const int64_t q = g.at(x, y, z);
+        const double b0 = beta[q];
+        o[OPESCI_M_BETA1][q] = (R)((b0 + (double)beta[g.at(x1, y, z)]) / 2.0);
+        o[OPESCI_M_BETA2][q] = (R)((b0 + (double)beta[g.at(x, y1, z)]) / 2.0);
+        o[OPESCI_M_BETA3][q] = (R)((b0 + (double)beta[g.at(x, y, z1)]) / 2.0);
+        const double i0 = 1.0 / (double)mu[q];
+        const double ix = 1.0 / (double)mu[g.at(x1, y, z)], iy = 1.0 / (double)mu[g.at(x, y1, z)],
+                     iz = 1.0 / (double)mu[g.at(x, y, z1)];
+        o[OPESCI_M_MU12][q] = (R)(1.0 / (0.25 * (i0 + ix + iy + 1.0 / (double)mu[g.at(x1, y1, z)])));
+        o[OPESCI_M_MU13][q] = (R)(1.0 / (0.25 * (i0 + ix + iz + 1.0 / (double)mu[g.at(x1, y, z1)])));
+        o[OPESCI_M_MU23][q] = (R)(1.0 / (0.25 * (i0 + iy + iz + 1.0 / (double)mu[g.at(x, y1, z1)])));
+    }
+}
+
+// ------------------------------------------------------------------ layout
+template <typename R, typename S>
+__global__ void pack_kernel(Geom g, const R* __restrict__ dense, S* __restrict__ field) {
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+        const int z = (int)(p % g.nz), y = (int)((p / g.nz) % g.ny), x = (int)(p / ((int64_t)g.nz * g.ny));
+        Conv<S, R>::st(field + g.at(x, y, z), dense[p]);
+    }
+}
+
+template <typename R, typename S>
+__global__ void unpack_kernel(Geom g, const S* __restrict__ field, R* __restrict__ dense) {
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+        const int z = (int)(p % g.nz), y = (int)((p / g.nz) % g.ny), x = (int)(p / ((int64_t)g.nz * g.ny));
+        dense[p] = Conv<S, R>::ld(field + g.at(x, y, z));
+    }
+}
+
+__global__ void zero_kernel(uint4* __restrict__ p, size_t n16) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = make_uint4(0, 0, 0, 0);
+}
+
+// ------------------------------------------------------------------ source / receivers
+// tests/src/test_ref_iso_elastic.cpp:227-290: after the velocity update record
+// U, V, W and (Txx+Tyy+Tzz)/3 at every receiver, then Tnn -= w[step]/3 at the
+// source.  The step counter lives on the device so a captured graph can be
+// replayed.
+template <typename R, typename S>
+__global__ __launch_bounds__(256) void record_source_kernel(Geom g, FieldPtrs<S> F, int t1, int nrec,
+                                                            const int32_t* __restrict__ rec,
+                                                            float* __restrict__ trace, int src_on, int sx, int sy,
+                                                            int sz, const float* __restrict__ w, int* step,
+                                                            int ntsteps) {
+    using C = Conv<S, R>;
+    const int s = *step;
+    if (s < ntsteps && trace) {
+        for (int r = threadIdx.x; r < nrec; r += blockDim.x) {
+            const int64_t q = g.at(rec[3 * r], rec[3 * r + 1], rec[3 * r + 2]);
+            float* o = trace + ((size_t)s * nrec + r) * 4;
+            o[0] = (float)C::ld(F.f[0][t1] + q);
+            o[1] = (float)C::ld(F.f[1][t1] + q);
+            o[2] = (float)C::ld(F.f[2][t1] + q);
+            o[3] = (float)((C::ld(F.f[3][t1] + q) + C::ld(F.f[4][t1] + q) + C::ld(F.f[5][t1] + q)) / (R)3);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (src_on && s < ntsteps) {
+            const int64_t q = g.at(sx, sy, sz);
+            const R v = (R)w[s] / (R)3;
+            for (int f = 3; f < 6; ++f) C::st(F.f[f][t1] + q, C::ld(F.f[f][t1] + q) - v);
+        }
+        *step = s + 1;
+    }
+}
+
+// ------------------------------------------------------------------ launchers
+template <typename R, typename S>
+static hipError_t bc_t(const Geom& g, void* fields, const void* media, const OpesciBcStmt* st, const OpesciBcTerm* te,
+                       const OpesciExprOp* code, const BcLevel& lv, int t0, int t1, hipStream_t s) {
+    if (lv.count <= 0 || lv.maxpts <= 0) return hipSuccess;
+    dim3 grid((lv.maxpts + 255) / 256, lv.count);
+    hipLaunchKernelGGL((bc_kernel<R, S>), grid, dim3(256), 0, s, g, *static_cast<const FieldPtrs<S>*>(fields),
+                       *static_cast<const MediaPtrs<R>*>(media), st, te, code, lv.first, t0, t1);
+    return hipGetLastError();
+}
+
+#define OPESCI_TYPED(RB, ST, CALL_F, CALL_D, CALL_H, CALL_B) \
+    ((RB) == 8 ? (CALL_D) : (ST) == OPESCI_STORE_F16 ? (CALL_H) : (ST) == OPESCI_STORE_BF16 ? (CALL_B) : (CALL_F))
+
+hipError_t launch_bc(const Geom& g, int real_bytes, int storage, void* fields, const void* media,
+                     const OpesciBcStmt* d_stmts, const OpesciBcTerm* d_terms, const OpesciExprOp* d_code,
+                     const BcLevel& lvl, int t0, int t1, hipStream_t s) {
+    return OPESCI_TYPED(real_bytes, storage,
+                        (bc_t<float, float>(g, fields, media, d_stmts, d_terms, d_code, lvl, t0, t1, s)),
+                        (bc_t<double, double>(g, fields, media, d_stmts, d_terms, d_code, lvl, t0, t1, s)),
+                        (bc_t<float, _Float16>(g, fields, media, d_stmts, d_terms, d_code, lvl, t0, t1, s)),
+                        (bc_t<float, bf16_t>(g, fields, media, d_stmts, d_terms, d_code, lvl, t0, t1, s)));
+}
+
+static SolTable make_tab(const int* off, const int* len, const double* dx) {
+    SolTable t;
+    for (int f = 0; f < NF; ++f) {
+        t.off[f] = off[f];
+        t.len[f] = len[f];
+        t.t[f] = 0.0;
+    }
+    for (int a = 0; a < 3; ++a) t.dx[a] = dx[a];
+    return t;
+}
+
+template <typename R, typename S>
+static hipError_t init_t(const Geom& g, void* fields, const void* media, const OpesciExprOp* sol, SolTable tab,
+                         hipStream_t s) {
+    hipLaunchKernelGGL((initial_kernel<R, S>), dim3(512, NF), dim3(256), 0, s, g,
+                       *static_cast<const FieldPtrs<S>*>(fields), *static_cast<const MediaPtrs<R>*>(media), sol, tab);
+    return hipGetLastError();
+}
+
+hipError_t launch_initial(const Geom& g, int real_bytes, int storage, void* fields, const void* media,
+                          const OpesciExprOp* d_sol, const int* sol_off, const int* sol_len, const double* spacing,
+                          double dt, hipStream_t s) {
+    SolTable tab = make_tab(sol_off, sol_len, spacing);
+    for (int f = 0; f < NF; ++f) tab.t[f] = time_stag(f) ? dt / 2.0 : 0.0;
+    return OPESCI_TYPED(real_bytes, storage, (init_t<float, float>(g, fields, media, d_sol, tab, s)),
+                        (init_t<double, double>(g, fields, media, d_sol, tab, s)),
+                        (init_t<float, _Float16>(g, fields, media, d_sol, tab, s)),
+                        (init_t<float, bf16_t>(g, fields, media, d_sol, tab, s)));
+}
+
+template <typename R, typename S>
+static hipError_t l2_t(const Geom& g, void* fields, const void* media, const OpesciExprOp* sol, SolTable tab,
+                       int level, double* partial, int nblk, hipStream_t s) {
+    hipLaunchKernelGGL((l2_kernel<R, S>), dim3(nblk, NF), dim3(256), 0, s, g, *static_cast<const FieldPtrs<S>*>(fields),
+                       *static_cast<const MediaPtrs<R>*>(media), sol, tab, level, partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_l2(const Geom& g, int real_bytes, int storage, void* fields, const void* media,
+                     const OpesciExprOp* d_sol, const int* sol_off, const int* sol_len, const double* spacing,
+                     int level, const double t_field[NF], double* d_partial, int partial_cap, double* out_host,
+                     hipStream_t s) {
+    SolTable tab = make_tab(sol_off, sol_len, spacing);
+    for (int f = 0; f < NF; ++f) tab.t[f] = t_field[f];
+    const int nblk = partial_cap / NF;
+    hipError_t e = OPESCI_TYPED(real_bytes, storage,
+                                (l2_t<float, float>(g, fields, media, d_sol, tab, level, d_partial, nblk, s)),
+                                (l2_t<double, double>(g, fields, media, d_sol, tab, level, d_partial, nblk, s)),
+                                (l2_t<float, _Float16>(g, fields, media, d_sol, tab, level, d_partial, nblk, s)),
+                                (l2_t<float, bf16_t>(g, fields, media, d_sol, tab, level, d_partial, nblk, s)));
+    if (e != hipSuccess) return e;
+    std::vector<double> h((size_t)NF * nblk);
+    e = hipMemcpyAsync(h.data(), d_partial, h.size() * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return e;
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    const double vol = spacing[0] * spacing[1] * spacing[2];
+    for (int f = 0; f < NF; ++f) {
+        double a = 0.0;
+        for (int b = 0; b < nblk; ++b) a += h[(size_t)f * nblk + b];
+        out_host[f] = sqrt(a * vol);
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_media(const Geom& g, int real_bytes, const float* d_rho, const float* d_vp, const float* d_vs,
+                        void* const* d_media, hipStream_t s) {
+    if (real_bytes == 8) {
+        MediaPtrs<double> mp;
+        for (int i = 0; i < OPESCI_M_COUNT; ++i) mp.m[i] = (const double*)d_media[i];
+        hipLaunchKernelGGL((media_base_kernel<double>), dim3(1024), dim3(256), 0, s, g, d_rho, d_vp, d_vs, mp);
+        hipLaunchKernelGGL((media_eff_kernel<double>), dim3(1024), dim3(256), 0, s, g, mp);
+    } else {
+        MediaPtrs<float> mp;
+        for (int i = 0; i < OPESCI_M_COUNT; ++i) mp.m[i] = (const float*)d_media[i];
+        hipLaunchKernelGGL((media_base_kernel<float>), dim3(1024), dim3(256), 0, s, g, d_rho, d_vp, d_vs, mp);
+        hipLaunchKernelGGL((media_eff_kernel<float>), dim3(1024), dim3(256), 0, s, g, mp);
+    }
+    return hipGetLastError();
+}
+
+template <typename R, typename S>
+static hipError_t pack_t(const Geom& g, const void* dense, void* field, hipStream_t s) {
+    hipLaunchKernelGGL((pack_kernel<R, S>), dim3(1024), dim3(256), 0, s, g, (const R*)dense, (S*)field);
+    return hipGetLastError();
+}
+template <typename R, typename S>
+static hipError_t unpack_t(const Geom& g, const void* field, void* dense, hipStream_t s) {
+    hipLaunchKernelGGL((unpack_kernel<R, S>), dim3(1024), dim3(256), 0, s, g, (const S*)field, (R*)dense);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack(const Geom& g, int real_bytes, int storage, const void* dense, void* field, hipStream_t s) {
+    return OPESCI_TYPED(real_bytes, storage, (pack_t<float, float>(g, dense, field, s)),
+                        (pack_t<double, double>(g, dense, field, s)), (pack_t<float, _Float16>(g, dense, field, s)),
+                        (pack_t<float, bf16_t>(g, dense, field, s)));
+}
+
+hipError_t launch_unpack(const Geom& g, int real_bytes, int storage, const void* field, void* dense, hipStream_t s) {
+    return OPESCI_TYPED(real_bytes, storage, (unpack_t<float, float>(g, field, dense, s)),
+                        (unpack_t<double, double>(g, field, dense, s)),
+                        (unpack_t<float, _Float16>(g, field, dense, s)), (unpack_t<float, bf16_t>(g, field, dense, s)));
+}
+
+hipError_t launch_pack_media(const Geom& g, int real_bytes, const void* dense, void* media, hipStream_t s) {
+    return real_bytes == 8 ? pack_t<double, double>(g, dense, media, s) : pack_t<float, float>(g, dense, media, s);
+}
+
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t s) {
+    const size_t n16 = bytes / 16;
+    if (n16) hipLaunchKernelGGL(zero_kernel, dim3(2048), dim3(256), 0, s, (uint4*)p, n16);
+    return hipGetLastError();
+}
+
+template <typename R, typename S>
+static hipError_t rec_t(const Geom& g, void* fields, int t1, int nrec, const int32_t* rec, float* trace, int src_on,
+                        const int32_t* si, const float* w, int* step, int ntsteps, hipStream_t s) {
+    hipLaunchKernelGGL((record_source_kernel<R, S>), dim3(1), dim3(256), 0, s, g,
+                       *static_cast<const FieldPtrs<S>*>(fields), t1, nrec, rec, trace, src_on, si[0], si[1], si[2],
+                       w, step, ntsteps);
+    return hipGetLastError();
+}
+
+hipError_t launch_record_source(const Geom& g, int real_bytes, int storage, void* fields, int t1, int nrec,
+                                const int32_t* d_rec, float* d_trace, int src_on, const int32_t* src_idx,
+                                const float* d_wavelet, int* d_step, int ntsteps, hipStream_t s) {
+    return OPESCI_TYPED(
+        real_bytes, storage,
+        (rec_t<float, float>(g, fields, t1, nrec, d_rec, d_trace, src_on, src_idx, d_wavelet, d_step, ntsteps, s)),
+        (rec_t<double, double>(g, fields, t1, nrec, d_rec, d_trace, src_on, src_idx, d_wavelet, d_step, ntsteps, s)),
+        (rec_t<float, _Float16>(g, fields, t1, nrec, d_rec, d_trace, src_on, src_idx, d_wavelet, d_step, ntsteps, s)),
+        (rec_t<float, bf16_t>(g, fields, t1, nrec, d_rec, d_trace, src_on, src_idx, d_wavelet, d_step, ntsteps, s)));
+}
+
+}  // namespace opesci

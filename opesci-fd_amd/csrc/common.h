@@ -1,0 +1,156 @@
+// Internal definitions shared by the libopesci_hip translation units.
+//
+// Device layout of one field time level: F[x][y][zoff + z], x slowest,
+// z contiguous, rows padded to `pitch` elements so that the first interior
+// z (= m) of every row starts on a 128-byte boundary for 4-byte storage.
+// Pointers handed to kernels are pre-offset by zoff, so element (x,y,z) is
+// p[(x*ny + y)*pitch + z] for z in [0, nz); the padding in front of z = 0
+// keeps every stencil read (|offset| <= m) inside the allocation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "opesci_hip.h"
+
+namespace opesci {
+
+constexpr int NF = 9;
+constexpr int MAXM = OPESCI_MAX_HALF_ORDER;
+
+// field staggering (x, y, z) in the order U, V, W, Txx, Tyy, Tzz, Txy, Tyz, Txz
+__host__ __device__ constexpr int stag(int f, int a) {
+    return (f == 0 && a == 0) || (f == 1 && a == 1) || (f == 2 && a == 2) ||
+                   (f == 6 && a != 2) || (f == 7 && a != 0) || (f == 8 && a != 1)
+               ? 1
+               : 0;
+}
+__host__ __device__ constexpr int time_stag(int f) { return f < 3 ? 1 : 0; }
+
+// ---------------------------------------------------------------- storage
+struct bf16_t {
+    uint16_t bits;
+};
+
+template <typename S, typename R>
+struct Conv;
+template <>
+struct Conv<float, float> {
+    __device__ __forceinline__ static float ld(const float* p) { return *p; }
+    __device__ __forceinline__ static void st(float* p, float v) { *p = v; }
+    __device__ __forceinline__ static float from(float v) { return v; }
+    __device__ __forceinline__ static float to(float v) { return v; }
+};
+template <>
+struct Conv<double, double> {
+    __device__ __forceinline__ static double ld(const double* p) { return *p; }
+    __device__ __forceinline__ static void st(double* p, double v) { *p = v; }
+    __device__ __forceinline__ static double from(double v) { return v; }
+    __device__ __forceinline__ static double to(double v) { return v; }
+};
+template <>
+struct Conv<_Float16, float> {
+    __device__ __forceinline__ static float from(_Float16 v) { return (float)v; }
+    __device__ __forceinline__ static _Float16 to(float v) { return (_Float16)v; }  // round to nearest even
+    __device__ __forceinline__ static float ld(const _Float16* p) { return from(*p); }
+    __device__ __forceinline__ static void st(_Float16* p, float v) { *p = to(v); }
+};
+template <>
+struct Conv<bf16_t, float> {
+    __device__ __forceinline__ static float from(bf16_t v) { return __uint_as_float((uint32_t)v.bits << 16); }
+    __device__ __forceinline__ static bf16_t to(float v) {
+        uint32_t u = __float_as_uint(v);
+        bf16_t r;
+        if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) {
+            r.bits = (uint16_t)((u >> 16) | 0x40u);  // quiet NaN
+        } else {
+            u += 0x7fffu + ((u >> 16) & 1u);          // round to nearest even
+            r.bits = (uint16_t)(u >> 16);
+        }
+        return r;
+    }
+    __device__ __forceinline__ static float ld(const bf16_t* p) { return from(*p); }
+    __device__ __forceinline__ static void st(bf16_t* p, float v) { *p = to(v); }
+};
+
+// ---------------------------------------------------------------- geometry
+struct Geom {
+    int nx, ny, nz;   // dims incl. ghosts
+    int m;            // margin = half order
+    int pitch;        // row pitch in elements
+    int zoff;         // elements in front of z = 0
+    int64_t sy, sx;   // strides (sy = pitch, sx = ny*pitch)
+    __host__ __device__ int64_t at(int x, int y, int z) const { return (int64_t)x * sx + (int64_t)y * sy + z; }
+};
+
+// per-field, per-level typed pointers (already offset by zoff)
+template <typename S>
+struct FieldPtrs {
+    S* f[NF][2];
+};
+
+template <typename R>
+struct MediaPtrs {
+    const R* m[OPESCI_M_COUNT];  // padded layout like the fields; null when constant
+};
+
+// coefficients, converted to the compute type
+template <typename R>
+struct Coefs {
+    R cn[3][3][MAXM];
+    R cs[3][2][MAXM];
+    R cv[3][3][MAXM];
+};
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+#define OPESCI_HIP_TRY(call)                                                                   \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return ::opesci::fail(OPESCI_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// ---------------------------------------------------------------- launch wrappers
+// interior kernels (interior.hip); variant 0 = LDS/register-queue sweep, 1 = plain gather
+struct InteriorLaunch {
+    int variant;
+    int tile_y;
+    int x_chunk;
+};
+
+hipError_t launch_stress(const Geom& g, int real_bytes, int storage, bool het, void* fields, const void* media,
+                         const void* coefs, int t0, int t1, const InteriorLaunch& cfg, hipStream_t s);
+hipError_t launch_velocity(const Geom& g, int real_bytes, int storage, bool het, void* fields, const void* media,
+                           const void* coefs, int t0, int t1, const InteriorLaunch& cfg, hipStream_t s);
+
+// auxiliary kernels (aux.hip)
+struct BcLevel {
+    int first, count, maxpts;
+};
+hipError_t launch_bc(const Geom& g, int real_bytes, int storage, void* fields, const void* media,
+                     const OpesciBcStmt* d_stmts, const OpesciBcTerm* d_terms, const OpesciExprOp* d_code,
+                     const BcLevel& lvl, int t0, int t1, hipStream_t s);
+hipError_t launch_initial(const Geom& g, int real_bytes, int storage, void* fields, const void* media,
+                          const OpesciExprOp* d_sol, const int* sol_off, const int* sol_len, const double* spacing,
+                          double dt, hipStream_t s);
+hipError_t launch_l2(const Geom& g, int real_bytes, int storage, void* fields, const void* media,
+                     const OpesciExprOp* d_sol, const int* sol_off, const int* sol_len, const double* spacing,
+                     int level, const double t_field[NF], double* d_partial, int partial_cap, double* out_host,
+                     hipStream_t s);
+hipError_t launch_media(const Geom& g, int real_bytes, const float* d_rho, const float* d_vp, const float* d_vs,
+                        void* const* d_media, hipStream_t s);
+hipError_t launch_pack(const Geom& g, int real_bytes, int storage, const void* dense, void* field, hipStream_t s);
+hipError_t launch_unpack(const Geom& g, int real_bytes, int storage, const void* field, void* dense, hipStream_t s);
+hipError_t launch_pack_media(const Geom& g, int real_bytes, const void* dense, void* media, hipStream_t s);
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
+hipError_t launch_record_source(const Geom& g, int real_bytes, int storage, void* fields, int t1, int nrec,
+                                const int32_t* d_rec, float* d_trace, int src_on, const int32_t* src_idx,
+                                const float* d_wavelet, int* d_step, int ntsteps, hipStream_t s);
+
+}  // namespace opesci

@@ -1,0 +1,698 @@
+// Context management, time loop and the C ABI of libopesci_hip.so.
+//
+// A context owns one model on the current device: 9 fields x 2 time levels
+// in the padded layout of common.h, media arrays, the boundary program and
+// the analytic-solution bytecode.  The time loop follows the reference
+// (opesci/templates/staggered3d_tmpl.py:40-58): t0 = step % 2, t1 = 1 - t0;
+// stress kernel, stress boundary launches, velocity kernel, velocity boundary
+// launches [, receivers + source].  Two steps are captured in one hipGraph
+// and replayed.
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.h"
+
+namespace opesci {
+
+static thread_local std::string g_error;
+void set_error(const std::string& msg) { g_error = msg; }
+int fail(int code, const std::string& msg) {
+    g_error = msg;
+    return code;
+}
+
+}  // namespace opesci
+
+using namespace opesci;
+
+struct OpesciContext {
+    OpesciModelDesc desc;
+    Geom g;
+    int real_bytes = 4, storage = 0, elem = 4;
+    bool het = false;
+    hipStream_t stream = nullptr;
+    void* base[NF][2] = {};
+    void* fptr[NF][2] = {};         // FieldPtrs<S> image
+    void* mbase[OPESCI_M_COUNT] = {};
+    void* mptr[OPESCI_M_COUNT] = {};  // MediaPtrs<R> image
+    Coefs<float> cf;
+    Coefs<double> cd;
+    std::vector<OpesciBcStmt> stmts;  // sorted by (phase, sched)
+    std::vector<BcLevel> levels[4];
+    OpesciBcStmt* d_stmts = nullptr;
+    OpesciBcTerm* d_terms = nullptr;
+    OpesciExprOp* d_code = nullptr;
+    OpesciExprOp* d_sol = nullptr;
+    int sol_off[NF], sol_len[NF];
+    double* d_partial = nullptr;
+    int partial_cap = 0;
+    void* d_stage = nullptr;          // dense staging buffer (one field level, real_t)
+    int32_t* d_rec = nullptr;
+    float* d_trace = nullptr;
+    float* d_wavelet = nullptr;
+    int* d_step = nullptr;
+    int nrec = 0;
+    bool src_on = false;
+    int32_t src_idx[3] = {0, 0, 0};
+    hipGraphExec_t graph = nullptr;
+    int steps = 0;
+    bool initialised = false;
+    size_t field_bytes = 0;
+    InteriorLaunch cfg{0, 0, 0};
+
+    const void* coefs() const { return real_bytes == 8 ? (const void*)&cd : (const void*)&cf; }
+};
+
+namespace {
+
+OpesciContext* g_default = nullptr;
+OpesciModelDesc g_loaded;
+bool g_have_loaded = false;
+std::vector<char> g_loaded_blob;  // deep copy of the arrays the loaded desc points to
+
+template <typename T>
+int upload(T** dst, const T* src, size_t n, hipStream_t s) {
+    if (n == 0) {
+        *dst = nullptr;
+        return 0;
+    }
+    OPESCI_HIP_TRY(hipMalloc((void**)dst, n * sizeof(T)));
+    OPESCI_HIP_TRY(hipMemcpyAsync(*dst, src, n * sizeof(T), hipMemcpyHostToDevice, s));
+    return 0;
+}
+
+// Stack discipline of a bytecode program: every op sees the operands it
+// pops, the stack never exceeds the device evaluator's 16 slots and ends with
+// exactly one value.
+bool program_ok(const OpesciExprOp* code, int n) {
+    int depth = 0;
+    for (int i = 0; i < n; ++i) {
+        const int op = code[i].op;
+        if (op == OPESCI_OP_CONST || op == OPESCI_OP_COORD || op == OPESCI_OP_TIME || op == OPESCI_OP_MEDIA) {
+            if (++depth > 16) return false;
+        } else if (op >= OPESCI_OP_ADD && op <= OPESCI_OP_DIV) {
+            if (depth < 2) return false;
+            --depth;
+        } else if (op >= OPESCI_OP_NEG && op <= OPESCI_OP_ROUNDF) {
+            if (depth < 1) return false;
+        } else {
+            return false;
+        }
+        if (op == OPESCI_OP_COORD && (code[i].a[0] < 0 || code[i].a[0] > 2)) return false;
+    }
+    return depth == 1;
+}
+
+int validate(const OpesciModelDesc* d) {
+    if (!d) return fail(OPESCI_EINVAL, "null model description");
+    if (d->abi_version != OPESCI_ABI_VERSION) return fail(OPESCI_EINVAL, "model description ABI version mismatch");
+    if (d->space_order < 2 || d->space_order > 2 * MAXM || d->space_order % 2)
+        return fail(OPESCI_EUNSUPPORTED, "space order must be 2, 4, 6 or 8");
+    const int m = d->space_order / 2;
+    for (int a = 0; a < 3; ++a)
+        if (d->dims[a] < 2 * m + 2 || d->dims[a] > 1 << 15) return fail(OPESCI_EINVAL, "grid dims out of range");
+    if (d->real_bytes != 4 && d->real_bytes != 8) return fail(OPESCI_EINVAL, "real_bytes must be 4 or 8");
+    if (d->storage < 0 || d->storage > 2) return fail(OPESCI_EINVAL, "unknown storage type");
+    if (d->storage != OPESCI_STORE_NATIVE && d->real_bytes != 4)
+        return fail(OPESCI_EUNSUPPORTED, "16-bit storage needs FP32 compute");
+    if (d->ntsteps < 0) return fail(OPESCI_EINVAL, "negative ntsteps");
+    if (d->media_arrays && (!d->rho || !d->vp || !d->vs)) return fail(OPESCI_EINVAL, "media arrays missing");
+    if (d->nstmts < 0 || d->nterms < 0 || d->ncode < 0 || d->nsol < 0)
+        return fail(OPESCI_EINVAL, "negative table size");
+    if ((d->nstmts && !d->stmts) || (d->nterms && !d->terms) || (d->ncode && !d->bc_code) || (d->nsol && !d->sol_code))
+        return fail(OPESCI_EINVAL, "null table pointer");
+    // every boundary access must stay inside the grid
+    for (int i = 0; i < d->nstmts; ++i) {
+        const OpesciBcStmt& s = d->stmts[i];
+        if (s.field < 0 || s.field >= NF || s.axis < 0 || s.axis > 2 || s.level < 0 || s.level > 1 || s.phase < 0 ||
+            s.phase > 3 || s.sched < 0 || s.term0 < 0 || s.nterms < 0 || s.term0 + s.nterms > d->nterms)
+            return fail(OPESCI_EINVAL, "malformed boundary statement");
+        const int o[2] = {s.axis == 0 ? 1 : 0, s.axis == 2 ? 1 : 2};
+        if (s.plane < 0 || s.plane >= d->dims[s.axis]) return fail(OPESCI_EINVAL, "boundary plane outside the grid");
+        for (int k = 0; k < 2; ++k)
+            if (s.hi[k] > s.lo[k] && (s.lo[k] < 0 || s.hi[k] > d->dims[o[k]]))
+                return fail(OPESCI_EINVAL, "boundary loop outside the grid");
+        for (int t = s.term0; t < s.term0 + s.nterms; ++t) {
+            const OpesciBcTerm& tm = d->terms[t];
+            if (tm.field < 0 || tm.field >= NF || tm.level < 0 || tm.level > 1 || tm.plane < 0 ||
+                tm.plane >= d->dims[s.axis])
+                return fail(OPESCI_EINVAL, "malformed boundary term");
+            for (int k = 0; k < 2; ++k)
+                if (s.hi[k] > s.lo[k] && (s.lo[k] + tm.off[k] < 0 || s.hi[k] + tm.off[k] > d->dims[o[k]]))
+                    return fail(OPESCI_EINVAL, "boundary term reads outside the grid");
+            if (tm.prog >= 0) {
+                if (tm.prog + tm.prog_len > d->ncode || !program_ok(d->bc_code + tm.prog, tm.prog_len))
+                    return fail(OPESCI_EINVAL, "boundary program out of range or malformed");
+                for (int c = tm.prog; c < tm.prog + tm.prog_len; ++c) {
+                    const OpesciExprOp& op = d->bc_code[c];
+                    if (op.op == OPESCI_OP_MEDIA) {
+                        if (!d->media_arrays || op.a[0] < 0 || op.a[0] >= OPESCI_M_COUNT)
+                            return fail(OPESCI_EINVAL, "media reference without media arrays");
+                        for (int k = 0; k < 2; ++k)
+                            if (s.hi[k] > s.lo[k] &&
+                                (s.lo[k] + op.a[1 + o[k]] < 0 || s.hi[k] + op.a[1 + o[k]] > d->dims[o[k]]))
+                                return fail(OPESCI_EINVAL, "media read outside the grid");
+                        if (op.a[1 + s.axis] < 0 || op.a[1 + s.axis] >= d->dims[s.axis])
+                            return fail(OPESCI_EINVAL, "media plane outside the grid");
+                    }
+                }
+            }
+        }
+    }
+    for (int f = 0; f < NF; ++f) {
+        if (d->sol_off[f] < 0 || d->sol_len[f] < 0 || d->sol_off[f] + d->sol_len[f] > d->nsol)
+            return fail(OPESCI_EINVAL, "analytic solution table out of range");
+        if (d->sol_len[f] > 0 && !program_ok(d->sol_code + d->sol_off[f], d->sol_len[f]))
+            return fail(OPESCI_EINVAL, "malformed analytic-solution program");
+        for (int c = d->sol_off[f]; c < d->sol_off[f] + d->sol_len[f]; ++c) {
+            const OpesciExprOp& op = d->sol_code[c];
+            if (op.op == OPESCI_OP_MEDIA) {
+                if (!d->media_arrays || op.a[0] < 0 || op.a[0] >= OPESCI_M_COUNT)
+                    return fail(OPESCI_EINVAL, "media reference without media arrays");
+                for (int k = 1; k < 4; ++k)
+                    if (op.a[k] < -m || op.a[k] > m) return fail(OPESCI_EINVAL, "media offset out of range");
+            }
+        }
+    }
+    if (d->src_on) {
+        if (!d->src_wavelet) return fail(OPESCI_EINVAL, "source without wavelet");
+        for (int a = 0; a < 3; ++a)
+            if (d->src_idx[a] < 0 || d->src_idx[a] >= d->dims[a]) return fail(OPESCI_EINVAL, "source outside grid");
+    }
+    if (d->nrec < 0 || (d->nrec && !d->rec_idx)) return fail(OPESCI_EINVAL, "bad receivers");
+    for (int r = 0; r < d->nrec; ++r)
+        for (int a = 0; a < 3; ++a)
+            if (d->rec_idx[3 * r + a] < 0 || d->rec_idx[3 * r + a] >= d->dims[a])
+                return fail(OPESCI_EINVAL, "receiver outside grid");
+    return 0;
+}
+
+int build_levels(OpesciContext* c) {
+    std::stable_sort(c->stmts.begin(), c->stmts.end(), [](const OpesciBcStmt& a, const OpesciBcStmt& b) {
+        return a.phase != b.phase ? a.phase < b.phase : a.sched < b.sched;
+    });
+    for (int i = 0; i < (int)c->stmts.size();) {
+        const OpesciBcStmt& s = c->stmts[i];
+        BcLevel lv{i, 0, 0};
+        int j = i;
+        while (j < (int)c->stmts.size() && c->stmts[j].phase == s.phase && c->stmts[j].sched == s.sched) {
+            const OpesciBcStmt& t = c->stmts[j];
+            const int n = std::max(0, t.hi[0] - t.lo[0]) * std::max(0, t.hi[1] - t.lo[1]);
+            lv.maxpts = std::max(lv.maxpts, n);
+            ++j;
+        }
+        lv.count = j - i;
+        if (lv.count > 65535) return fail(OPESCI_EUNSUPPORTED, "too many boundary statements in one launch");
+        c->levels[s.phase].push_back(lv);
+        i = j;
+    }
+    return 0;
+}
+
+int alloc_zero(OpesciContext* c, void** p, size_t bytes) {
+    OPESCI_HIP_TRY(hipMalloc(p, bytes));
+    OPESCI_HIP_TRY(launch_zero(*p, bytes, c->stream));
+    return 0;
+}
+
+int run_bc_phase(OpesciContext* c, int phase, int t0, int t1) {
+    for (const BcLevel& lv : c->levels[phase])
+        OPESCI_HIP_TRY(launch_bc(c->g, c->real_bytes, c->storage, c->fptr, c->mptr, c->d_stmts, c->d_terms, c->d_code,
+                                 lv, t0, t1, c->stream));
+    return 0;
+}
+
+int step_once(OpesciContext* c, int t0) {
+    const int t1 = 1 - t0;
+    OPESCI_HIP_TRY(launch_stress(c->g, c->real_bytes, c->storage, c->het, c->fptr, c->mptr, c->coefs(), t0, t1,
+                                 c->cfg, c->stream));
+    if (int e = run_bc_phase(c, 2, t0, t1)) return e;
+    OPESCI_HIP_TRY(launch_velocity(c->g, c->real_bytes, c->storage, c->het, c->fptr, c->mptr, c->coefs(), t0, t1,
+                                   c->cfg, c->stream));
+    if (int e = run_bc_phase(c, 3, t0, t1)) return e;
+    if (c->nrec > 0 || c->src_on)
+        OPESCI_HIP_TRY(launch_record_source(c->g, c->real_bytes, c->storage, c->fptr, t1, c->nrec, c->d_rec,
+                                            c->d_trace, c->src_on ? 1 : 0, c->src_idx, c->d_wavelet, c->d_step,
+                                            c->desc.ntsteps, c->stream));
+    return 0;
+}
+
+int build_graph(OpesciContext* c) {
+    if (c->graph) return 0;
+    hipGraph_t gr = nullptr;
+    OPESCI_HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int e = step_once(c, 0);
+    if (!e) e = step_once(c, 1);
+    hipError_t he = hipStreamEndCapture(c->stream, &gr);
+    if (e) {
+        if (gr) (void)hipGraphDestroy(gr);
+        return e;
+    }
+    if (he != hipSuccess) return fail(OPESCI_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(he));
+    he = hipGraphInstantiate(&c->graph, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (he != hipSuccess) {
+        c->graph = nullptr;
+        return fail(OPESCI_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(he));
+    }
+    return 0;
+}
+
+void destroy(OpesciContext* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    for (int f = 0; f < NF; ++f)
+        for (int l = 0; l < 2; ++l)
+            if (c->base[f][l]) (void)hipFree(c->base[f][l]);
+    for (int i = 0; i < OPESCI_M_COUNT; ++i)
+        if (c->mbase[i]) (void)hipFree(c->mbase[i]);
+    void* ptrs[] = {c->d_stmts, c->d_terms, c->d_code, c->d_sol, c->d_partial, c->d_stage,
+                    c->d_rec,   c->d_trace, c->d_wavelet, c->d_step};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int create(const OpesciModelDesc* d, OpesciContext** out) {
+    if (int e = validate(d)) return e;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(OPESCI_EHIP, "no HIP device available");
+    std::unique_ptr<OpesciContext, void (*)(OpesciContext*)> c(new OpesciContext(), destroy);
+    c->desc = *d;
+    c->real_bytes = d->real_bytes;
+    c->storage = d->storage;
+    c->elem = d->real_bytes == 8 ? 8 : (d->storage == OPESCI_STORE_NATIVE ? 4 : 2);
+    c->het = d->media_arrays != 0;
+    c->cfg.variant = d->kernel_variant;
+    c->cfg.tile_y = d->tile_y;
+    c->cfg.x_chunk = d->x_chunk;
+    OPESCI_HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    Geom& g = c->g;
+    g.nx = d->dims[0];
+    g.ny = d->dims[1];
+    g.nz = d->dims[2];
+    g.m = d->space_order / 2;
+    const int align = 128 / c->elem;  // first interior z on a 128-byte boundary
+    g.zoff = (align - g.m % align) % align;
+    g.pitch = ((g.zoff + g.nz + g.m + align - 1) / align) * align;
+    g.sy = g.pitch;
+    g.sx = (int64_t)g.ny * g.pitch;
+    c->field_bytes = (size_t)g.nx * g.sx * c->elem;
+    for (int f = 0; f < NF; ++f)
+        for (int l = 0; l < 2; ++l) {
+            if (int e = alloc_zero(c.get(), &c->base[f][l], c->field_bytes)) return e;
+            c->fptr[f][l] = (char*)c->base[f][l] + (size_t)g.zoff * c->elem;
+        }
+    const size_t rb = d->real_bytes;
+    const size_t dense = (size_t)g.nx * g.ny * g.nz;
+    OPESCI_HIP_TRY(hipMalloc(&c->d_stage, dense * std::max(rb, sizeof(float) * 3)));
+    if (c->het) {
+        const size_t mbytes = (size_t)g.nx * g.sx * rb;
+        for (int i = 0; i < OPESCI_M_COUNT; ++i) {
+            if (int e = alloc_zero(c.get(), &c->mbase[i], mbytes)) return e;
+            c->mptr[i] = (char*)c->mbase[i] + (size_t)g.zoff * rb;
+        }
+        float* st = (float*)c->d_stage;
+        OPESCI_HIP_TRY(hipMemcpyAsync(st, d->rho, dense * 4, hipMemcpyHostToDevice, c->stream));
+        OPESCI_HIP_TRY(hipMemcpyAsync(st + dense, d->vp, dense * 4, hipMemcpyHostToDevice, c->stream));
+        OPESCI_HIP_TRY(hipMemcpyAsync(st + 2 * dense, d->vs, dense * 4, hipMemcpyHostToDevice, c->stream));
+        OPESCI_HIP_TRY(launch_media(g, d->real_bytes, st, st + dense, st + 2 * dense, c->mptr, c->stream));
+    }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            for (int k = 0; k < MAXM; ++k) {
+                c->cd.cn[i][j][k] = d->cn[i][j][k];
+                c->cf.cn[i][j][k] = (float)d->cn[i][j][k];
+                c->cd.cv[i][j][k] = d->cv[i][j][k];
+                c->cf.cv[i][j][k] = (float)d->cv[i][j][k];
+            }
+    for (int s = 0; s < 3; ++s)
+        for (int gq = 0; gq < 2; ++gq)
+            for (int k = 0; k < MAXM; ++k) {
+                c->cd.cs[s][gq][k] = d->cs[s][gq][k];
+                c->cf.cs[s][gq][k] = (float)d->cs[s][gq][k];
+            }
+    c->stmts.assign(d->stmts, d->stmts + d->nstmts);
+    if (int e = build_levels(c.get())) return e;
+    if (int e = upload(&c->d_stmts, c->stmts.data(), c->stmts.size(), c->stream)) return e;
+    if (int e = upload(&c->d_terms, d->terms, (size_t)d->nterms, c->stream)) return e;
+    if (int e = upload(&c->d_code, d->bc_code, (size_t)d->ncode, c->stream)) return e;
+    if (int e = upload(&c->d_sol, d->sol_code, (size_t)d->nsol, c->stream)) return e;
+    for (int f = 0; f < NF; ++f) {
+        c->sol_off[f] = d->sol_off[f];
+        c->sol_len[f] = d->sol_len[f];
+    }
+    c->partial_cap = NF * 512;
+    OPESCI_HIP_TRY(hipMalloc((void**)&c->d_partial, c->partial_cap * sizeof(double)));
+    c->nrec = d->nrec;
+    c->src_on = d->src_on != 0;
+    if (c->nrec || c->src_on) {
+        if (int e = alloc_zero(c.get(), (void**)&c->d_step, 256)) return e;
+        if (c->nrec) {
+            if (int e = upload(&c->d_rec, d->rec_idx, (size_t)3 * c->nrec, c->stream)) return e;
+            const size_t tb = (size_t)std::max(d->ntsteps, 1) * c->nrec * 4 * sizeof(float);
+            if (int e = alloc_zero(c.get(), (void**)&c->d_trace, tb)) return e;
+        }
+        if (c->src_on) {
+            if (int e = upload(&c->d_wavelet, d->src_wavelet, (size_t)std::max(d->ntsteps, 1), c->stream)) return e;
+            for (int a = 0; a < 3; ++a) c->src_idx[a] = d->src_idx[a];
+        }
+    }
+    // pointers into the caller's arrays are not kept
+    c->desc.rho = c->desc.vp = c->desc.vs = nullptr;
+    c->desc.stmts = nullptr;
+    c->desc.terms = nullptr;
+    c->desc.bc_code = nullptr;
+    c->desc.sol_code = nullptr;
+    c->desc.src_wavelet = nullptr;
+    c->desc.rec_idx = nullptr;
+    OPESCI_HIP_TRY(hipStreamSynchronize(c->stream));
+    *out = c.release();
+    return 0;
+}
+
+int ctx_init(OpesciContext* c) {
+    for (int f = 0; f < NF; ++f)
+        for (int l = 0; l < 2; ++l) OPESCI_HIP_TRY(launch_zero(c->base[f][l], c->field_bytes, c->stream));
+    OPESCI_HIP_TRY(launch_initial(c->g, c->real_bytes, c->storage, c->fptr, c->mptr, c->d_sol, c->sol_off,
+                                  c->sol_len, c->desc.spacing, c->desc.dt, c->stream));
+    if (int e = run_bc_phase(c, 0, 0, 0)) return e;
+    if (int e = run_bc_phase(c, 1, 0, 0)) return e;
+    if (c->d_step) OPESCI_HIP_TRY(launch_zero(c->d_step, 256, c->stream));
+    OPESCI_HIP_TRY(hipStreamSynchronize(c->stream));
+    c->steps = 0;
+    c->initialised = true;
+    return 0;
+}
+
+int ctx_run(OpesciContext* c, int n, double* seconds) {
+    if (!c->initialised) return fail(OPESCI_ESTATE, "opesci_ctx_init must run before the time loop");
+    if (n < 0) return fail(OPESCI_EINVAL, "negative step count");
+    if (c->desc.use_graph && n >= 2) {
+        if (int e = build_graph(c)) return e;
+    }
+    hipEvent_t e0, e1;
+    OPESCI_HIP_TRY(hipEventCreate(&e0));
+    OPESCI_HIP_TRY(hipEventCreate(&e1));
+    OPESCI_HIP_TRY(hipEventRecord(e0, c->stream));
+    int left = n;
+    int rc = 0;
+    if (c->steps % 2 == 1 && left > 0) {
+        rc = step_once(c, 1);
+        --left;
+        ++c->steps;
+    }
+    while (!rc && left >= 2 && c->graph) {
+        hipError_t he = hipGraphLaunch(c->graph, c->stream);
+        if (he != hipSuccess) rc = fail(OPESCI_EHIP, std::string("hipGraphLaunch: ") + hipGetErrorString(he));
+        left -= 2;
+        c->steps += 2;
+    }
+    while (!rc && left > 0) {
+        rc = step_once(c, c->steps % 2);
+        --left;
+        ++c->steps;
+    }
+    OPESCI_HIP_TRY(hipEventRecord(e1, c->stream));
+    OPESCI_HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0;
+    OPESCI_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    OPESCI_HIP_TRY(hipGetLastError());
+    if (seconds) *seconds = ms * 1e-3;
+    return 0;
+}
+
+int ctx_l2(OpesciContext* c, double out[NF]) {
+    if (!c->initialised) return fail(OPESCI_ESTATE, "no state: call opesci_ctx_init first");
+    double t[NF];
+    for (int f = 0; f < NF; ++f) t[f] = c->desc.dt * c->steps + (time_stag(f) ? c->desc.dt / 2.0 : 0.0);
+    OPESCI_HIP_TRY(launch_l2(c->g, c->real_bytes, c->storage, c->fptr, c->mptr, c->d_sol, c->sol_off, c->sol_len,
+                             c->desc.spacing, c->steps % 2, t, c->d_partial, c->partial_cap, out, c->stream));
+    return 0;
+}
+
+int ctx_download(OpesciContext* c, int f, int level, void* host) {
+    if (f < 0 || f >= NF || level < -1 || level > 1 || !host) return fail(OPESCI_EINVAL, "bad download arguments");
+    const int l = level < 0 ? c->steps % 2 : level;
+    const size_t dense = (size_t)c->g.nx * c->g.ny * c->g.nz;
+    OPESCI_HIP_TRY(launch_unpack(c->g, c->real_bytes, c->storage, c->fptr[f][l], c->d_stage, c->stream));
+    OPESCI_HIP_TRY(hipMemcpyAsync(host, c->d_stage, dense * c->real_bytes, hipMemcpyDeviceToHost, c->stream));
+    OPESCI_HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int ctx_upload(OpesciContext* c, int f, int level, const void* host) {
+    if (f < 0 || f >= NF || level < 0 || level > 1 || !host) return fail(OPESCI_EINVAL, "bad upload arguments");
+    const size_t dense = (size_t)c->g.nx * c->g.ny * c->g.nz;
+    OPESCI_HIP_TRY(hipMemcpyAsync(c->d_stage, host, dense * c->real_bytes, hipMemcpyHostToDevice, c->stream));
+    OPESCI_HIP_TRY(launch_pack(c->g, c->real_bytes, c->storage, c->d_stage, c->fptr[f][level], c->stream));
+    OPESCI_HIP_TRY(hipStreamSynchronize(c->stream));
+    c->initialised = true;
+    return 0;
+}
+
+double flops_per_point(const OpesciContext* c) {
+    // reference op model (RegularGrid.get_kernel_ai): one add + one multiply
+    // per stencil point of each update: 3*3*2M (normal) + 3*2*2M (shear) + 3*3*2M (velocity)
+    const int m = c->g.m;
+    return 2.0 * (18.0 * m + 12.0 * m + 18.0 * m);
+}
+
+double interior_points(const OpesciContext* c) {
+    const int m = c->g.m;
+    return (double)(c->g.nx - 2 * m) * (c->g.ny - 2 * m) * (c->g.nz - 2 * m);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* opesci_last_error(void) { return g_error.c_str(); }
+int opesci_abi_version(void) { return OPESCI_ABI_VERSION; }
+
+int opesci_device_info(char* name, int32_t name_len, char* arch, int32_t arch_len) {
+    if (name && name_len > 0) name[0] = 0;
+    if (arch && arch_len > 0) arch[0] = 0;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(OPESCI_EHIP, "no HIP device available");
+    hipDeviceProp_t p;
+    OPESCI_HIP_TRY(hipGetDeviceProperties(&p, 0));
+    if (name && name_len > 0) {
+        strncpy(name, p.name, name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    if (arch && arch_len > 0) {
+        strncpy(arch, p.gcnArchName, arch_len - 1);
+        arch[arch_len - 1] = 0;
+    }
+    return 0;
+}
+
+int opesci_ctx_create(const OpesciModelDesc* desc, OpesciContext** out) {
+    if (!out) return fail(OPESCI_EINVAL, "null output handle");
+    *out = nullptr;
+    return create(desc, out);
+}
+
+int opesci_ctx_destroy(OpesciContext* ctx) {
+    if (ctx == g_default) g_default = nullptr;
+    destroy(ctx);
+    return 0;
+}
+
+int opesci_ctx_init(OpesciContext* ctx) {
+    if (!ctx) return fail(OPESCI_EINVAL, "null context");
+    return ctx_init(ctx);
+}
+
+int opesci_ctx_run(OpesciContext* ctx, int32_t nsteps, double* seconds) {
+    if (!ctx) return fail(OPESCI_EINVAL, "null context");
+    return ctx_run(ctx, nsteps, seconds);
+}
+
+int opesci_ctx_l2(OpesciContext* ctx, double out[OPESCI_NFIELDS]) {
+    if (!ctx || !out) return fail(OPESCI_EINVAL, "null argument");
+    return ctx_l2(ctx, out);
+}
+
+int opesci_ctx_download(OpesciContext* ctx, int32_t field, int32_t level, void* host) {
+    if (!ctx) return fail(OPESCI_EINVAL, "null context");
+    return ctx_download(ctx, field, level, host);
+}
+
+int opesci_ctx_upload(OpesciContext* ctx, int32_t field, int32_t level, const void* host) {
+    if (!ctx) return fail(OPESCI_EINVAL, "null context");
+    return ctx_upload(ctx, field, level, host);
+}
+
+int opesci_ctx_steps(OpesciContext* ctx, int32_t* steps) {
+    if (!ctx || !steps) return fail(OPESCI_EINVAL, "null argument");
+    *steps = ctx->steps;
+    return 0;
+}
+
+int opesci_ctx_receivers(OpesciContext* ctx, float* host, int64_t capacity) {
+    if (!ctx) return fail(OPESCI_EINVAL, "null context");
+    if (ctx->nrec == 0) return 0;
+    const int64_t n = (int64_t)std::min(ctx->steps, ctx->desc.ntsteps) * ctx->nrec * 4;
+    if (!host || capacity < n) return fail(OPESCI_EINVAL, "receiver buffer too small");
+    OPESCI_HIP_TRY(hipMemcpyAsync(host, ctx->d_trace, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    OPESCI_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int opesci_ctx_kernel_times(OpesciContext* ctx, int32_t nsteps, double out_ms[3]) {
+    if (!ctx || !out_ms || nsteps <= 0) return fail(OPESCI_EINVAL, "bad arguments");
+    if (!ctx->initialised) return fail(OPESCI_ESTATE, "opesci_ctx_init must run first");
+    std::vector<hipEvent_t> ev((size_t)nsteps * 5);
+    for (auto& e : ev) OPESCI_HIP_TRY(hipEventCreate(&e));
+    int rc = 0;
+    for (int s = 0; s < nsteps && !rc; ++s) {
+        const int t0 = ctx->steps % 2, t1 = 1 - t0;
+        hipEvent_t* e = &ev[(size_t)s * 5];
+        OPESCI_HIP_TRY(hipEventRecord(e[0], ctx->stream));
+        OPESCI_HIP_TRY(launch_stress(ctx->g, ctx->real_bytes, ctx->storage, ctx->het, ctx->fptr, ctx->mptr,
+                                     ctx->coefs(), t0, t1, ctx->cfg, ctx->stream));
+        OPESCI_HIP_TRY(hipEventRecord(e[1], ctx->stream));
+        rc = run_bc_phase(ctx, 2, t0, t1);
+        OPESCI_HIP_TRY(hipEventRecord(e[2], ctx->stream));
+        OPESCI_HIP_TRY(launch_velocity(ctx->g, ctx->real_bytes, ctx->storage, ctx->het, ctx->fptr, ctx->mptr,
+                                       ctx->coefs(), t0, t1, ctx->cfg, ctx->stream));
+        OPESCI_HIP_TRY(hipEventRecord(e[3], ctx->stream));
+        if (!rc) rc = run_bc_phase(ctx, 3, t0, t1);
+        if (!rc && (ctx->nrec > 0 || ctx->src_on))
+            OPESCI_HIP_TRY(launch_record_source(ctx->g, ctx->real_bytes, ctx->storage, ctx->fptr, t1, ctx->nrec,
+                                                ctx->d_rec, ctx->d_trace, ctx->src_on ? 1 : 0, ctx->src_idx,
+                                                ctx->d_wavelet, ctx->d_step, ctx->desc.ntsteps, ctx->stream));
+        OPESCI_HIP_TRY(hipEventRecord(e[4], ctx->stream));
+        ++ctx->steps;
+    }
+    OPESCI_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    double acc[3] = {0, 0, 0};
+    for (int s = 0; s < nsteps; ++s) {
+        hipEvent_t* e = &ev[(size_t)s * 5];
+        float a = 0, b = 0, c2 = 0, d2 = 0;
+        OPESCI_HIP_TRY(hipEventElapsedTime(&a, e[0], e[1]));
+        OPESCI_HIP_TRY(hipEventElapsedTime(&b, e[2], e[3]));
+        OPESCI_HIP_TRY(hipEventElapsedTime(&c2, e[1], e[2]));
+        OPESCI_HIP_TRY(hipEventElapsedTime(&d2, e[3], e[4]));
+        acc[0] += a;
+        acc[1] += b;
+        acc[2] += c2 + d2;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    for (int i = 0; i < 3; ++i) out_ms[i] = acc[i] / nsteps;
+    return rc;
+}
+
+// ---------------------------------------------------------------- reference-named ABI
+int opesci_model_load(const OpesciModelDesc* desc) {
+    if (int e = validate(desc)) return e;
+    if (g_default) {
+        destroy(g_default);
+        g_default = nullptr;
+    }
+    // keep a deep copy: the generated TU owned its tables for the process lifetime
+    const size_t dense = (size_t)desc->dims[0] * desc->dims[1] * desc->dims[2];
+    const size_t sz_st = (size_t)desc->nstmts * sizeof(OpesciBcStmt), sz_te = (size_t)desc->nterms * sizeof(OpesciBcTerm);
+    const size_t sz_co = (size_t)desc->ncode * sizeof(OpesciExprOp), sz_so = (size_t)desc->nsol * sizeof(OpesciExprOp);
+    const size_t sz_me = desc->media_arrays ? 3 * dense * sizeof(float) : 0;
+    const size_t sz_w = desc->src_on ? (size_t)std::max(desc->ntsteps, 1) * sizeof(float) : 0;
+    const size_t sz_r = (size_t)desc->nrec * 3 * sizeof(int32_t);
+    g_loaded_blob.assign(sz_st + sz_te + sz_co + sz_so + sz_me + sz_w + sz_r + 64, 0);
+    char* p = g_loaded_blob.data();
+    g_loaded = *desc;
+    auto take = [&](const void* src, size_t n) -> void* {
+        if (!n) return nullptr;
+        void* dst = p;
+        memcpy(dst, src, n);
+        p += (n + 7) & ~size_t(7);
+        return dst;
+    };
+    g_loaded.stmts = (const OpesciBcStmt*)take(desc->stmts, sz_st);
+    g_loaded.terms = (const OpesciBcTerm*)take(desc->terms, sz_te);
+    g_loaded.bc_code = (const OpesciExprOp*)take(desc->bc_code, sz_co);
+    g_loaded.sol_code = (const OpesciExprOp*)take(desc->sol_code, sz_so);
+    if (desc->media_arrays) {
+        g_loaded.rho = (const float*)take(desc->rho, dense * sizeof(float));
+        g_loaded.vp = (const float*)take(desc->vp, dense * sizeof(float));
+        g_loaded.vs = (const float*)take(desc->vs, dense * sizeof(float));
+    }
+    g_loaded.src_wavelet = desc->src_on ? (const float*)take(desc->src_wavelet, sz_w) : nullptr;
+    g_loaded.rec_idx = desc->nrec ? (const int32_t*)take(desc->rec_idx, sz_r) : nullptr;
+    g_have_loaded = true;
+    return 0;
+}
+
+int opesci_execute(OpesciGrid* grid, OpesciProfiling* profiling) {
+    if (!grid) return fail(OPESCI_EINVAL, "null grid");
+    if (!g_have_loaded) return fail(OPESCI_ESTATE, "no model loaded: call opesci_model_load first");
+    if (g_default) {
+        destroy(g_default);
+        g_default = nullptr;
+    }
+    if (int e = create(&g_loaded, &g_default)) return e;
+    OpesciContext* c = g_default;
+    if (int e = ctx_init(c)) return e;
+    double secs = 0;
+    if (int e = ctx_run(c, c->desc.ntsteps, &secs)) return e;
+    const size_t dense = (size_t)c->g.nx * c->g.ny * c->g.nz;
+    void** slots = (void**)grid;
+    for (int f = 0; f < NF; ++f) {
+        void* h = nullptr;
+        if (posix_memalign(&h, 4096, 2 * dense * c->real_bytes) != 0 || !h)
+            return fail(OPESCI_ENOMEM, "host allocation for the result fields failed");
+        for (int l = 0; l < 2; ++l)
+            if (int e = ctx_download(c, f, l, (char*)h + (size_t)l * dense * c->real_bytes)) {
+                free(h);
+                return e;
+            }
+        slots[f] = h;
+    }
+    if (profiling) {
+        profiling->g_rtime = (float)secs;
+        profiling->g_ptime = (float)secs;
+        profiling->g_mflops =
+            secs > 0 ? (float)(flops_per_point(c) * interior_points(c) * c->desc.ntsteps / secs * 1e-6) : 0.f;
+    }
+    return 0;
+}
+
+int opesci_convergence(OpesciGrid* grid, void* conv) {
+    if (!grid || !conv) return fail(OPESCI_EINVAL, "null argument");
+    if (!g_default || !g_default->initialised)
+        return fail(OPESCI_ESTATE, "convergence before execute: call opesci_execute first");
+    double l2[NF];
+    if (int e = ctx_l2(g_default, l2)) return e;
+    for (int f = 0; f < NF; ++f) {
+        if (g_default->real_bytes == 8)
+            ((double*)conv)[f] = l2[f];
+        else
+            ((float*)conv)[f] = (float)l2[f];
+    }
+    return 0;
+}
+
+int opesci_free(OpesciGrid* grid) {
+    if (!grid) return fail(OPESCI_EINVAL, "null grid");
+    void** slots = (void**)grid;
+    for (int f = 0; f < NF; ++f) {
+        free(slots[f]);
+        slots[f] = nullptr;
+    }
+    return 0;
+}
+
+}  // extern "C"

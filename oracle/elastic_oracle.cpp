@@ -63,6 +63,7 @@ typedef struct OracleParams {
 int oracle_run(const OracleParams* p, void* fields_out[9], double l2_f64[9],
                double l2_ref[9], double timing[2]);
 int oracle_fd_weights(int half_order, double* w);
+int oracle_norms(const OracleParams* p, void* const fields_in[9], double l2_f64[9], double l2_ref[9]);
 }
 
 namespace {
@@ -963,6 +964,26 @@ extern "C" int oracle_run(const OracleParams* p, void* fields_out[9], double l2_
     if (p->nthreads > 0) omp_set_num_threads(p->nthreads);
     if (p->real_bytes == 8) return run_typed<double>(p, fields_out, l2_f64, l2_ref, timing);
     return run_typed<float>(p, fields_out, l2_f64, l2_ref, timing);
+}
+
+/* L2 norms of externally produced final fields (dense [dim1][dim2][dim3]
+ * real_t, the level after ntsteps) with the same conventions as oracle_run. */
+template <typename R>
+int norms_typed(const OracleParams* p, void* const in[9], double l2_f64[9], double l2_ref[9]) {
+    OracleParams q = *p;
+    q.ic_mode = 1;
+    Sim<R> s(q);
+    int lvl = p->ntsteps % 2;
+    for (int f = 0; f < NF; ++f) std::memcpy(s.F[f].data() + (size_t)lvl * s.vol, in[f], s.vol * sizeof(R));
+    s.norms(l2_f64, l2_ref);
+    return 0;
+}
+
+extern "C" int oracle_norms(const OracleParams* p, void* const fields_in[9], double l2_f64[9], double l2_ref[9]) {
+    if (!p || !fields_in) return 1;
+    if (p->media_arrays && (!p->rho_arr || !p->vp_arr || !p->vs_arr)) return 4;
+    if (p->real_bytes == 8) return norms_typed<double>(p, fields_in, l2_f64, l2_ref);
+    return norms_typed<float>(p, fields_in, l2_f64, l2_ref);
 }
 
 extern "C" int oracle_fd_weights(int half_order, double* w) {

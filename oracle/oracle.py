@@ -68,6 +68,9 @@ def load(native=False):
         lib.oracle_run.argtypes = [ctypes.POINTER(OracleParams), ctypes.c_void_p * 9,
                                    ctypes.c_double * 9, ctypes.c_double * 9,
                                    ctypes.c_double * 2]
+        lib.oracle_norms.restype = ctypes.c_int
+        lib.oracle_norms.argtypes = [ctypes.POINTER(OracleParams), ctypes.c_void_p * 9,
+                                     ctypes.c_double * 9, ctypes.c_double * 9]
         lib.oracle_fd_weights.restype = ctypes.c_int
         lib.oracle_fd_weights.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         _LIBS[name] = lib
@@ -83,6 +86,47 @@ def fd_weights(half_order):
 def media_from_velocities(rho, vp, vs):
     """StaggeredGrid.set_media_params(read=False) (opesci/staggeredgrid.py:279-282)."""
     return rho, rho * (vp ** 2 - 2 * vs ** 2), rho * vs ** 2
+
+
+def _params(order, n, ntsteps, dt, L, double, rho, lam, mu, vp, vs, snapshot, nthreads, bc_algo,
+            ic_mode):
+    if lam is None or mu is None:
+        _, lam, mu = media_from_velocities(rho, vp, vs)
+    p = OracleParams()
+    p.order = order
+    p.n[:] = n
+    p.ntsteps = ntsteps
+    p.real_bytes = 8 if double else 4
+    p.snapshot = 1 if snapshot else 0
+    p.nthreads = nthreads
+    p.bc_algo = bc_algo
+    p.ic_mode = ic_mode
+    p.L[:] = L
+    p.dt = dt
+    p.rho, p.lam, p.mu = rho, lam, mu
+    return p
+
+
+def norms(fields, order=4, n=(16, 16, 16), ntsteps=10, dt=None, L=(1.0, 1.0, 1.0), double=False,
+          rho=1.0, lam=None, mu=None, vp=1.0, vs=0.5, snapshot=False, media=None):
+    """L2 norms (f64-accumulated and reference-style) of externally computed
+    final fields with the oracle's conventions; returns (l2_f64, l2_ref)."""
+    p = _params(order, tuple(n), ntsteps, dt, L, double, rho, lam, mu, vp, vs, snapshot, 0, -1, 1)
+    keep = []
+    if media is not None:
+        arrs = [np.ascontiguousarray(a, dtype=np.float32).reshape(-1) for a in media]
+        keep += arrs
+        p.media_arrays = 1
+        p.rho_arr, p.vp_arr, p.vs_arr = [a.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) for a in arrs]
+    dtype = np.float64 if double else np.float32
+    fs = [np.ascontiguousarray(f, dtype=dtype) for f in fields]
+    ptrs = (ctypes.c_void_p * 9)(*[f.ctypes.data for f in fs])
+    l2 = (ctypes.c_double * 9)()
+    l2r = (ctypes.c_double * 9)()
+    rc = load().oracle_norms(ctypes.byref(p), ptrs, l2, l2r)
+    if rc != 0:
+        raise RuntimeError("oracle_norms failed with code %d" % rc)
+    return list(l2), list(l2r)
 
 
 def run(order=4, n=(16, 16, 16), ntsteps=10, dt=None, L=(1.0, 1.0, 1.0), double=False,

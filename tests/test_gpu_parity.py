@@ -1,0 +1,189 @@
+"""GPU parity: libopesci_hip (through its C ABI) against the plan interpreter
+(same arithmetic contract) and the oracle (independent restatement of the
+reference), plus the reference's own known answer.
+
+Tolerances (stated here, per north star: reference L2-vs-analytic within 1e-5
+relative):
+  * fields vs plan interpreter: FP64 1e-12, FP32 2e-6 of the field maximum
+  * L2 vs oracle: FP64 1e-10 relative, FP32 1e-5 relative
+  * test3d known answer: 1e-5 relative (the reference's own -O3 vs
+    -march=native builds differ at ~1e-6)
+"""
+import numpy as np
+import pytest
+
+import oracle
+from opesci.backend import Context
+from planvm import PlanVM
+from workloads import eigen_model, hetero_model
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = oracle.FIELDS
+
+
+def gpu_run(plan, steps=None):
+    with Context(plan) as ctx:
+        ctx.init()
+        ctx.run(steps)
+        return ctx.fields(), ctx.l2()
+
+
+def max_rel(a, b, scale):
+    return float(np.abs(a.astype(np.float64) - b.astype(np.float64)).max() / scale)
+
+
+@pytest.mark.parametrize("order,double", [(2, True), (2, False), (4, True), (4, False), (6, True),
+                                          (8, True), (8, False)])
+def test_fields_match_plan_interpreter(order, double):
+    g, kw = eigen_model(21, order, 15, double=double)
+    plan = g.lower()
+    fields, _ = gpu_run(plan)
+    vm = PlanVM(plan)
+    vm.init()
+    vm.run()
+    ref = vm.latest()
+    scale = np.abs(ref[0]).max()
+    tol = 1e-12 if double else 2e-6
+    for f in range(9):
+        assert max_rel(fields[f], ref[f], scale) < tol, (FIELDS[f], max_rel(fields[f], ref[f], scale))
+
+
+@pytest.mark.parametrize("order,double", [(2, True), (4, True), (4, False), (8, True), (8, False)])
+def test_l2_matches_oracle(order, double):
+    g, kw = eigen_model(25, order, 24, double=double)
+    plan = g.lower()
+    fields, l2 = gpu_run(plan)
+    ref = oracle.run(**kw)
+    tol = 1e-10 if double else 1e-5
+    for f in range(9):
+        assert abs(l2[f] - ref['l2'][f]) <= tol * abs(ref['l2'][f]) + 1e-14, (FIELDS[f], l2[f], ref['l2'][f])
+    scale = np.abs(ref['fields'][0]).max()
+    for f in range(9):
+        assert max_rel(fields[f], ref['fields'][f], scale) < (1e-11 if double else 1e-5), FIELDS[f]
+
+
+def test_initial_state_matches_oracle():
+    g, kw = eigen_model(17, 4, 0, double=True)
+    fields, l2 = gpu_run(g.lower(), 0)
+    kw['ntsteps'] = 0
+    ref = oracle.run(**kw)
+    for f in range(9):
+        assert max_rel(fields[f], ref['fields'][f], 1.0) < 1e-14, FIELDS[f]
+    for f in range(9):
+        assert l2[f] < 1e-12
+
+
+@pytest.mark.parametrize("order", [4, 8])
+def test_heterogeneous_matches_oracle(order):
+    g, kw = hetero_model(21, order, 12, double=True, sigma=2.0)
+    plan = g.lower()
+    fields, _ = gpu_run(plan)
+    ref = oracle.run(**kw)
+    scale = np.abs(ref['fields'][0]).max()
+    for f in range(9):
+        assert max_rel(fields[f], ref['fields'][f], scale) < 1e-9, FIELDS[f]
+    vm = PlanVM(plan)
+    vm.init()
+    vm.run()
+    for f in range(9):
+        assert max_rel(fields[f], vm.latest()[f], scale) < 1e-12, FIELDS[f]
+
+
+def test_heterogeneous_fp32_matches_oracle():
+    g, kw = hetero_model(33, 4, 30, double=False, sigma=3.0)
+    fields, _ = gpu_run(g.lower())
+    ref = oracle.run(**kw)
+    for f in range(9):
+        a, b = fields[f].astype(np.float64), ref['fields'][f]
+        rel_l2 = np.sqrt(np.sum((a - b) ** 2) / max(np.sum(b ** 2), 1e-300))
+        assert rel_l2 < 1e-5, (FIELDS[f], rel_l2)
+
+
+def test_known_answer_test3d(fixtures):
+    """benchmarks/test3d.cpp: 4th order, 100 cells, dt=0.002, 1000 steps,
+    mu = lambda = 0.5, float, older surfaces.  GPU fields, reference-style
+    (serial real_t) L2 accumulation."""
+    ref = fixtures['test3d_known_answer']['l2']
+    g, kw = eigen_model(101, 4, 1000, legacy=True, lam=0.5, mu=0.5, rho=1.0, dt=0.002)
+    plan = g.lower()
+    assert plan.ntsteps == 1000 and plan.dims == (105, 105, 105)
+    fields, l2 = gpu_run(plan)
+    _, l2_ref = oracle.norms(fields, order=4, n=(100, 100, 100), ntsteps=1000, dt=0.002, lam=0.5, mu=0.5,
+                             snapshot=True)
+    for f, name in enumerate(FIELDS):
+        assert abs(l2_ref[f] - ref[name]) <= 1e-5 * ref[name], (name, l2_ref[f], ref[name])
+
+
+def test_graph_replay_equals_direct_steps():
+    g, _ = eigen_model(21, 4, 11)
+    plan = g.lower()
+    plan.use_graph = 1
+    a, _ = gpu_run(plan, 11)
+    plan.use_graph = 0
+    with Context(plan) as ctx:
+        ctx.init()
+        for _ in range(11):
+            ctx.run(1)
+        b = ctx.fields()
+    for f in range(9):
+        assert np.array_equal(a[f], b[f]), FIELDS[f]
+
+
+def test_run_in_chunks_equals_one_run():
+    g, _ = eigen_model(21, 8, 13)
+    plan = g.lower()
+    a, _ = gpu_run(plan, 13)
+    with Context(plan) as ctx:
+        ctx.init()
+        ctx.run(7)
+        ctx.run(6)
+        assert ctx.steps() == 13
+        b = ctx.fields()
+    for f in range(9):
+        assert np.array_equal(a[f], b[f])
+
+
+@pytest.mark.parametrize("storage", ["native", "fp16", "bf16"])
+def test_upload_download_roundtrip(storage):
+    g, _ = eigen_model(17, 4, 0, storage=storage)
+    plan = g.lower()
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(plan.dims).astype(np.float32)
+    with Context(plan) as ctx:
+        ctx.upload(3, 1, x)
+        y = ctx.download(3, 1)
+    if storage == 'native':
+        assert np.array_equal(x, y)
+    else:
+        tol = 2 ** -11 if storage == 'fp16' else 2 ** -8
+        assert np.all(np.abs(y - x) <= tol * np.abs(x) + 1e-30)
+
+
+def test_reference_abi_execute_convergence(capsys):
+    """Grid.execute / Grid.convergence through opesci_model_load /
+    opesci_execute / opesci_convergence / opesci_free (opesci/grid.py:85-156)."""
+    g, kw = eigen_model(17, 4, 10, double=True)
+    g.execute('model.h', compiler='hip')
+    conv = g.convergence()
+    U = g.field_array('U')
+    assert U.shape == (2,) + tuple(g.lower().dims)
+    ref = oracle.run(**kw)
+    assert np.allclose(U[10 % 2], ref['fields'][0], rtol=0, atol=1e-13)
+    for f, name in enumerate(FIELDS):
+        assert abs(conv[name + '_l2'] - ref['l2'][f]) <= 1e-10 * ref['l2'][f] + 1e-15
+    g.free()
+    out = capsys.readouterr().out
+    assert 'Convergence:' in out and 'U_l2:' in out
+
+
+def test_c1_second_order_fp64_64cube():
+    """Plain eigenwave, 2nd order, 64^3 interior, FP64, rho = lambda = mu = 1,
+    dt = 0.5 dx / sqrt(3 (lambda + 2 mu)/rho), 100 steps."""
+    g, kw = eigen_model(64, 2, 100, double=True)
+    plan = g.lower()
+    assert plan.dims == (66, 66, 66)
+    fields, l2 = gpu_run(plan)
+    ref = oracle.run(**kw)
+    for f in range(9):
+        assert abs(l2[f] - ref['l2'][f]) <= 1e-10 * ref['l2'][f] + 1e-15, FIELDS[f]
