@@ -43,7 +43,8 @@ typedef struct OracleParams {
     int32_t media_arrays; /* 0: constant media; 1: rho/vp/vs arrays of dim1*dim2*dim3 floats */
     int32_t snapshot;     /* 1: benchmarks/test3d.cpp conventions */
     int32_t nthreads;     /* OpenMP threads (0: runtime default) */
-    int32_t bc_algo;      /* -1: reference rule (levander iff order==4); 0: robertsson; 1: levander */
+    int32_t bc_algo;      /* -1: reference rule (levander iff order==4); 0: robertsson; 1: levander;
+                             2: levander without the normal-stress recompute (older generator) */
     int32_t ic_mode;      /* 0: analytic eigenmode (tests/eigenwave3d.py:87-108); 1: all zero */
     int32_t src_on;       /* 1: inject src_wavelet at src_idx into Txx/Tyy/Tzz */
     int32_t src_idx[3];
@@ -102,6 +103,7 @@ struct Sim {
     int d[3];              /* dim1..dim3 */
     size_t vol;
     bool levander;
+    bool no_tnn = false;
     R dx[3];               /* const real_t dx1..dx3 */
     R dt;
     double dxd[3], dtd;
@@ -121,7 +123,8 @@ struct Sim {
         M = p.order / 2;
         for (int a = 0; a < 3; ++a) d[a] = p.n[a] + 1 + 2 * M;
         vol = (size_t)d[0] * d[1] * d[2];
-        levander = p.bc_algo < 0 ? (p.order == 4) : (p.bc_algo == 1);
+        levander = p.bc_algo < 0 ? (p.order == 4) : (p.bc_algo >= 1);
+        no_tnn = p.bc_algo == 2;
         for (int a = 0; a < 3; ++a) {
             dxd[a] = p.L[a] / (double)p.n[a];  /* regulargrid.py:194-196 */
             dx[a] = (R)dxd[a];
@@ -639,7 +642,7 @@ struct Sim {
             for (int dax = 0; dax < 3; ++dax) {
                 bool own = normal ? (f - TXX == dax) : (STAG[f][dax] != 0);
                 if (normal && !own) {
-                    if (init || !levander || P.snapshot) continue;
+                    if (init || !levander || P.snapshot || no_tnn) continue;
                     for (int side = 0; side < 2; ++side) stress_tnn(f, dax, side, t0, t1);
                     continue;
                 }
