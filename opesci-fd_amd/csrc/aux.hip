@@ -49,45 +49,48 @@ __device__ __forceinline__ double coord_of(double dx, int i, int m, int stagger)
 }
 
 // ------------------------------------------------------------------ boundary program
-// One launch per hazard level: blockIdx.y picks the statement, threads cover
-// its plane loop; statements of one level write disjoint cells and never
-// read what another statement of the level writes.
+// One launch per hazard level, blockIdx.y = statement.  Statements of a
+// level write disjoint cells and never read what another statement of the
+// level writes.  Terms are loaded independently and summed in term order.
 template <typename R, typename S>
 __global__ __launch_bounds__(256) void bc_kernel(Geom g, FieldPtrs<S> F, MediaPtrs<R> md,
-                                                 const OpesciBcStmt* __restrict__ stmts,
-                                                 const OpesciBcTerm* __restrict__ terms,
+                                                 const BcDesc* __restrict__ desc,
                                                  const OpesciExprOp* __restrict__ code, int first, int t0, int t1) {
     using C = Conv<S, R>;
-    const OpesciBcStmt s = stmts[first + blockIdx.y];
-    const int n0 = s.hi[0] - s.lo[0], n1 = s.hi[1] - s.lo[1];
+    const BcDesc* D = desc + first + blockIdx.y;
+    const int npts = D->npts;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n0 <= 0 || n1 <= 0 || p >= n0 * n1) return;
-    const int i0 = s.lo[0] + p / n1, i1 = s.lo[1] + p % n1;
-    const int o0 = s.axis == 0 ? 1 : 0, o1 = s.axis == 2 ? 1 : 2;
-    R acc = (R)0;
-    for (int k = 0; k < s.nterms; ++k) {
-        const OpesciBcTerm t = terms[s.term0 + k];
-        int r[3];
-        r[s.axis] = t.plane;
-        r[o0] = i0 + t.off[0];
-        r[o1] = i1 + t.off[1];
-        const R v = C::ld(F.f[t.field][t.level ? t1 : t0] + g.at(r[0], r[1], r[2]));
-        R coef = (R)t.coef;
-        if (t.prog >= 0) {
-            int a[3];
-            a[s.axis] = 0;
-            a[o0] = i0;
-            a[o1] = i1;
-            const double c0[3] = {0, 0, 0};
-            coef = (R)(t.coef * vm_eval<R>(code + t.prog, t.prog_len, c0, 0.0, md, g, a[0], a[1], a[2]));
+    if (p >= npts) return;
+    const int n1 = D->n1, axis = D->axis, nt = D->nterms;
+    const int i0 = D->lo0 + p / n1, i1 = D->lo1 + p % n1;
+    const int o0 = axis == 0 ? 1 : 0, o1 = axis == 2 ? 1 : 2;
+    const int64_t so0 = o0 == 0 ? g.sx : g.sy, so1 = o1 == 1 ? g.sy : 1;
+    const int64_t dst = (int64_t)i0 * so0 + (int64_t)i1 * so1 + D->disp;
+    R v[BC_MAXT];
+#pragma unroll
+    for (int k = 0; k < BC_MAXT; ++k)
+        if (k < nt) {
+            const int fl = D->t[k].fl;
+            v[k] = C::ld(F.f[fl >> 1][(fl & 1) ? t1 : t0] + dst + D->t[k].disp);
         }
-        acc = fma(coef, v, acc);
-    }
-    int q[3];
-    q[s.axis] = s.plane;
-    q[o0] = i0;
-    q[o1] = i1;
-    C::st(F.f[s.field][s.level ? t1 : t0] + g.at(q[0], q[1], q[2]), acc);
+    R acc = (R)0;
+#pragma unroll
+    for (int k = 0; k < BC_MAXT; ++k)
+        if (k < nt) {
+            R coef = (R)D->t[k].coef;
+            if (D->t[k].prog >= 0) {
+                int a[3];
+                a[axis] = 0;
+                a[o0] = i0;
+                a[o1] = i1;
+                const double c0[3] = {0, 0, 0};
+                coef = (R)((double)D->t[k].coef *
+                           vm_eval<R>(code + D->t[k].prog, D->prog_len[k], c0, 0.0, md, g, a[0], a[1], a[2]));
+            }
+            acc = fma(coef, v[k], acc);
+        }
+    const int fl = D->fl;
+    C::st(F.f[fl >> 1][(fl & 1) ? t1 : t0] + dst, acc);
 }
 
 struct SolTable {
@@ -257,12 +260,12 @@ __global__ __launch_bounds__(256) void record_source_kernel(Geom g, FieldPtrs<S>
 
 // ------------------------------------------------------------------ launchers
 template <typename R, typename S>
-static hipError_t bc_t(const Geom& g, void* fields, const void* media, const OpesciBcStmt* st, const OpesciBcTerm* te,
-                       const OpesciExprOp* code, const BcLevel& lv, int t0, int t1, hipStream_t s) {
+static hipError_t bc_t(const Geom& g, void* fields, const void* media, const BcDesc* desc, const OpesciExprOp* code,
+                       const BcLevel& lv, int t0, int t1, hipStream_t s) {
     if (lv.count <= 0 || lv.maxpts <= 0) return hipSuccess;
     dim3 grid((lv.maxpts + 255) / 256, lv.count);
     hipLaunchKernelGGL((bc_kernel<R, S>), grid, dim3(256), 0, s, g, *static_cast<const FieldPtrs<S>*>(fields),
-                       *static_cast<const MediaPtrs<R>*>(media), st, te, code, lv.first, t0, t1);
+                       *static_cast<const MediaPtrs<R>*>(media), desc, code, lv.first, t0, t1);
     return hipGetLastError();
 }
 
@@ -270,13 +273,12 @@ static hipError_t bc_t(const Geom& g, void* fields, const void* media, const Ope
     ((RB) == 8 ? (CALL_D) : (ST) == OPESCI_STORE_F16 ? (CALL_H) : (ST) == OPESCI_STORE_BF16 ? (CALL_B) : (CALL_F))
 
 hipError_t launch_bc(const Geom& g, int real_bytes, int storage, void* fields, const void* media,
-                     const OpesciBcStmt* d_stmts, const OpesciBcTerm* d_terms, const OpesciExprOp* d_code,
-                     const BcLevel& lvl, int t0, int t1, hipStream_t s) {
-    return OPESCI_TYPED(real_bytes, storage,
-                        (bc_t<float, float>(g, fields, media, d_stmts, d_terms, d_code, lvl, t0, t1, s)),
-                        (bc_t<double, double>(g, fields, media, d_stmts, d_terms, d_code, lvl, t0, t1, s)),
-                        (bc_t<float, _Float16>(g, fields, media, d_stmts, d_terms, d_code, lvl, t0, t1, s)),
-                        (bc_t<float, bf16_t>(g, fields, media, d_stmts, d_terms, d_code, lvl, t0, t1, s)));
+                     const BcDesc* d_desc, const OpesciExprOp* d_code, const BcLevel& lvl, int t0, int t1,
+                     hipStream_t s) {
+    return OPESCI_TYPED(real_bytes, storage, (bc_t<float, float>(g, fields, media, d_desc, d_code, lvl, t0, t1, s)),
+                        (bc_t<double, double>(g, fields, media, d_desc, d_code, lvl, t0, t1, s)),
+                        (bc_t<float, _Float16>(g, fields, media, d_desc, d_code, lvl, t0, t1, s)),
+                        (bc_t<float, bf16_t>(g, fields, media, d_desc, d_code, lvl, t0, t1, s)));
 }
 
 static SolTable make_tab(const int* off, const int* len, const double* dx) {

@@ -130,12 +130,34 @@ hipError_t launch_velocity(const Geom& g, int real_bytes, int storage, bool het,
                            const void* coefs, int t0, int t1, const InteriorLaunch& cfg, hipStream_t s);
 
 // auxiliary kernels (aux.hip)
+// Boundary program as the device runs it: one self-contained descriptor per
+// statement (fixed stride, terms inlined), grouped by hazard level, so a
+// thread does one descriptor load and then independent value loads.  Reads
+// are the destination's linear index plus a displacement.  Coefficients are
+// float literals (the generator rounds every folded constant to float), so a
+// float holds them exactly in either precision.
+constexpr int BC_MAXT = 16;
+struct BcTermDev {
+    int32_t fl;      // field*2 + level
+    int32_t disp;    // source - destination linear offset
+    float coef;
+    int32_t prog;    // -1 or offset of a media program (length in prog_len)
+};
+struct BcDesc {
+    int32_t fl;      // destination field*2 + level
+    int32_t disp;    // plane * stride(axis)
+    int32_t nterms;
+    int32_t axis;
+    int32_t lo0, lo1, n1, npts;
+    int32_t prog_len[BC_MAXT];
+    BcTermDev t[BC_MAXT];
+};
 struct BcLevel {
-    int first, count, maxpts;
+    int first, count, maxpts;   // descriptors [first, first+count)
 };
 hipError_t launch_bc(const Geom& g, int real_bytes, int storage, void* fields, const void* media,
-                     const OpesciBcStmt* d_stmts, const OpesciBcTerm* d_terms, const OpesciExprOp* d_code,
-                     const BcLevel& lvl, int t0, int t1, hipStream_t s);
+                     const BcDesc* d_desc, const OpesciExprOp* d_code, const BcLevel& lvl, int t0, int t1,
+                     hipStream_t s);
 hipError_t launch_initial(const Geom& g, int real_bytes, int storage, void* fields, const void* media,
                           const OpesciExprOp* d_sol, const int* sol_off, const int* sol_len, const double* spacing,
                           double dt, hipStream_t s);

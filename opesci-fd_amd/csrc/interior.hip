@@ -132,49 +132,407 @@ __global__ __launch_bounds__(256) void velocity_gather(Geom g, FieldPtrs<S> F, M
     }
 }
 
-// ------------------------------------------------------------------ dispatch
-template <typename R, typename S, int M, bool HET>
-static hipError_t run_stress(const Geom& g, void* fields, const void* media, const void* coefs, int t0, int t1,
-                             const InteriorLaunch& cfg, hipStream_t s) {
-    const FieldPtrs<S>& F = *static_cast<const FieldPtrs<S>*>(fields);
-    const MediaPtrs<R>& md = *static_cast<const MediaPtrs<R>*>(media);
-    const Coefs<R>& c = *static_cast<const Coefs<R>*>(coefs);
-    const int ni[3] = {g.nx - 2 * M, g.ny - 2 * M, g.nz - 2 * M};
-    const int xchunk = cfg.x_chunk > 0 ? cfg.x_chunk : 16;
-    const int tz = (ni[2] + 63) / 64, ty = (ni[1] + 3) / 4, tx = (ni[0] + xchunk - 1) / xchunk;
-    const int nb = tz * ty * tx;
-    const int grid = ((nb + 7) / 8) * 8;
-    hipLaunchKernelGGL((stress_gather<R, S, M, HET>), dim3(grid), dim3(256), 0, s, g, F, md, c, t0, t1, tz, ty,
-                       xchunk, nb);
-    return hipGetLastError();
+// ------------------------------------------------------------------ sweep
+// LDS plane of a 64 x TY tile with an M-cell halo.  Halo cells: the y-halo
+// (M rows above and below, tile columns only) and the z-halo (M columns left
+// and right, tile rows only); corners are never read.
+template <int M, int TY>
+struct Plane {
+    static constexpr int W = 64 + 2 * M;
+    static constexpr int H = TY + 2 * M;
+    static constexpr int NT = 64 * TY;
+    static constexpr int NY = 2 * M * 64;   // y-halo cells
+    static constexpr int NZ = TY * 2 * M;   // z-halo cells
+    static constexpr int JY = (NY + NT - 1) / NT;
+    static_assert(NZ <= NT, "z halo larger than the block");
+    __device__ __forceinline__ static void yhalo(int h, int& row, int& col) {
+        const int r = h / 64;
+        row = r < M ? r : r + TY;
+        col = M + h % 64;
+    }
+    __device__ __forceinline__ static void zhalo(int h, int& row, int& col) {
+        row = M + h / (2 * M);
+        const int c = h % (2 * M);
+        col = c < M ? c : 64 + c;
+    }
+};
+
+// Per-thread halo slots: LDS offset and in-plane global offset of every halo
+// cell this thread stages (-1 = none).
+template <int M, int TY>
+struct HaloSlots {
+    using P = Plane<M, TY>;
+    int ly[P::JY], gy[P::JY];
+    int lz, gz;
+    __device__ void init(const Geom& g, int y0, int z0) {
+        const int t = threadIdx.x;
+#pragma unroll
+        for (int j = 0; j < P::JY; ++j) {
+            const int h = t + j * P::NT;
+            if (h < P::NY) {
+                int row, cl;
+                P::yhalo(h, row, cl);
+                const int yy = min(max(y0 - M + row, 0), g.ny - 1), zz = min(max(z0 - M + cl, 0), g.nz - 1);
+                ly[j] = row * P::W + cl;
+                gy[j] = yy * (int)g.sy + zz;
+            } else {
+                ly[j] = -1;
+                gy[j] = 0;
+            }
+        }
+        if (t < P::NZ) {
+            int row, cl;
+            P::zhalo(t, row, cl);
+            const int yy = min(max(y0 - M + row, 0), g.ny - 1), zz = min(max(z0 - M + cl, 0), g.nz - 1);
+            lz = row * P::W + cl;
+            gz = yy * (int)g.sy + zz;
+        } else {
+            lz = -1;
+            gz = 0;
+        }
+    }
+};
+
+// Values staged for one plane: y-halo and z-halo registers.
+template <typename R, int M, int TY, bool HY, bool HZ>
+struct Staged {
+    using P = Plane<M, TY>;
+    R y[HY ? P::JY : 1];
+    R z;
+    template <typename S>
+    __device__ __forceinline__ void fetch(const S* base, int64_t px, const HaloSlots<M, TY>& hs) {
+        using C = Conv<S, R>;
+        if (HY) {
+#pragma unroll
+            for (int j = 0; j < P::JY; ++j)
+                if (hs.ly[j] >= 0) y[j] = C::ld(base + px + hs.gy[j]);
+        }
+        if (HZ && hs.lz >= 0) z = C::ld(base + px + hs.gz);
+    }
+    __device__ __forceinline__ void put(R* lds, const HaloSlots<M, TY>& hs) const {
+        if (HY) {
+#pragma unroll
+            for (int j = 0; j < P::JY; ++j)
+                if (hs.ly[j] >= 0) lds[hs.ly[j]] = y[j];
+        }
+        if (HZ && hs.lz >= 0) lds[hs.lz] = z;
+    }
+};
+
+template <typename R, typename S, int M, int TY, bool HET>
+__global__ __launch_bounds__(64 * TY) void stress_sweep(Geom g, FieldPtrs<S> F, MediaPtrs<R> md, Coefs<R> c,
+                                                        int t0, int t1, int tz, int ty, int xchunk, int nblocks) {
+    using P = Plane<M, TY>;
+    using C = Conv<S, R>;
+    __shared__ R sU[2][P::H * P::W], sV[2][P::H * P::W], sW[2][P::H * P::W];
+    const int l = logical_block(nblocks);
+    if (l < 0) return;
+    const int bz = l % tz, by = (l / tz) % ty, bx = l / (tz * ty);
+    const int lz = threadIdx.x & 63, ly = threadIdx.x >> 6;
+    const int z0 = M + bz * 64, y0 = M + by * TY;
+    const int z = z0 + lz, y = y0 + ly;
+    const bool active = z < g.nz - M && y < g.ny - M;
+    const int64_t col = (int64_t)min(y, g.ny - 1) * g.sy + min(z, g.nz - 1);
+    const int xb = M + bx * xchunk, xe = min(xb + xchunk, g.nx - M);
+    const S* U0 = F.f[0][t0];
+    const S* V0 = F.f[1][t0];
+    const S* W0 = F.f[2][t0];
+    HaloSlots<M, TY> hs;
+    hs.init(g, y0, z0);
+    // register queues along x: qU[i] = U(x+i-M), qV[i] = V(x+i-M+1), qW[i] = W(x+i-M+1)
+    R qU[2 * M], qV[2 * M], qW[2 * M];
+#pragma unroll
+    for (int i = 1; i < 2 * M; ++i) {
+        qU[i] = C::ld(U0 + (int64_t)(xb - M - 1 + i) * g.sx + col);
+        qV[i] = C::ld(V0 + (int64_t)(xb - M + i) * g.sx + col);
+        qW[i] = C::ld(W0 + (int64_t)(xb - M + i) * g.sx + col);
+    }
+    // values of the next plane, fetched one iteration ahead
+    R nU, nV, nW, nT[6];
+    Staged<R, M, TY, true, true> nhU, nhV, nhW;
+    auto fetch = [&](int x) {
+        const int64_t px = (int64_t)x * g.sx;
+        nU = C::ld(U0 + px + (int64_t)(M - 1) * g.sx + col);
+        nV = C::ld(V0 + px + (int64_t)M * g.sx + col);
+        nW = C::ld(W0 + px + (int64_t)M * g.sx + col);
+#pragma unroll
+        for (int f = 0; f < 6; ++f) nT[f] = C::ld(F.f[3 + f][t0] + px + col);
+        nhU.fetch(U0, px, hs);
+        nhV.fetch(V0, px, hs);
+        nhW.fetch(W0, px, hs);
+    };
+    fetch(xb);
+    for (int x = xb; x < xe; ++x) {
+        const int b = (x - xb) & 1;
+#pragma unroll
+        for (int i = 0; i < 2 * M - 1; ++i) {
+            qU[i] = qU[i + 1];
+            qV[i] = qV[i + 1];
+            qW[i] = qW[i + 1];
+        }
+        qU[2 * M - 1] = nU;
+        qV[2 * M - 1] = nV;
+        qW[2 * M - 1] = nW;
+        R T0[6];
+#pragma unroll
+        for (int f = 0; f < 6; ++f) T0[f] = nT[f];
+        R* LU = sU[b];
+        R* LV = sV[b];
+        R* LW = sW[b];
+        const int me = (ly + M) * P::W + lz + M;
+        LU[me] = qU[M];
+        LV[me] = qV[M - 1];
+        LW[me] = qW[M - 1];
+        nhU.put(LU, hs);
+        nhV.put(LV, hs);
+        nhW.put(LW, hs);
+        if (x + 1 < xe) fetch(x + 1);
+        __syncthreads();
+        if (active) {
+            const int64_t q = (int64_t)x * g.sx + col;
+            R dx[M], dy[M], dz[M];
+#pragma unroll
+            for (int k = 0; k < M; ++k) {
+                dx[k] = qU[M + k] - qU[M - 1 - k];
+                dy[k] = LV[me + k * P::W] - LV[me - (1 + k) * P::W];
+                dz[k] = LW[me + k] - LW[me - 1 - k];
+            }
+            R lam = 0, l2m = 0;
+            if (HET) {
+                lam = md.m[OPESCI_M_LAMBDA][q];
+                l2m = lam + (R)2 * md.m[OPESCI_M_MU][q];
+            }
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                R acc = T0[i];
+                R s = 0;
+#pragma unroll
+                for (int k = 0; k < M; ++k) s = fmac(c.cn[i][0][k], dx[k], s);
+                acc = HET ? fmac(i == 0 ? l2m : lam, s, acc) : acc + s;
+                s = 0;
+#pragma unroll
+                for (int k = 0; k < M; ++k) s = fmac(c.cn[i][1][k], dy[k], s);
+                acc = HET ? fmac(i == 1 ? l2m : lam, s, acc) : acc + s;
+                s = 0;
+#pragma unroll
+                for (int k = 0; k < M; ++k) s = fmac(c.cn[i][2][k], dz[k], s);
+                acc = HET ? fmac(i == 2 ? l2m : lam, s, acc) : acc + s;
+                C::st(F.f[3 + i][t1] + q, acc);
+            }
+            // Txy: dU/dy then dV/dx;  Tyz: dV/dz then dW/dy;  Txz: dU/dz then dW/dx
+            R sxy = 0, syz = 0, sxz = 0;
+#pragma unroll
+            for (int k = 0; k < M; ++k) sxy = fmac(c.cs[0][0][k], LU[me + (1 + k) * P::W] - LU[me - k * P::W], sxy);
+#pragma unroll
+            for (int k = 0; k < M; ++k) sxy = fmac(c.cs[0][1][k], qV[M + k] - qV[M - 1 - k], sxy);
+#pragma unroll
+            for (int k = 0; k < M; ++k) syz = fmac(c.cs[1][0][k], LV[me + 1 + k] - LV[me - k], syz);
+#pragma unroll
+            for (int k = 0; k < M; ++k) syz = fmac(c.cs[1][1][k], LW[me + (1 + k) * P::W] - LW[me - k * P::W], syz);
+#pragma unroll
+            for (int k = 0; k < M; ++k) sxz = fmac(c.cs[2][0][k], LU[me + 1 + k] - LU[me - k], sxz);
+#pragma unroll
+            for (int k = 0; k < M; ++k) sxz = fmac(c.cs[2][1][k], qW[M + k] - qW[M - 1 - k], sxz);
+            if (HET) {
+                C::st(F.f[6][t1] + q, fmac(md.m[OPESCI_M_MU12][q], sxy, T0[3]));
+                C::st(F.f[7][t1] + q, fmac(md.m[OPESCI_M_MU23][q], syz, T0[4]));
+                C::st(F.f[8][t1] + q, fmac(md.m[OPESCI_M_MU13][q], sxz, T0[5]));
+            } else {
+                C::st(F.f[6][t1] + q, T0[3] + sxy);
+                C::st(F.f[7][t1] + q, T0[4] + syz);
+                C::st(F.f[8][t1] + q, T0[5] + sxz);
+            }
+        }
+    }
 }
 
-template <typename R, typename S, int M, bool HET>
-static hipError_t run_velocity(const Geom& g, void* fields, const void* media, const void* coefs, int t0, int t1,
+template <typename R, typename S, int M, int TY, bool HET>
+__global__ __launch_bounds__(64 * TY) void velocity_sweep(Geom g, FieldPtrs<S> F, MediaPtrs<R> md, Coefs<R> c,
+                                                          int t0, int t1, int tz, int ty, int xchunk, int nblocks) {
+    using P = Plane<M, TY>;
+    using C = Conv<S, R>;
+    __shared__ R sYY[2][P::H * P::W], sZZ[2][P::H * P::W], sXY[2][P::H * P::W], sYZ[2][P::H * P::W],
+        sXZ[2][P::H * P::W];
+    const int l = logical_block(nblocks);
+    if (l < 0) return;
+    const int bz = l % tz, by = (l / tz) % ty, bx = l / (tz * ty);
+    const int lz = threadIdx.x & 63, ly = threadIdx.x >> 6;
+    const int z0 = M + bz * 64, y0 = M + by * TY;
+    const int z = z0 + lz, y = y0 + ly;
+    const bool active = z < g.nz - M && y < g.ny - M;
+    const int64_t col = (int64_t)min(y, g.ny - 1) * g.sy + min(z, g.nz - 1);
+    const int xb = M + bx * xchunk, xe = min(xb + xchunk, g.nx - M);
+    const S* TXX = F.f[3][t1];
+    const S* TYY = F.f[4][t1];
+    const S* TZZ = F.f[5][t1];
+    const S* TXY = F.f[6][t1];
+    const S* TYZ = F.f[7][t1];
+    const S* TXZ = F.f[8][t1];
+    HaloSlots<M, TY> hs;
+    hs.init(g, y0, z0);
+    // qXX[i] = Txx(x+i-M+1) (a = 1), qXY[i] = Txy(x+i-M), qXZ[i] = Txz(x+i-M) (a = 0)
+    R qXX[2 * M], qXY[2 * M], qXZ[2 * M];
+#pragma unroll
+    for (int i = 1; i < 2 * M; ++i) {
+        qXX[i] = C::ld(TXX + (int64_t)(xb - M + i) * g.sx + col);
+        qXY[i] = C::ld(TXY + (int64_t)(xb - M - 1 + i) * g.sx + col);
+        qXZ[i] = C::ld(TXZ + (int64_t)(xb - M - 1 + i) * g.sx + col);
+    }
+    R nXX, nXY, nXZ, nYY, nZZ, nYZ, nU, nV, nW;
+    Staged<R, M, TY, true, false> hYY, hXY;
+    Staged<R, M, TY, false, true> hZZ, hXZ;
+    Staged<R, M, TY, true, true> hYZ;
+    auto fetch = [&](int x) {
+        const int64_t px = (int64_t)x * g.sx;
+        nXX = C::ld(TXX + px + (int64_t)M * g.sx + col);
+        nXY = C::ld(TXY + px + (int64_t)(M - 1) * g.sx + col);
+        nXZ = C::ld(TXZ + px + (int64_t)(M - 1) * g.sx + col);
+        nYY = C::ld(TYY + px + col);
+        nZZ = C::ld(TZZ + px + col);
+        nYZ = C::ld(TYZ + px + col);
+        nU = C::ld(F.f[0][t0] + px + col);
+        nV = C::ld(F.f[1][t0] + px + col);
+        nW = C::ld(F.f[2][t0] + px + col);
+        hYY.fetch(TYY, px, hs);
+        hXY.fetch(TXY, px, hs);
+        hZZ.fetch(TZZ, px, hs);
+        hXZ.fetch(TXZ, px, hs);
+        hYZ.fetch(TYZ, px, hs);
+    };
+    fetch(xb);
+    for (int x = xb; x < xe; ++x) {
+        const int b = (x - xb) & 1;
+#pragma unroll
+        for (int i = 0; i < 2 * M - 1; ++i) {
+            qXX[i] = qXX[i + 1];
+            qXY[i] = qXY[i + 1];
+            qXZ[i] = qXZ[i + 1];
+        }
+        qXX[2 * M - 1] = nXX;
+        qXY[2 * M - 1] = nXY;
+        qXZ[2 * M - 1] = nXZ;
+        const R u0 = nU, v0 = nV, w0 = nW;
+        R* LYY = sYY[b];
+        R* LZZ = sZZ[b];
+        R* LXY = sXY[b];
+        R* LYZ = sYZ[b];
+        R* LXZ = sXZ[b];
+        const int me = (ly + M) * P::W + lz + M;
+        LYY[me] = nYY;
+        LZZ[me] = nZZ;
+        LYZ[me] = nYZ;
+        LXY[me] = qXY[M];
+        LXZ[me] = qXZ[M];
+        hYY.put(LYY, hs);
+        hXY.put(LXY, hs);
+        hZZ.put(LZZ, hs);
+        hXZ.put(LXZ, hs);
+        hYZ.put(LYZ, hs);
+        if (x + 1 < xe) fetch(x + 1);
+        __syncthreads();
+        if (active) {
+            const int64_t q = (int64_t)x * g.sx + col;
+            R su = 0, sv = 0, sw = 0;
+#pragma unroll
+            for (int k = 0; k < M; ++k) su = fmac(c.cv[0][0][k], qXX[M + k] - qXX[M - 1 - k], su);
+#pragma unroll
+            for (int k = 0; k < M; ++k) su = fmac(c.cv[0][1][k], LXY[me + k * P::W] - LXY[me - (1 + k) * P::W], su);
+#pragma unroll
+            for (int k = 0; k < M; ++k) su = fmac(c.cv[0][2][k], LXZ[me + k] - LXZ[me - 1 - k], su);
+#pragma unroll
+            for (int k = 0; k < M; ++k) sv = fmac(c.cv[1][0][k], qXY[M + k] - qXY[M - 1 - k], sv);
+#pragma unroll
+            for (int k = 0; k < M; ++k) sv = fmac(c.cv[1][1][k], LYY[me + (1 + k) * P::W] - LYY[me - k * P::W], sv);
+#pragma unroll
+            for (int k = 0; k < M; ++k) sv = fmac(c.cv[1][2][k], LYZ[me + k] - LYZ[me - 1 - k], sv);
+#pragma unroll
+            for (int k = 0; k < M; ++k) sw = fmac(c.cv[2][0][k], qXZ[M + k] - qXZ[M - 1 - k], sw);
+#pragma unroll
+            for (int k = 0; k < M; ++k) sw = fmac(c.cv[2][1][k], LYZ[me + k * P::W] - LYZ[me - (1 + k) * P::W], sw);
+#pragma unroll
+            for (int k = 0; k < M; ++k) sw = fmac(c.cv[2][2][k], LZZ[me + 1 + k] - LZZ[me - k], sw);
+            if (HET) {
+                C::st(F.f[0][t1] + q, fmac(md.m[OPESCI_M_BETA1][q], su, u0));
+                C::st(F.f[1][t1] + q, fmac(md.m[OPESCI_M_BETA2][q], sv, v0));
+                C::st(F.f[2][t1] + q, fmac(md.m[OPESCI_M_BETA3][q], sw, w0));
+            } else {
+                C::st(F.f[0][t1] + q, u0 + su);
+                C::st(F.f[1][t1] + q, v0 + sv);
+                C::st(F.f[2][t1] + q, w0 + sw);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ dispatch
+struct Tiling {
+    int ty, tz, tyn, tx, xchunk, nb, grid;
+};
+
+template <int M>
+static Tiling tiling(const Geom& g, const InteriorLaunch& cfg, int ty) {
+    Tiling t;
+    const int ni[3] = {g.nx - 2 * M, g.ny - 2 * M, g.nz - 2 * M};
+    t.ty = ty;
+    t.tz = (ni[2] + 63) / 64;
+    t.tyn = (ni[1] + ty - 1) / ty;
+    int xc = cfg.x_chunk;
+    if (xc <= 0) {
+        // about 2048 blocks: enough to cover 256 CUs several times over
+        const int cols = t.tz * t.tyn;
+        xc = (ni[0] * cols + 2047) / 2048;
+        xc = xc < 8 ? 8 : xc;
+    }
+    t.xchunk = xc;
+    t.tx = (ni[0] + xc - 1) / xc;
+    t.nb = t.tz * t.tyn * t.tx;
+    t.grid = ((t.nb + 7) / 8) * 8;
+    return t;
+}
+
+template <typename R, typename S, int M, bool HET, bool VEL>
+static hipError_t run_interior(const Geom& g, void* fields, const void* media, const void* coefs, int t0, int t1,
                                const InteriorLaunch& cfg, hipStream_t s) {
     const FieldPtrs<S>& F = *static_cast<const FieldPtrs<S>*>(fields);
     const MediaPtrs<R>& md = *static_cast<const MediaPtrs<R>*>(media);
     const Coefs<R>& c = *static_cast<const Coefs<R>*>(coefs);
-    const int ni[3] = {g.nx - 2 * M, g.ny - 2 * M, g.nz - 2 * M};
-    const int xchunk = cfg.x_chunk > 0 ? cfg.x_chunk : 16;
-    const int tz = (ni[2] + 63) / 64, ty = (ni[1] + 3) / 4, tx = (ni[0] + xchunk - 1) / xchunk;
-    const int nb = tz * ty * tx;
-    const int grid = ((nb + 7) / 8) * 8;
-    hipLaunchKernelGGL((velocity_gather<R, S, M, HET>), dim3(grid), dim3(256), 0, s, g, F, md, c, t0, t1, tz, ty,
-                       xchunk, nb);
+    if (cfg.variant == 1) {
+        const Tiling t = tiling<M>(g, cfg, 4);
+        if (VEL)
+            hipLaunchKernelGGL((velocity_gather<R, S, M, HET>), dim3(t.grid), dim3(256), 0, s, g, F, md, c, t0, t1,
+                               t.tz, t.tyn, t.xchunk, t.nb);
+        else
+            hipLaunchKernelGGL((stress_gather<R, S, M, HET>), dim3(t.grid), dim3(256), 0, s, g, F, md, c, t0, t1,
+                               t.tz, t.tyn, t.xchunk, t.nb);
+        return hipGetLastError();
+    }
+    if (cfg.tile_y == 4) {
+        const Tiling t = tiling<M>(g, cfg, 4);
+        if (VEL)
+            hipLaunchKernelGGL((velocity_sweep<R, S, M, 4, HET>), dim3(t.grid), dim3(256), 0, s, g, F, md, c, t0, t1,
+                               t.tz, t.tyn, t.xchunk, t.nb);
+        else
+            hipLaunchKernelGGL((stress_sweep<R, S, M, 4, HET>), dim3(t.grid), dim3(256), 0, s, g, F, md, c, t0, t1,
+                               t.tz, t.tyn, t.xchunk, t.nb);
+        return hipGetLastError();
+    }
+    const Tiling t = tiling<M>(g, cfg, 8);
+    if (VEL)
+        hipLaunchKernelGGL((velocity_sweep<R, S, M, 8, HET>), dim3(t.grid), dim3(512), 0, s, g, F, md, c, t0, t1,
+                           t.tz, t.tyn, t.xchunk, t.nb);
+    else
+        hipLaunchKernelGGL((stress_sweep<R, S, M, 8, HET>), dim3(t.grid), dim3(512), 0, s, g, F, md, c, t0, t1,
+                           t.tz, t.tyn, t.xchunk, t.nb);
     return hipGetLastError();
 }
 
 template <typename R, typename S, bool VEL>
 static hipError_t dispatch_m(const Geom& g, bool het, void* fields, const void* media, const void* coefs, int t0,
                              int t1, const InteriorLaunch& cfg, hipStream_t s) {
-#define OPESCI_CASE(MM)                                                                            \
-    case MM:                                                                                       \
-        if (VEL)                                                                                   \
-            return het ? run_velocity<R, S, MM, true>(g, fields, media, coefs, t0, t1, cfg, s)     \
-                       : run_velocity<R, S, MM, false>(g, fields, media, coefs, t0, t1, cfg, s);   \
-        return het ? run_stress<R, S, MM, true>(g, fields, media, coefs, t0, t1, cfg, s)           \
-                   : run_stress<R, S, MM, false>(g, fields, media, coefs, t0, t1, cfg, s);
+#define OPESCI_CASE(MM)                                                                        \
+    case MM:                                                                                   \
+        return het ? run_interior<R, S, MM, true, VEL>(g, fields, media, coefs, t0, t1, cfg, s) \
+                   : run_interior<R, S, MM, false, VEL>(g, fields, media, coefs, t0, t1, cfg, s);
     switch (g.m) {
         OPESCI_CASE(1)
         OPESCI_CASE(2)

@@ -44,10 +44,9 @@ struct OpesciContext {
     void* mptr[OPESCI_M_COUNT] = {};  // MediaPtrs<R> image
     Coefs<float> cf;
     Coefs<double> cd;
-    std::vector<OpesciBcStmt> stmts;  // sorted by (phase, sched)
+    std::vector<OpesciBcStmt> stmts;  // sorted by (phase, sched, axis, ranges)
     std::vector<BcLevel> levels[4];
-    OpesciBcStmt* d_stmts = nullptr;
-    OpesciBcTerm* d_terms = nullptr;
+    BcDesc* d_desc = nullptr;
     OpesciExprOp* d_code = nullptr;
     OpesciExprOp* d_sol = nullptr;
     int sol_off[NF], sol_len[NF];
@@ -194,25 +193,58 @@ int validate(const OpesciModelDesc* d) {
     return 0;
 }
 
-int build_levels(OpesciContext* c) {
+// Sort statements by (phase, launch), build one self-contained descriptor
+// per statement with the linear displacements the device uses (all indices
+// were validated against the grid; the grid is checked to fit int32 offsets).
+int build_program(OpesciContext* c, const OpesciBcTerm* terms) {
     std::stable_sort(c->stmts.begin(), c->stmts.end(), [](const OpesciBcStmt& a, const OpesciBcStmt& b) {
         return a.phase != b.phase ? a.phase < b.phase : a.sched < b.sched;
     });
-    for (int i = 0; i < (int)c->stmts.size();) {
-        const OpesciBcStmt& s = c->stmts[i];
-        BcLevel lv{i, 0, 0};
+    const Geom& g = c->g;
+    if ((int64_t)g.nx * g.sx >= ((int64_t)1 << 31)) return fail(OPESCI_EUNSUPPORTED, "grid too large for int32 offsets");
+    const int64_t stride[3] = {g.sx, g.sy, 1};
+    std::vector<BcDesc> descs;
+    const int n = (int)c->stmts.size();
+    for (int i = 0; i < n;) {
+        const OpesciBcStmt& L = c->stmts[i];
+        BcLevel lv{(int)descs.size(), 0, 0};
         int j = i;
-        while (j < (int)c->stmts.size() && c->stmts[j].phase == s.phase && c->stmts[j].sched == s.sched) {
-            const OpesciBcStmt& t = c->stmts[j];
-            const int n = std::max(0, t.hi[0] - t.lo[0]) * std::max(0, t.hi[1] - t.lo[1]);
-            lv.maxpts = std::max(lv.maxpts, n);
-            ++j;
+        for (; j < n && c->stmts[j].phase == L.phase && c->stmts[j].sched == L.sched; ++j) {
+            const OpesciBcStmt& a = c->stmts[j];
+            if (a.nterms > BC_MAXT) return fail(OPESCI_EUNSUPPORTED, "boundary statement with more than 16 terms");
+            const int n0 = std::max(0, a.hi[0] - a.lo[0]), n1 = std::max(0, a.hi[1] - a.lo[1]);
+            if (n0 * n1 == 0) continue;
+            BcDesc d;
+            memset(&d, 0, sizeof(d));
+            d.fl = a.field * 2 + a.level;
+            d.disp = (int32_t)(a.plane * stride[a.axis]);
+            d.nterms = a.nterms;
+            d.axis = a.axis;
+            d.lo0 = a.lo[0];
+            d.lo1 = a.lo[1];
+            d.n1 = n1;
+            d.npts = n0 * n1;
+            const int o[2] = {a.axis == 0 ? 1 : 0, a.axis == 2 ? 1 : 2};
+            for (int k = 0; k < a.nterms; ++k) {
+                const OpesciBcTerm& tm = terms[a.term0 + k];
+                d.t[k].fl = tm.field * 2 + tm.level;
+                d.t[k].disp = (int32_t)((int64_t)(tm.plane - a.plane) * stride[a.axis] + (int64_t)tm.off[0] * stride[o[0]] +
+                                        (int64_t)tm.off[1] * stride[o[1]]);
+                d.t[k].coef = (float)tm.coef;
+                if ((double)d.t[k].coef != tm.coef)
+                    return fail(OPESCI_EINVAL, "boundary coefficient is not a float literal");
+                d.t[k].prog = tm.prog;
+                d.prog_len[k] = tm.prog_len;
+            }
+            descs.push_back(d);
+            lv.maxpts = std::max(lv.maxpts, d.npts);
         }
-        lv.count = j - i;
+        lv.count = (int)descs.size() - lv.first;
         if (lv.count > 65535) return fail(OPESCI_EUNSUPPORTED, "too many boundary statements in one launch");
-        c->levels[s.phase].push_back(lv);
+        if (lv.count > 0) c->levels[L.phase].push_back(lv);
         i = j;
     }
+    if (int e = upload(&c->d_desc, descs.data(), descs.size(), c->stream)) return e;
     return 0;
 }
 
@@ -224,8 +256,8 @@ int alloc_zero(OpesciContext* c, void** p, size_t bytes) {
 
 int run_bc_phase(OpesciContext* c, int phase, int t0, int t1) {
     for (const BcLevel& lv : c->levels[phase])
-        OPESCI_HIP_TRY(launch_bc(c->g, c->real_bytes, c->storage, c->fptr, c->mptr, c->d_stmts, c->d_terms, c->d_code,
-                                 lv, t0, t1, c->stream));
+        OPESCI_HIP_TRY(launch_bc(c->g, c->real_bytes, c->storage, c->fptr, c->mptr, c->d_desc, c->d_code, lv, t0,
+                                 t1, c->stream));
     return 0;
 }
 
@@ -274,7 +306,7 @@ void destroy(OpesciContext* c) {
             if (c->base[f][l]) (void)hipFree(c->base[f][l]);
     for (int i = 0; i < OPESCI_M_COUNT; ++i)
         if (c->mbase[i]) (void)hipFree(c->mbase[i]);
-    void* ptrs[] = {c->d_stmts, c->d_terms, c->d_code, c->d_sol, c->d_partial, c->d_stage,
+    void* ptrs[] = {c->d_desc, c->d_code, c->d_sol, c->d_partial, c->d_stage,
                     c->d_rec,   c->d_trace, c->d_wavelet, c->d_step};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -342,9 +374,7 @@ int create(const OpesciModelDesc* d, OpesciContext** out) {
                 c->cf.cs[s][gq][k] = (float)d->cs[s][gq][k];
             }
     c->stmts.assign(d->stmts, d->stmts + d->nstmts);
-    if (int e = build_levels(c.get())) return e;
-    if (int e = upload(&c->d_stmts, c->stmts.data(), c->stmts.size(), c->stream)) return e;
-    if (int e = upload(&c->d_terms, d->terms, (size_t)d->nterms, c->stream)) return e;
+    if (int e = build_program(c.get(), d->terms)) return e;
     if (int e = upload(&c->d_code, d->bc_code, (size_t)d->ncode, c->stream)) return e;
     if (int e = upload(&c->d_sol, d->sol_code, (size_t)d->nsol, c->stream)) return e;
     for (int f = 0; f < NF; ++f) {

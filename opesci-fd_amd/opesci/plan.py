@@ -268,6 +268,10 @@ class _BcBuilder(object):
             plane = int(lhs.indices[1 + axis])
             t0 = len(self.terms)
             rhs = expand(eq.rhs)
+            # group the terms reading the same cell: constant coefficients add,
+            # media-dependent ones become one summed coefficient program
+            groups = {}
+            order = []
             if rhs != 0:
                 for term in Add.make_args(rhs):
                     num, other, fld = _split(term, self.names)
@@ -276,16 +280,31 @@ class _BcBuilder(object):
                     src = self.names[str(fld.base.label)]
                     lvl = _time_level(fld.indices[0], g)
                     p = int(fld.indices[1 + axis])
-                    offs = [int(fld.indices[1 + a] - idx[a]) for a in others]
-                    prog, plen, coef = -1, 0, f_lit(num)
-                    if other:
-                        expr = Mul(*other)
-                        code = exprvm.compile_expr(expr, coords={}, consts={},
-                                                   media_anchor=self.media_anchor_for(axis))
-                        code = [(op, a, f_lit(v) if op == exprvm.OP_CONST else v) for op, a, v in code]
-                        prog, plen = len(self.code), len(code)
-                        self.code += code
-                    self.terms.append((src, lvl, p, offs, prog, plen, coef))
+                    offs = tuple(int(fld.indices[1 + a] - idx[a]) for a in others)
+                    key = (src, lvl, p, offs)
+                    if key not in groups:
+                        groups[key] = []
+                        order.append(key)
+                    groups[key].append((num, other))
+            for key in order:
+                src, lvl, p, offs = key
+                parts = groups[key]
+                if all(not o for _, o in parts):
+                    prog, plen, coef = -1, 0, f_lit(sum(f_lit(n) for n, _ in parts))
+                elif len(parts) == 1:
+                    num, other = parts[0]
+                    code = exprvm.compile_expr(Mul(*other), coords={}, consts={},
+                                               media_anchor=self.media_anchor_for(axis))
+                    code = [(op, a, f_lit(v) if op == exprvm.OP_CONST else v) for op, a, v in code]
+                    prog, plen, coef = len(self.code), len(code), f_lit(num)
+                    self.code += code
+                else:
+                    expr = Add(*[Float(f_lit(n)) * Mul(*o) for n, o in parts], evaluate=False)
+                    code = exprvm.compile_expr(expr, coords={}, consts={},
+                                               media_anchor=self.media_anchor_for(axis))
+                    prog, plen, coef = len(self.code), len(code), 1.0
+                    self.code += code
+                self.terms.append((src, lvl, p, list(offs), prog, plen, coef))
             self.stmts.append(dict(field=dst, level=_time_level(lhs.indices[0], g), axis=axis,
                                    plane=plane, lo=[r[0] for r in ranges], hi=[r[1] for r in ranges],
                                    term0=t0, nterms=len(self.terms) - t0, phase=phase, sched=0))

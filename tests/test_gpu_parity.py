@@ -207,3 +207,27 @@ def test_c1_second_order_fp64_64cube():
     ref = oracle.run(**kw)
     for f in range(9):
         assert abs(l2[f] - ref['l2'][f]) <= 1e-10 * ref['l2'][f] + 1e-15, FIELDS[f]
+
+
+@pytest.mark.parametrize("order,double,storage,het,tile_y", [
+    (2, False, 'native', False, 8), (4, False, 'native', False, 8), (4, False, 'native', False, 4),
+    (6, True, 'native', False, 8), (8, False, 'native', False, 8), (8, True, 'native', False, 4),
+    (4, False, 'fp16', False, 8), (4, False, 'bf16', False, 8), (4, False, 'native', True, 8),
+    (8, True, 'native', True, 8)])
+def test_sweep_equals_gather_bitwise(order, double, storage, het, tile_y):
+    """The LDS/register-queue sweep kernels and the plain gather kernels keep
+    the same accumulation order, so their results agree bit for bit (odd
+    sizes exercise partial tiles and x-chunks)."""
+    if het:
+        g, _ = hetero_model(27, order, 7, double=double, storage=storage, sigma=2.0)
+    else:
+        g, _ = eigen_model(27, order, 7, double=double, storage=storage)
+    plan = g.lower()
+    plan.tile_y = tile_y
+    plan.x_chunk = 5
+    plan.kernel_variant = 0
+    a, _ = gpu_run(plan)
+    plan.kernel_variant = 1
+    b, _ = gpu_run(plan)
+    for f in range(9):
+        assert np.array_equal(a[f], b[f]), FIELDS[f]
