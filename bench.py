@@ -8,7 +8,13 @@ Levander free surfaces on all faces.  A step is one full leapfrog step
 launches) over the whole grid, run by libopesci_hip through its C ABI with
 the fields resident in HBM.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--copy-probe MIB]
+
+--copy-probe runs three device-to-device copies of MIB MiB before the
+workload: a plain copy kernel of known size against which rocprofv3's
+FETCH_SIZE / WRITE_SIZE counters are scaled.  The HBM traffic of the stress
+kernel measured that way is read from profiles/*_pmc.md (lines starting with
+"traffic:"); it is null for a workload without such a line.
 
 Prints one JSON line.  The roofline entry is for the stress kernel (the
 dominant launch: 9 field loads + 6 stores per point); its duration is taken
@@ -27,6 +33,7 @@ for _p in (os.path.join(ROOT, 'opesci-fd_amd'), os.path.join(ROOT, 'tests'), ROO
     if _p not in sys.path:
         sys.path.insert(0, _p)
 
+PROFILES = os.path.join(ROOT, 'profiles')
 METRIC = 'elastic grid-pt updates/s and % of HBM roofline, 1 MI355X, at stated L2 error'
 HBM_PEAK = 8.0e12
 
@@ -44,6 +51,9 @@ def main():
     ap.add_argument('--variant', type=int, default=None, help='interior kernel variant (0 sweep, 1 gather)')
     ap.add_argument('--x-chunk', type=int, default=0)
     ap.add_argument('--tile-y', type=int, default=0)
+    ap.add_argument('--copy-probe', type=int, default=0, help='MiB copied device-to-device 3x first')
+    ap.add_argument('--split-z', action='store_true',
+                    help='run the deep rows of the stress z-face loops as one launch ahead of the others')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py runs on exactly one GPU: this backend has no multi-GPU path '
@@ -54,8 +64,13 @@ def main():
     from opesci.backend import Context
     from workloads import eigen_model
 
+    hip = _hip()
+    if args.copy_probe:
+        copy_probe(hip, args.copy_probe << 20)
     N, order, K, W = args.size, args.order, args.steps, args.warmup
     g, kw = eigen_model(N, order, K + W, storage=args.storage)
+    if args.split_z:
+        g.fuse_boundary = True
     plan = g.lower()
     if args.variant is not None:
         plan.kernel_variant = args.variant
@@ -66,11 +81,15 @@ def main():
     with Context(plan) as ctx:
         ctx.init()
         ctx.run(W)
+        _check(hip.hipDeviceSynchronize(), 'hipDeviceSynchronize')
         t0 = time.perf_counter()
         secs = ctx.run(K)   # HIP events on the context stream, synchronised on return
+        _check(hip.hipDeviceSynchronize(), 'hipDeviceSynchronize')
         wall = time.perf_counter() - t0
         l2 = ctx.l2()
         kt = ctx.kernel_times(10)   # per-launch averages, after the timed region
+        bc_stats = ctx.stats()
+    workload_key = 'stress_sweep %d^3 order %d %s' % (N, order, args.storage)
     ms_step = secs * 1e3 / K
     value = pts * K / secs
     stress_bytes = pts * 15 * elem            # 9 loads + 6 stores per point
@@ -98,7 +117,8 @@ def main():
             'step_hbm_roofline_frac': value * step_bytes / pts / HBM_PEAK,
             'wall_s': wall,
             'kernel_ms': {'stress': kt[0], 'velocity': kt[1], 'boundary': kt[2]},
-            'boundary_launches_per_step': int(plan.nlaunches_step),
+            'boundary_launches_per_step': bc_stats[0],
+            'deep_z_face_statements': bc_stats[1],
         },
         'roofline': {
             'bound': 'hbm',
@@ -106,7 +126,7 @@ def main():
             'peak': HBM_PEAK / 1e9,
             'unit': 'GB/s',
             'frac': achieved / HBM_PEAK,
-            'traffic': None,
+            'traffic': committed_traffic(workload_key),
             'kernel': 'stress update (6 stresses at t1 from U,V,W and stresses at t0)',
             'algorithmic_bytes_per_launch': stress_bytes,
         },
@@ -114,6 +134,46 @@ def main():
     if not args.no_cpu:
         out['cpu_baseline'] = cpu_baseline(N, order, args.cpu_steps)
     print(json.dumps(out))
+
+
+def _hip():
+    import ctypes
+    hip = ctypes.CDLL('libamdhip64.so')
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    return hip
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise RuntimeError('%s failed with HIP error %d' % (what, rc))
+
+
+def copy_probe(hip, nbytes):
+    """Three hipMemcpyDeviceToDevice of nbytes (the runtime's copy kernel)."""
+    import ctypes
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    _check(hip.hipMalloc(ctypes.byref(a), nbytes), 'hipMalloc')
+    _check(hip.hipMalloc(ctypes.byref(b), nbytes), 'hipMalloc')
+    for _ in range(3):
+        _check(hip.hipMemcpy(b, a, nbytes, 3), 'hipMemcpy')   # 3 = hipMemcpyDeviceToDevice
+    _check(hip.hipDeviceSynchronize(), 'hipDeviceSynchronize')
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
+def committed_traffic(key):
+    """HBM bytes per launch from a committed PMC summary, or None."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(PROFILES, '*_pmc.md'))):
+        with open(path) as f:
+            for line in f:
+                if line.startswith('traffic:'):
+                    rec = json.loads(line[len('traffic:'):])
+                    if rec.get('workload') == key:
+                        return float(rec['bytes_per_launch'])
+    return None
 
 
 def cpu_baseline(N, order, steps):

@@ -99,7 +99,10 @@ typedef struct OpesciBcStmt {
     int32_t hi[2];
     int32_t term0;   /* first term in the term table */
     int32_t nterms;
-    int32_t phase;   /* 0: init stress, 1: init velocity, 2: step stress, 3: step velocity */
+    int32_t phase;   /* 0: init stress, 1: init velocity, 2: step stress, 3: step velocity,
+                        4: step stress, rows of z-face loops away from the x/y faces: they
+                           commute and run before every phase-2 launch (the generator
+                           proves the result equals the reference order) */
     int32_t sched;   /* launch index within the phase */
 } OpesciBcStmt;
 
@@ -209,6 +212,11 @@ int opesci_ctx_steps(OpesciContext* ctx, int32_t* steps);
 /* average device time per launch of each hot kernel over the last run
  * (HIP events on the context stream): [0] stress, [1] velocity, [2] all BC launches of a step */
 int opesci_ctx_kernel_times(OpesciContext* ctx, int32_t nsteps, double out_ms[3]);
+/* Boundary execution plan of a context: out[0] = boundary launches per time
+ * step, out[1] = phase-4 statements (deep z-face rows of the stress phase),
+ * out[2] = launches they take (0 or 1), out[3] = 0.  No reference
+ * counterpart. */
+int opesci_ctx_stats(OpesciContext* ctx, int32_t out[4]);
 
 const char* opesci_last_error(void);
 int opesci_abi_version(void);

@@ -1,45 +1,8 @@
 // Boundary program, analytic initial state, L2 norms, effective media,
 // layout conversion and point source / receivers.
-#include "common.h"
+#include "bcdev.h"
 
 namespace opesci {
-
-// ------------------------------------------------------------------ bytecode
-// Stack machine over OpesciExprOp (opcodes in include/opesci_hip.h),
-// evaluated in double.  MEDIA reads media a[0] at anchor + (a[1],a[2],a[3]).
-template <typename R>
-__device__ double vm_eval(const OpesciExprOp* code, int n, const double coord[3], double t,
-                          const MediaPtrs<R>& md, const Geom& g, int ax, int ay, int az) {
-    double st[16];
-    int sp = 0;
-    for (int i = 0; i < n; ++i) {
-        const OpesciExprOp o = code[i];
-        switch (o.op) {
-            case OPESCI_OP_CONST: st[sp++] = o.val; break;
-            case OPESCI_OP_COORD: st[sp++] = coord[o.a[0]]; break;
-            case OPESCI_OP_TIME: st[sp++] = t; break;
-            case OPESCI_OP_MEDIA: {
-                const R* p = md.m[o.a[0]];
-                st[sp++] = p ? (double)p[g.at(ax + o.a[1], ay + o.a[2], az + o.a[3])] : 0.0;
-                break;
-            }
-            case OPESCI_OP_ADD: --sp; st[sp - 1] = st[sp - 1] + st[sp]; break;
-            case OPESCI_OP_SUB: --sp; st[sp - 1] = st[sp - 1] - st[sp]; break;
-            case OPESCI_OP_MUL: --sp; st[sp - 1] = st[sp - 1] * st[sp]; break;
-            case OPESCI_OP_DIV: --sp; st[sp - 1] = st[sp - 1] / st[sp]; break;
-            case OPESCI_OP_NEG: st[sp - 1] = -st[sp - 1]; break;
-            case OPESCI_OP_POWC: st[sp - 1] = pow(st[sp - 1], o.val); break;
-            case OPESCI_OP_SQRT: st[sp - 1] = sqrt(st[sp - 1]); break;
-            case OPESCI_OP_SIN: st[sp - 1] = sin(st[sp - 1]); break;
-            case OPESCI_OP_COS: st[sp - 1] = cos(st[sp - 1]); break;
-            case OPESCI_OP_EXP: st[sp - 1] = exp(st[sp - 1]); break;
-            case OPESCI_OP_LOG: st[sp - 1] = log(st[sp - 1]); break;
-            case OPESCI_OP_ROUNDF: st[sp - 1] = (double)(float)st[sp - 1]; break;
-            default: break;
-        }
-    }
-    return sp > 0 ? st[0] : 0.0;
-}
 
 // real_t x = dx*(i - m [+ 0.5]) of the reference loops, rounded once to R
 template <typename R>
@@ -52,45 +15,15 @@ __device__ __forceinline__ double coord_of(double dx, int i, int m, int stagger)
 // One launch per hazard level, blockIdx.y = statement.  Statements of a
 // level write disjoint cells and never read what another statement of the
 // level writes.  Terms are loaded independently and summed in term order.
-template <typename R, typename S>
+template <typename R, typename S, bool PROG>
 __global__ __launch_bounds__(256) void bc_kernel(Geom g, FieldPtrs<S> F, MediaPtrs<R> md,
                                                  const BcDesc* __restrict__ desc,
                                                  const OpesciExprOp* __restrict__ code, int first, int t0, int t1) {
-    using C = Conv<S, R>;
     const BcDesc* D = desc + first + blockIdx.y;
-    const int npts = D->npts;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npts) return;
-    const int n1 = D->n1, axis = D->axis, nt = D->nterms;
-    const int i0 = D->lo0 + p / n1, i1 = D->lo1 + p % n1;
-    const int o0 = axis == 0 ? 1 : 0, o1 = axis == 2 ? 1 : 2;
-    const int64_t so0 = o0 == 0 ? g.sx : g.sy, so1 = o1 == 1 ? g.sy : 1;
-    const int64_t dst = (int64_t)i0 * so0 + (int64_t)i1 * so1 + D->disp;
-    R v[BC_MAXT];
-#pragma unroll
-    for (int k = 0; k < BC_MAXT; ++k)
-        if (k < nt) {
-            const int fl = D->t[k].fl;
-            v[k] = C::ld(F.f[fl >> 1][(fl & 1) ? t1 : t0] + dst + D->t[k].disp);
-        }
-    R acc = (R)0;
-#pragma unroll
-    for (int k = 0; k < BC_MAXT; ++k)
-        if (k < nt) {
-            R coef = (R)D->t[k].coef;
-            if (D->t[k].prog >= 0) {
-                int a[3];
-                a[axis] = 0;
-                a[o0] = i0;
-                a[o1] = i1;
-                const double c0[3] = {0, 0, 0};
-                coef = (R)((double)D->t[k].coef *
-                           vm_eval<R>(code + D->t[k].prog, D->prog_len[k], c0, 0.0, md, g, a[0], a[1], a[2]));
-            }
-            acc = fma(coef, v[k], acc);
-        }
-    const int fl = D->fl;
-    C::st(F.f[fl >> 1][(fl & 1) ? t1 : t0] + dst, acc);
+    if (p >= D->npts) return;
+    const int n1 = D->n1;
+    bc_point<R, S, PROG>(g, F, md, D, code, D->lo0 + p / n1, D->lo1 + p % n1, t0, t1);
 }
 
 struct SolTable {
@@ -264,8 +197,13 @@ static hipError_t bc_t(const Geom& g, void* fields, const void* media, const BcD
                        const BcLevel& lv, int t0, int t1, hipStream_t s) {
     if (lv.count <= 0 || lv.maxpts <= 0) return hipSuccess;
     dim3 grid((lv.maxpts + 255) / 256, lv.count);
-    hipLaunchKernelGGL((bc_kernel<R, S>), grid, dim3(256), 0, s, g, *static_cast<const FieldPtrs<S>*>(fields),
-                       *static_cast<const MediaPtrs<R>*>(media), desc, code, lv.first, t0, t1);
+    if (lv.has_prog)
+        hipLaunchKernelGGL((bc_kernel<R, S, true>), grid, dim3(256), 0, s, g, *static_cast<const FieldPtrs<S>*>(fields),
+                           *static_cast<const MediaPtrs<R>*>(media), desc, code, lv.first, t0, t1);
+    else
+        hipLaunchKernelGGL((bc_kernel<R, S, false>), grid, dim3(256), 0, s, g,
+                           *static_cast<const FieldPtrs<S>*>(fields), *static_cast<const MediaPtrs<R>*>(media), desc,
+                           code, lv.first, t0, t1);
     return hipGetLastError();
 }
 

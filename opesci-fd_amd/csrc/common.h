@@ -124,6 +124,7 @@ struct InteriorLaunch {
     int x_chunk;
 };
 
+
 hipError_t launch_stress(const Geom& g, int real_bytes, int storage, bool het, void* fields, const void* media,
                          const void* coefs, int t0, int t1, const InteriorLaunch& cfg, hipStream_t s);
 hipError_t launch_velocity(const Geom& g, int real_bytes, int storage, bool het, void* fields, const void* media,
@@ -154,6 +155,7 @@ struct BcDesc {
 };
 struct BcLevel {
     int first, count, maxpts;   // descriptors [first, first+count)
+    int has_prog;               // some term has a media program
 };
 hipError_t launch_bc(const Geom& g, int real_bytes, int storage, void* fields, const void* media,
                      const BcDesc* d_desc, const OpesciExprOp* d_code, const BcLevel& lvl, int t0, int t1,

@@ -45,7 +45,7 @@ struct OpesciContext {
     Coefs<float> cf;
     Coefs<double> cd;
     std::vector<OpesciBcStmt> stmts;  // sorted by (phase, sched, axis, ranges)
-    std::vector<BcLevel> levels[4];
+    std::vector<BcLevel> levels[5];    // phases 0..3 + 4 (deep z-face rows of the stress phase, one launch)
     BcDesc* d_desc = nullptr;
     OpesciExprOp* d_code = nullptr;
     OpesciExprOp* d_sol = nullptr;
@@ -84,6 +84,7 @@ int upload(T** dst, const T* src, size_t n, hipStream_t s) {
     }
     OPESCI_HIP_TRY(hipMalloc((void**)dst, n * sizeof(T)));
     OPESCI_HIP_TRY(hipMemcpyAsync(*dst, src, n * sizeof(T), hipMemcpyHostToDevice, s));
+    OPESCI_HIP_TRY(hipStreamSynchronize(s));  // src may be a temporary
     return 0;
 }
 
@@ -131,7 +132,7 @@ int validate(const OpesciModelDesc* d) {
     for (int i = 0; i < d->nstmts; ++i) {
         const OpesciBcStmt& s = d->stmts[i];
         if (s.field < 0 || s.field >= NF || s.axis < 0 || s.axis > 2 || s.level < 0 || s.level > 1 || s.phase < 0 ||
-            s.phase > 3 || s.sched < 0 || s.term0 < 0 || s.nterms < 0 || s.term0 + s.nterms > d->nterms)
+            s.phase > 4 || s.sched < 0 || s.term0 < 0 || s.nterms < 0 || s.term0 + s.nterms > d->nterms)
             return fail(OPESCI_EINVAL, "malformed boundary statement");
         const int o[2] = {s.axis == 0 ? 1 : 0, s.axis == 2 ? 1 : 2};
         if (s.plane < 0 || s.plane >= d->dims[s.axis]) return fail(OPESCI_EINVAL, "boundary plane outside the grid");
@@ -207,7 +208,7 @@ int build_program(OpesciContext* c, const OpesciBcTerm* terms) {
     const int n = (int)c->stmts.size();
     for (int i = 0; i < n;) {
         const OpesciBcStmt& L = c->stmts[i];
-        BcLevel lv{(int)descs.size(), 0, 0};
+        BcLevel lv{(int)descs.size(), 0, 0, 0};
         int j = i;
         for (; j < n && c->stmts[j].phase == L.phase && c->stmts[j].sched == L.sched; ++j) {
             const OpesciBcStmt& a = c->stmts[j];
@@ -235,6 +236,7 @@ int build_program(OpesciContext* c, const OpesciBcTerm* terms) {
                     return fail(OPESCI_EINVAL, "boundary coefficient is not a float literal");
                 d.t[k].prog = tm.prog;
                 d.prog_len[k] = tm.prog_len;
+                if (tm.prog >= 0) lv.has_prog = 1;
             }
             descs.push_back(d);
             lv.maxpts = std::max(lv.maxpts, d.npts);
@@ -265,6 +267,7 @@ int step_once(OpesciContext* c, int t0) {
     const int t1 = 1 - t0;
     OPESCI_HIP_TRY(launch_stress(c->g, c->real_bytes, c->storage, c->het, c->fptr, c->mptr, c->coefs(), t0, t1,
                                  c->cfg, c->stream));
+    if (int e = run_bc_phase(c, 4, t0, t1)) return e;
     if (int e = run_bc_phase(c, 2, t0, t1)) return e;
     OPESCI_HIP_TRY(launch_velocity(c->g, c->real_bytes, c->storage, c->het, c->fptr, c->mptr, c->coefs(), t0, t1,
                                    c->cfg, c->stream));
@@ -306,8 +309,8 @@ void destroy(OpesciContext* c) {
             if (c->base[f][l]) (void)hipFree(c->base[f][l]);
     for (int i = 0; i < OPESCI_M_COUNT; ++i)
         if (c->mbase[i]) (void)hipFree(c->mbase[i]);
-    void* ptrs[] = {c->d_desc, c->d_code, c->d_sol, c->d_partial, c->d_stage,
-                    c->d_rec,   c->d_trace, c->d_wavelet, c->d_step};
+    void* ptrs[] = {c->d_desc, c->d_code,    c->d_sol,  c->d_partial, c->d_stage, c->d_rec,
+                    c->d_trace, c->d_wavelet, c->d_step};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -596,7 +599,8 @@ int opesci_ctx_kernel_times(OpesciContext* ctx, int32_t nsteps, double out_ms[3]
         OPESCI_HIP_TRY(launch_stress(ctx->g, ctx->real_bytes, ctx->storage, ctx->het, ctx->fptr, ctx->mptr,
                                      ctx->coefs(), t0, t1, ctx->cfg, ctx->stream));
         OPESCI_HIP_TRY(hipEventRecord(e[1], ctx->stream));
-        rc = run_bc_phase(ctx, 2, t0, t1);
+        rc = run_bc_phase(ctx, 4, t0, t1);
+        if (!rc) rc = run_bc_phase(ctx, 2, t0, t1);
         OPESCI_HIP_TRY(hipEventRecord(e[2], ctx->stream));
         OPESCI_HIP_TRY(launch_velocity(ctx->g, ctx->real_bytes, ctx->storage, ctx->het, ctx->fptr, ctx->mptr,
                                        ctx->coefs(), t0, t1, ctx->cfg, ctx->stream));
@@ -625,6 +629,17 @@ int opesci_ctx_kernel_times(OpesciContext* ctx, int32_t nsteps, double out_ms[3]
     for (auto& e : ev) (void)hipEventDestroy(e);
     for (int i = 0; i < 3; ++i) out_ms[i] = acc[i] / nsteps;
     return rc;
+}
+
+int opesci_ctx_stats(OpesciContext* ctx, int32_t out[4]) {
+    if (!ctx || !out) return fail(OPESCI_EINVAL, "null argument");
+    int n4 = 0;
+    for (const BcLevel& lv : ctx->levels[4]) n4 += lv.count;
+    out[0] = (int)(ctx->levels[2].size() + ctx->levels[3].size() + ctx->levels[4].size());
+    out[1] = n4;
+    out[2] = (int)ctx->levels[4].size();
+    out[3] = 0;
+    return 0;
 }
 
 // ---------------------------------------------------------------- reference-named ABI

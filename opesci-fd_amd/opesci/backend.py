@@ -52,7 +52,7 @@ _LIB = {}
 EXPORTS = ('opesci_model_load', 'opesci_execute', 'opesci_convergence', 'opesci_free',
            'opesci_ctx_create', 'opesci_ctx_destroy', 'opesci_ctx_init', 'opesci_ctx_run',
            'opesci_ctx_l2', 'opesci_ctx_download', 'opesci_ctx_upload', 'opesci_ctx_receivers',
-           'opesci_ctx_steps', 'opesci_ctx_kernel_times', 'opesci_last_error',
+           'opesci_ctx_steps', 'opesci_ctx_kernel_times', 'opesci_ctx_stats', 'opesci_last_error',
            'opesci_abi_version', 'opesci_device_info')
 
 
@@ -78,6 +78,7 @@ def load(path=None):
         lib.opesci_ctx_receivers.argtypes = [_vp, _vp, ctypes.c_int64]
         lib.opesci_ctx_steps.argtypes = [_vp, ctypes.POINTER(_i32)]
         lib.opesci_ctx_kernel_times.argtypes = [_vp, _i32, ctypes.POINTER(_f64)]
+        lib.opesci_ctx_stats.argtypes = [_vp, ctypes.POINTER(_i32)]
         lib.opesci_model_load.argtypes = [ctypes.POINTER(OpesciModelDesc)]
         lib.opesci_execute.argtypes = [_vp, _vp]
         lib.opesci_convergence.argtypes = [_vp, _vp]
@@ -250,3 +251,10 @@ class Context(object):
         out = (_f64 * 3)()
         _check(self.lib.opesci_ctx_kernel_times(self.h, nsteps, out), 'opesci_ctx_kernel_times')
         return list(out)
+
+    def stats(self):
+        """(boundary launches per step, deep z-face statements of the stress
+        phase, launches they take)."""
+        out = (_i32 * 4)()
+        _check(self.lib.opesci_ctx_stats(self.h, out), 'opesci_ctx_stats')
+        return tuple(out[:3])

@@ -311,14 +311,17 @@ class _BcBuilder(object):
             self.op_of_stmt.append(op_id)
 
     # -------- hazard analysis
-    def _boxes(self, s):
-        """write box and read boxes of a statement: (field, level, ((lo,hi) x3))."""
+    def _boxes(self, s, lo=None, hi=None):
+        """write box and read boxes of a statement over its loop ranges (or over
+        [lo, hi) instead): (field, level, ((lo,hi) x3))."""
         ax = s['axis']
+        lo = s['lo'] if lo is None else lo
+        hi = s['hi'] if hi is None else hi
         others = [a for a in range(3) if a != ax]
         w = [None] * 3
         w[ax] = (s['plane'], s['plane'] + 1)
         for k, a in enumerate(others):
-            w[a] = (s['lo'][k], s['hi'][k])
+            w[a] = (lo[k], hi[k])
         wbox = (s['field'], s['level'], tuple(w))
         rboxes = []
         for t in self.terms[s['term0']:s['term0'] + s['nterms']]:
@@ -326,9 +329,112 @@ class _BcBuilder(object):
             r = [None] * 3
             r[ax] = (p, p + 1)
             for k, a in enumerate(others):
-                r[a] = (s['lo'][k] + offs[k], s['hi'][k] + offs[k])
+                r[a] = (lo[k] + offs[k], hi[k] + offs[k])
             rboxes.append((src, lvl, tuple(r)))
         return wbox, rboxes
+
+    def _conflict(self, a, b):
+        (wa, ra), (wb, rb) = a, b
+        return self._hit(wa, wb) or any(self._hit(wa, r) for r in rb) or any(self._hit(r, wb) for r in ra)
+
+    # -------- z-face statements folded into the interior kernel
+    @staticmethod
+    def _deep_range(s, F, dims):
+        lo = [max(s['lo'][0], F), max(s['lo'][1], F)]
+        hi = [min(s['hi'][0], dims[0] - F), min(s['hi'][1], dims[1] - F)]
+        if lo[0] >= hi[0] or lo[1] >= hi[1]:
+            return None
+        return lo, hi
+
+    @staticmethod
+    def _frame(s, lo, hi):
+        """Loop box of s minus [lo, hi): up to four rectangles (x slabs, then y slabs)."""
+        L, H = s['lo'], s['hi']
+        out = []
+        for r in (((L[0], lo[0]), (L[1], H[1])), ((hi[0], H[0]), (L[1], H[1])),
+                  ((lo[0], hi[0]), (L[1], lo[1])), ((lo[0], hi[0]), (hi[1], H[1]))):
+            if r[0][0] < r[0][1] and r[1][0] < r[1][1]:
+                out.append(([r[0][0], r[1][0]], [r[0][1], r[1][1]]))
+        return out
+
+    def _try_fuse(self, idx, F, dims):
+        """Rows (x, y) in [F, dim-F)^2 of the z-face statements idx (reference
+        order within one phase), moved to run right after the interior update
+        and before every other boundary statement of the phase.  Valid when
+          * a moved part reads fields written in the phase only in its own row
+            (offsets 0, 0) and only cells no moved part writes: it then sees the
+            interior kernel's value, as in the reference order;
+          * moved parts never write the same cell twice;
+          * no statement left in place that precedes a moved part in the
+            reference order conflicts with it (read-after-write,
+            write-after-read, write-after-write).
+        Returns {statement index: (lo, hi)} or None."""
+        st = self.stmts
+        phase = st[idx[0]]['phase']
+        members = [i for i, s in enumerate(st) if s['phase'] == phase]
+        written = set((st[i]['field'], st[i]['level']) for i in members)
+        deep = {}
+        for j in idx:
+            d = self._deep_range(st[j], F, dims)
+            if d is not None:
+                deep[j] = d
+        if not deep:
+            return None
+        wcells = set()
+        for j, (lo, hi) in deep.items():
+            s = st[j]
+            key = (s['field'], s['level'], s['plane'])
+            if key in wcells:
+                return None
+            wcells.add(key)
+        for j in deep:
+            for t in self.terms[st[j]['term0']:st[j]['term0'] + st[j]['nterms']]:
+                if (t[0], t[1]) in written:
+                    if tuple(t[3]) != (0, 0) or (t[0], t[1], t[2]) in wcells:
+                        return None
+        for j, (lo, hi) in deep.items():
+            bj = self._boxes(st[j], lo, hi)
+            for i in members:
+                if i >= j:
+                    break
+                if i in deep:
+                    pieces = [self._boxes(st[i], a, b) for a, b in self._frame(st[i], *deep[i])]
+                else:
+                    pieces = [self._boxes(st[i])]
+                if any(self._conflict(bi, bj) for bi in pieces):
+                    return None
+        return deep
+
+    def fuse_z(self, phase, fused_phase, dims, margin):
+        """Split the z-face statements of a phase into a part folded into the
+        interior kernel (fused_phase) and the frame rows that stay in place,
+        for the smallest frame width F that keeps the reference semantics."""
+        idx = [i for i, s in enumerate(self.stmts) if s['phase'] == phase and s['axis'] == 2]
+        if not idx:
+            return 0
+        for F in range(margin + 1, min(dims[0], dims[1]) // 2):
+            deep = self._try_fuse(idx, F, dims)
+            if deep is not None:
+                break
+        else:
+            return 0
+        stmts, ops, seq = [], [], []
+        for i, s in enumerate(self.stmts):
+            if i not in deep:
+                stmts.append(s)
+                ops.append(self.op_of_stmt[i])
+                seq.append(self.seq[i])
+                continue
+            lo, hi = deep[i]
+            stmts.append(dict(s, lo=list(lo), hi=list(hi), phase=fused_phase))
+            ops.append(self.op_of_stmt[i])
+            seq.append(self.seq[i])
+            for a, b in self._frame(s, lo, hi):
+                stmts.append(dict(s, lo=list(a), hi=list(b)))
+                ops.append(self.op_of_stmt[i])
+                seq.append(self.seq[i])
+        self.stmts, self.op_of_stmt, self.seq = stmts, ops, seq
+        return F
 
     @staticmethod
     def _hit(a, b):
@@ -385,6 +491,11 @@ def _build_bc(grid, slots):
     order = sorted(range(len(b.stmts)), key=lambda i: b.stmts[i]['phase'])
     b.stmts = [b.stmts[i] for i in order]
     b.op_of_stmt = [b.op_of_stmt[i] for i in order]
+    b.seq = list(range(len(b.stmts)))
+    b.fused_frame = 0
+    if getattr(grid, 'fuse_boundary', False):
+        dims = [d.value for d in grid.dim]
+        b.fused_frame = b.fuse_z(2, 4, dims, grid.margin.value)
     b.schedule()
     return b
 
@@ -483,6 +594,8 @@ def build_plan(grid):
     p.cn, p.cs, p.cv = _extract_interior(grid, slots)
     b = _build_bc(grid, slots)
     p.stmts, p.terms, p.bc_code = b.arrays()
+    p.stmt_seq = np.asarray(b.seq, dtype=np.int64)
+    p.fused_frame = b.fused_frame
     p.nlaunches_step = int(sum(len(set(p.stmts['sched'][p.stmts['phase'] == ph])) for ph in (2, 3)))
     p.sol_code, p.sol_off, p.sol_len = _solutions(grid, slots)
     p.src_wavelet = getattr(grid, 'src_wavelet', None)

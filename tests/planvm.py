@@ -189,13 +189,23 @@ class PlanVM(object):
             F[i, t1][inner] = acc
 
     # ---------- boundary program ----------
-    def bc(self, phase, t0, t1):
+    def bc(self, phases, t0, t1):
+        """Run the statements of the given phases.  Scheduled order: phase by
+        phase as listed (statements folded into the interior kernel first),
+        launch level, then position.  Sequential order: the reference emission
+        order (pieces of one split statement together)."""
         p, F, R = self.p, self.F, self.dt_real
         lv = (t0, t1)
         st = p.stmts
-        idx = np.nonzero(st['phase'] == phase)[0]
+        if isinstance(phases, int):
+            phases = (phases,)
+        idx = [i for i in range(len(st)) if int(st['phase'][i]) in phases]
         if self.order_by_sched:
-            idx = sorted(idx, key=lambda i: (st['sched'][i], i))
+            rank = {ph: r for r, ph in enumerate(phases)}
+            idx = sorted(idx, key=lambda i: (rank[int(st['phase'][i])], st['sched'][i], i))
+        else:
+            seq = getattr(p, 'stmt_seq', np.arange(len(st)))
+            idx = sorted(idx, key=lambda i: (seq[i], i))
         for i in idx:
             s = st[i]
             ax = int(s['axis'])
@@ -234,7 +244,7 @@ class PlanVM(object):
         t0 = self.steps % 2
         t1 = 1 - t0
         self.stress(t0, t1)
-        self.bc(2, t0, t1)
+        self.bc((4, 2), t0, t1)
         self.velocity(t0, t1)
         self.bc(3, t0, t1)
         self.steps += 1

@@ -231,3 +231,29 @@ def test_sweep_equals_gather_bitwise(order, double, storage, het, tile_y):
     b, _ = gpu_run(plan)
     for f in range(9):
         assert np.array_equal(a[f], b[f]), FIELDS[f]
+
+
+@pytest.mark.parametrize("order,het,n,double", [(4, False, 70, False), (8, True, 45, True), (2, False, 130, False),
+                                                 (4, True, 40, False)])
+def test_fused_z_faces_equal_separate_launches(order, het, n, double):
+    """The deep z-face rows of the stress phase, split off and run as one
+    launch ahead of the other stress boundary launches, give the same bits
+    as the reference statement order (planner switch off)."""
+    if het:
+        g, _ = hetero_model(n, order, 9, double=double, sigma=2.0)
+    else:
+        g, _ = eigen_model(n, order, 9, double=double)
+    g.fuse_boundary = True
+    plan = g.lower()
+    assert plan.fused_frame > 0 and (plan.stmts['phase'] == 4).any()
+    with Context(plan) as ctx:
+        launches, deep, deep_launches = ctx.stats()
+    assert deep > 0 and deep_launches == 1, (launches, deep, deep_launches)
+    a, _ = gpu_run(plan)
+    g.fuse_boundary = False
+    g._invalidate()
+    plan2 = g.lower()
+    assert not (plan2.stmts['phase'] == 4).any()
+    b, _ = gpu_run(plan2)
+    for f in range(9):
+        assert np.array_equal(a[f], b[f]), FIELDS[f]

@@ -49,11 +49,15 @@ def test_plan_matches_oracle_heterogeneous(order):
         assert np.abs(vm.latest()[f] - ref['fields'][f]).max() / scale < 1e-8, oracle.FIELDS[f]
 
 
-def test_schedule_equals_sequential_order():
+@pytest.mark.parametrize("split", [False, True])
+def test_schedule_equals_sequential_order(split):
     """Hazard-scheduled boundary launches reproduce the reference's sequential
-    loop order bit for bit."""
+    loop order bit for bit, also with the deep z-face rows of the stress
+    phase split off and moved ahead of every other stress statement."""
     g, _ = eigen_model(13, 4, 6)
+    g.fuse_boundary = split
     plan = g.lower()
+    assert (plan.stmts['phase'] == 4).any() == split
     a, b = PlanVM(plan, True), PlanVM(plan, False)
     for vm in (a, b):
         vm.init()
