@@ -147,9 +147,11 @@ typedef struct OpesciModelDesc {
     const OpesciExprOp* sol_code;
     /* point source (Ricker etc.) into Txx/Tyy/Tzz and receivers
      * (tests/src/test_ref_iso_elastic.cpp:227-290) */
-    int32_t src_on;
+    int32_t src_on;             /* 0 none; 1 one trace for Txx, Tyy, Tzz; 3 traces x, y, z */
     int32_t src_idx[3];
-    const float* src_wavelet;   /* ntsteps values; Tnn -= w[t]/3 after the velocity step */
+    const float* src_wavelet;   /* src_on * ntsteps values (x trace, then y, then z);
+                                   Txx -= wx[t]/3, Tyy -= wy[t]/3, Tzz -= wz[t]/3 after the
+                                   velocity step and the receiver sampling */
     int32_t nrec;
     int32_t pad2;
     const int32_t* rec_idx;     /* 3*nrec grid indices */

@@ -162,33 +162,42 @@ __global__ void zero_kernel(uint4* __restrict__ p, size_t n16) {
 // U, V, W and (Txx+Tyy+Tzz)/3 at every receiver, then Tnn -= w[step]/3 at the
 // source.  The step counter lives on the device so a captured graph can be
 // replayed.
+// Receivers (tests/src/test_ref_iso_elastic.cpp:227-272): U, V, W and
+// (Txx+Tyy+Tzz)/3 at each receiver's grid point after the velocity update,
+// one thread per receiver, trace[step][receiver][4].
 template <typename R, typename S>
-__global__ __launch_bounds__(256) void record_source_kernel(Geom g, FieldPtrs<S> F, int t1, int nrec,
-                                                            const int32_t* __restrict__ rec,
-                                                            float* __restrict__ trace, int src_on, int sx, int sy,
-                                                            int sz, const float* __restrict__ w, int* step,
-                                                            int ntsteps) {
+__global__ __launch_bounds__(256) void receiver_kernel(Geom g, FieldPtrs<S> F, int t1, int nrec,
+                                                       const int32_t* __restrict__ rec, float* __restrict__ trace,
+                                                       const int* __restrict__ step, int ntsteps) {
     using C = Conv<S, R>;
     const int s = *step;
-    if (s < ntsteps && trace) {
-        for (int r = threadIdx.x; r < nrec; r += blockDim.x) {
-            const int64_t q = g.at(rec[3 * r], rec[3 * r + 1], rec[3 * r + 2]);
-            float* o = trace + ((size_t)s * nrec + r) * 4;
-            o[0] = (float)C::ld(F.f[0][t1] + q);
-            o[1] = (float)C::ld(F.f[1][t1] + q);
-            o[2] = (float)C::ld(F.f[2][t1] + q);
-            o[3] = (float)((C::ld(F.f[3][t1] + q) + C::ld(F.f[4][t1] + q) + C::ld(F.f[5][t1] + q)) / (R)3);
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ntsteps || r >= nrec) return;
+    const int64_t q = g.at(rec[3 * r], rec[3 * r + 1], rec[3 * r + 2]);
+    float4 o;
+    o.x = (float)C::ld(F.f[0][t1] + q);
+    o.y = (float)C::ld(F.f[1][t1] + q);
+    o.z = (float)C::ld(F.f[2][t1] + q);
+    o.w = (float)((C::ld(F.f[3][t1] + q) + C::ld(F.f[4][t1] + q) + C::ld(F.f[5][t1] + q)) / (R)3);
+    reinterpret_cast<float4*>(trace)[(size_t)s * nrec + r] = o;
+}
+
+// Point source (test_ref_iso_elastic.cpp:274-290), after the receivers:
+// Txx -= wx[t]/3, Tyy -= wy[t]/3, Tzz -= wz[t]/3 (one trace for all three
+// when ntr == 1); then the device step counter advances.
+template <typename R, typename S>
+__global__ void source_kernel(Geom g, FieldPtrs<S> F, int t1, int ntr, int sx, int sy, int sz,
+                              const float* __restrict__ w, int* step, int ntsteps) {
+    using C = Conv<S, R>;
+    const int s = *step;
+    if (ntr > 0 && s < ntsteps) {
+        const int64_t q = g.at(sx, sy, sz);
+        for (int f = 0; f < 3; ++f) {
+            const R v = (R)w[(ntr == 3 ? f : 0) * (size_t)ntsteps + s] / (R)3;
+            C::st(F.f[3 + f][t1] + q, C::ld(F.f[3 + f][t1] + q) - v);
         }
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        if (src_on && s < ntsteps) {
-            const int64_t q = g.at(sx, sy, sz);
-            const R v = (R)w[s] / (R)3;
-            for (int f = 3; f < 6; ++f) C::st(F.f[f][t1] + q, C::ld(F.f[f][t1] + q) - v);
-        }
-        *step = s + 1;
-    }
+    *step = s + 1;
 }
 
 // ------------------------------------------------------------------ launchers
@@ -334,11 +343,14 @@ hipError_t launch_zero(void* p, size_t bytes, hipStream_t s) {
 }
 
 template <typename R, typename S>
-static hipError_t rec_t(const Geom& g, void* fields, int t1, int nrec, const int32_t* rec, float* trace, int src_on,
+static hipError_t rec_t(const Geom& g, void* fields, int t1, int nrec, const int32_t* rec, float* trace, int ntr,
                         const int32_t* si, const float* w, int* step, int ntsteps, hipStream_t s) {
-    hipLaunchKernelGGL((record_source_kernel<R, S>), dim3(1), dim3(256), 0, s, g,
-                       *static_cast<const FieldPtrs<S>*>(fields), t1, nrec, rec, trace, src_on, si[0], si[1], si[2],
-                       w, step, ntsteps);
+    const FieldPtrs<S>& F = *static_cast<const FieldPtrs<S>*>(fields);
+    if (nrec > 0 && trace)
+        hipLaunchKernelGGL((receiver_kernel<R, S>), dim3((nrec + 255) / 256), dim3(256), 0, s, g, F, t1, nrec, rec,
+                           trace, step, ntsteps);
+    hipLaunchKernelGGL((source_kernel<R, S>), dim3(1), dim3(1), 0, s, g, F, t1, ntr, si[0], si[1], si[2], w, step,
+                       ntsteps);
     return hipGetLastError();
 }
 

@@ -58,7 +58,7 @@ struct OpesciContext {
     float* d_wavelet = nullptr;
     int* d_step = nullptr;
     int nrec = 0;
-    bool src_on = false;
+    int src_ntr = 0;                  // source traces: 0, 1 (all of Txx/Tyy/Tzz) or 3
     int32_t src_idx[3] = {0, 0, 0};
     hipGraphExec_t graph = nullptr;
     int steps = 0;
@@ -181,6 +181,7 @@ int validate(const OpesciModelDesc* d) {
             }
         }
     }
+    if (d->src_on != 0 && d->src_on != 1 && d->src_on != 3) return fail(OPESCI_EINVAL, "src_on must be 0, 1 or 3");
     if (d->src_on) {
         if (!d->src_wavelet) return fail(OPESCI_EINVAL, "source without wavelet");
         for (int a = 0; a < 3; ++a)
@@ -272,9 +273,9 @@ int step_once(OpesciContext* c, int t0) {
     OPESCI_HIP_TRY(launch_velocity(c->g, c->real_bytes, c->storage, c->het, c->fptr, c->mptr, c->coefs(), t0, t1,
                                    c->cfg, c->stream));
     if (int e = run_bc_phase(c, 3, t0, t1)) return e;
-    if (c->nrec > 0 || c->src_on)
+    if (c->nrec > 0 || c->src_ntr)
         OPESCI_HIP_TRY(launch_record_source(c->g, c->real_bytes, c->storage, c->fptr, t1, c->nrec, c->d_rec,
-                                            c->d_trace, c->src_on ? 1 : 0, c->src_idx, c->d_wavelet, c->d_step,
+                                            c->d_trace, c->src_ntr, c->src_idx, c->d_wavelet, c->d_step,
                                             c->desc.ntsteps, c->stream));
     return 0;
 }
@@ -387,16 +388,17 @@ int create(const OpesciModelDesc* d, OpesciContext** out) {
     c->partial_cap = NF * 512;
     OPESCI_HIP_TRY(hipMalloc((void**)&c->d_partial, c->partial_cap * sizeof(double)));
     c->nrec = d->nrec;
-    c->src_on = d->src_on != 0;
-    if (c->nrec || c->src_on) {
+    c->src_ntr = d->src_on;
+    if (c->nrec || c->src_ntr) {
         if (int e = alloc_zero(c.get(), (void**)&c->d_step, 256)) return e;
         if (c->nrec) {
             if (int e = upload(&c->d_rec, d->rec_idx, (size_t)3 * c->nrec, c->stream)) return e;
             const size_t tb = (size_t)std::max(d->ntsteps, 1) * c->nrec * 4 * sizeof(float);
             if (int e = alloc_zero(c.get(), (void**)&c->d_trace, tb)) return e;
         }
-        if (c->src_on) {
-            if (int e = upload(&c->d_wavelet, d->src_wavelet, (size_t)std::max(d->ntsteps, 1), c->stream)) return e;
+        if (c->src_ntr) {
+            if (int e = upload(&c->d_wavelet, d->src_wavelet, (size_t)std::max(d->ntsteps, 1) * c->src_ntr, c->stream))
+                return e;
             for (int a = 0; a < 3; ++a) c->src_idx[a] = d->src_idx[a];
         }
     }
@@ -606,9 +608,9 @@ int opesci_ctx_kernel_times(OpesciContext* ctx, int32_t nsteps, double out_ms[3]
                                        ctx->coefs(), t0, t1, ctx->cfg, ctx->stream));
         OPESCI_HIP_TRY(hipEventRecord(e[3], ctx->stream));
         if (!rc) rc = run_bc_phase(ctx, 3, t0, t1);
-        if (!rc && (ctx->nrec > 0 || ctx->src_on))
+        if (!rc && (ctx->nrec > 0 || ctx->src_ntr))
             OPESCI_HIP_TRY(launch_record_source(ctx->g, ctx->real_bytes, ctx->storage, ctx->fptr, t1, ctx->nrec,
-                                                ctx->d_rec, ctx->d_trace, ctx->src_on ? 1 : 0, ctx->src_idx,
+                                                ctx->d_rec, ctx->d_trace, ctx->src_ntr, ctx->src_idx,
                                                 ctx->d_wavelet, ctx->d_step, ctx->desc.ntsteps, ctx->stream));
         OPESCI_HIP_TRY(hipEventRecord(e[4], ctx->stream));
         ++ctx->steps;
@@ -654,7 +656,7 @@ int opesci_model_load(const OpesciModelDesc* desc) {
     const size_t sz_st = (size_t)desc->nstmts * sizeof(OpesciBcStmt), sz_te = (size_t)desc->nterms * sizeof(OpesciBcTerm);
     const size_t sz_co = (size_t)desc->ncode * sizeof(OpesciExprOp), sz_so = (size_t)desc->nsol * sizeof(OpesciExprOp);
     const size_t sz_me = desc->media_arrays ? 3 * dense * sizeof(float) : 0;
-    const size_t sz_w = desc->src_on ? (size_t)std::max(desc->ntsteps, 1) * sizeof(float) : 0;
+    const size_t sz_w = (size_t)std::max(desc->ntsteps, 1) * desc->src_on * sizeof(float);
     const size_t sz_r = (size_t)desc->nrec * 3 * sizeof(int32_t);
     g_loaded_blob.assign(sz_st + sz_te + sz_co + sz_so + sz_me + sz_w + sz_r + 64, 0);
     char* p = g_loaded_blob.data();

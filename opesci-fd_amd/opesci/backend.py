@@ -142,9 +142,9 @@ def make_desc(plan):
     d.sol_code = ptr(plan.sol_code)
     if plan.src_wavelet is not None:
         w = np.ascontiguousarray(plan.src_wavelet, dtype=np.float32)
-        if w.size < plan.ntsteps:
-            raise ValueError('source wavelet shorter than ntsteps')
-        d.src_on = 1
+        if w.ndim != 2 or w.shape[0] not in (1, 3) or w.shape[1] != max(plan.ntsteps, 1):
+            raise ValueError('source traces must be (1 or 3, ntsteps)')
+        d.src_on = w.shape[0]
         d.src_idx[:] = [int(v) for v in plan.src_idx]
         d.src_wavelet = ptr(w)
     if plan.rec_idx is not None:

@@ -26,6 +26,8 @@ class Grid(object):
     _arg_profiling = None
     _plan = None
     _ctx = None
+    _ctx_output = None
+    _host_fields = None
 
     def __init__(self):
         self._compiler = HIPCompiler()
@@ -109,12 +111,45 @@ class Grid(object):
         self._arg_grid = OpesciGrid()
         self._arg_profiling = OpesciProfiling()
         print('Executing on %s' % backend.device_string())
-        backend.execute(plan, self._arg_grid, self._arg_profiling)
+        if getattr(self, 'output_vts', False):
+            self._execute_with_output(plan, real)
+        else:
+            self._ctx_output = None
+            backend.execute(plan, self._arg_grid, self._arg_profiling)
         self._executed_plan = plan
         if self.profiling:
             print('PAPI:: Max real_time: %f (sec)' % self._arg_profiling.rtime)
             print('PAPI:: Max proc_time: %f (sec)' % self._arg_profiling.ptime)
             print('PAPI:: Total MFlops/s: %f' % self._arg_profiling.mflops)
+
+    def _execute_with_output(self, plan, real):
+        """Time loop with the first field written after every step as
+        <name>_<step>.vts (RegularGrid.output_step, opesci/regulargrid.py:
+        702-719; the template calls it at the end of each step).  Runs one
+        step per launch on a context and keeps the context for
+        convergence(); the final fields come back through the OpesciGrid
+        struct as with opesci_execute."""
+        import numpy as np
+        from . import backend
+        from .output import write_vts
+        if self._ctx_output is not None:
+            self._ctx_output.close()
+        ctx = backend.Context(plan)
+        ctx.init()
+        name = str(self.fields[0].label)
+        spacing = [float(s.value) for s in self.spacing]
+        secs = 0.0
+        for ti in range(plan.ntsteps):
+            secs += ctx.run(1)
+            write_vts('%s_%d' % (name, ti), ctx.download(0, -1), spacing, plan.M)
+        self._host_fields = []
+        for f in range(len(self.fields)):
+            both = np.stack([ctx.download(f, 0), ctx.download(f, 1)]).astype(np.float64 if real is ctypes.c_double
+                                                                               else np.float32)
+            self._host_fields.append(np.ascontiguousarray(both))
+            setattr(self._arg_grid, str(self.fields[f].label), self._host_fields[-1].ctypes.data_as(ctypes.POINTER(real)))
+        self._arg_profiling.rtime = self._arg_profiling.ptime = secs
+        self._ctx_output = ctx
 
     def convergence(self):
         """L2 norms of every field against its analytic solution."""
@@ -125,7 +160,11 @@ class Grid(object):
 You need to you run grid.execute() first!""")
         from . import backend
         plan = self._executed_plan
-        vals = backend.convergence(plan, self._arg_grid)
+        if getattr(self, '_ctx_output', None) is not None:
+            real = float if plan.real_bytes == 8 else (lambda v: float(__import__('numpy').float32(v)))
+            vals = [real(v) for v in self._ctx_output.l2()]
+        else:
+            vals = backend.convergence(plan, self._arg_grid)
         print('Convergence:')
         out = {}
         for f, v in zip(self.fields, vals):
@@ -149,6 +188,11 @@ You need to you run grid.execute() first!""")
     def free(self):
         """Release the host arrays handed out by execute() (opesci_free)."""
         from . import backend
+        if getattr(self, '_ctx_output', None) is not None:
+            self._ctx_output.close()
+            self._ctx_output = None
+            self._host_fields = None
+            self._arg_grid = None
         if self._arg_grid is not None:
             backend.free(self._arg_grid)
             self._arg_grid = None

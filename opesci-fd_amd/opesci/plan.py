@@ -553,6 +553,33 @@ def _media_arrays(grid, dims):
     return arrs
 
 
+# ---------------------------------------------------------------- sources
+def _sources(grid, p):
+    """Source traces as (1 or 3, ntsteps) float32, zero after their end (the
+    reference injects while ti < snt, tests/src/test_ref_iso_elastic.cpp:
+    275-290), the source index and the receiver indices (n, 3), all checked
+    against the grid."""
+    src, idx, rec = getattr(grid, 'src_wavelet', None), getattr(grid, 'src_idx', None), \
+        getattr(grid, 'rec_idx', None)
+    nt = max(p.ntsteps, 1)
+    if src is not None:
+        w = np.atleast_2d(np.asarray(src, dtype=np.float32))
+        if w.ndim != 2 or w.shape[0] not in (1, 3):
+            raise PlanError('a source is one trace or three (x, y, z)')
+        out = np.zeros((w.shape[0], nt), dtype=np.float32)
+        n = min(nt, w.shape[1])
+        out[:, :n] = w[:, :n]
+        src = out
+        idx = tuple(int(v) for v in idx)
+        if any(not (0 <= i < d) for i, d in zip(idx, p.dims)):
+            raise PlanError('source index %s outside the grid %s' % (idx, p.dims))
+    if rec is not None:
+        rec = np.ascontiguousarray(np.asarray(rec, dtype=np.int32).reshape(-1, 3))
+        if rec.size and ((rec < 0).any() or (rec >= np.asarray(p.dims)).any()):
+            raise PlanError('receiver outside the grid %s' % (p.dims,))
+    return src, (idx if idx is not None else (0, 0, 0)), rec
+
+
 # ---------------------------------------------------------------- entry
 def build_plan(grid):
     if grid.dimension != 3:
@@ -598,9 +625,7 @@ def build_plan(grid):
     p.fused_frame = b.fused_frame
     p.nlaunches_step = int(sum(len(set(p.stmts['sched'][p.stmts['phase'] == ph])) for ph in (2, 3)))
     p.sol_code, p.sol_off, p.sol_len = _solutions(grid, slots)
-    p.src_wavelet = getattr(grid, 'src_wavelet', None)
-    p.src_idx = getattr(grid, 'src_idx', (0, 0, 0))
-    p.rec_idx = getattr(grid, 'rec_idx', None)
+    p.src_wavelet, p.src_idx, p.rec_idx = _sources(grid, p)
     p.tile_y = getattr(grid, 'tile_y', 0)
     p.x_chunk = getattr(grid, 'x_chunk', 0)
     p.kernel_variant = getattr(grid, 'kernel_variant', 0)

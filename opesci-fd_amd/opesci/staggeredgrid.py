@@ -9,6 +9,7 @@ Same public API and semantics as opesci/staggeredgrid.py:
    initialise_bc, :754-879), here turned into a boundary program by
    opesci.plan instead of C loops.
 """
+import numpy as np
 from sympy import Eq, Indexed, Rational, Symbol, expand
 
 from .derivative import DDerivative
@@ -197,6 +198,43 @@ class StaggeredGrid(RegularGrid):
             self.set_variable('beta', 1.0 / rho, 'float', True)
             self.set_variable('lambda', rho * (vp ** 2 - 2 * vs ** 2), 'float', True)
             self.set_variable('mu', rho * (vs ** 2), 'float', True)
+        self._invalidate()
+
+    # ---------------- point source and receivers ----------------
+    def set_source(self, wavelet, location=None, index=None, sdt=None):
+        """Point stress source (tests/src/test_ref_iso_elastic.cpp:274-290):
+        after every velocity update and receiver sampling, Txx, Tyy and Tzz at
+        the source point decrease by w[t]/3 while t is inside the trace.
+        wavelet: one trace (the same for the three) or three (x, y, z);
+        location: model coordinates (nearest unstaggered grid point) or index:
+        grid indices; sdt: sampling interval of the traces when it differs
+        from the time step (spectral resampling, src/opesciHandy.cpp:125-195)."""
+        from . import sources
+        w = np.atleast_2d(np.asarray(wavelet, dtype=np.float32))
+        if w.ndim != 2 or w.shape[0] not in (1, 3):
+            raise ValueError('a source is one trace or three (x, y, z)')
+        if sdt is not None:
+            w = np.stack([sources.resample_timeseries(t, self.dt.value, sdt) for t in w])
+        if (location is None) == (index is None):
+            raise ValueError('give the source location or its grid index')
+        if index is None:
+            index = sources.nearest_index(location, [s.value for s in self.spacing], self.margin.value)
+        self.src_wavelet = w
+        self.src_idx = tuple(int(i) for i in index)
+        self._invalidate()
+
+    def set_receivers(self, locations=None, index=None):
+        """Receivers (test_ref_iso_elastic.cpp:227-272): U, V, W and the
+        pressure (Txx + Tyy + Tzz)/3 at each receiver's nearest grid point
+        after every velocity update, before the source injection.  Read the
+        traces with Context.receivers(): [step][receiver][U, V, W, p]."""
+        from . import sources
+        if (locations is None) == (index is None):
+            raise ValueError('give receiver locations or grid indices')
+        if index is None:
+            h = [s.value for s in self.spacing]
+            index = [sources.nearest_index(c, h, self.margin.value) for c in np.asarray(locations).reshape(-1, 3)]
+        self.rec_idx = np.asarray(index, dtype=np.int32).reshape(-1, 3)
         self._invalidate()
 
     def create_media_dict(self):

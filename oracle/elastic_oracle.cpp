@@ -30,6 +30,7 @@
 #include <cstring>
 #include <vector>
 #include <algorithm>
+#include <limits>
 #include <omp.h>
 
 extern "C" {
@@ -46,7 +47,7 @@ typedef struct OracleParams {
     int32_t bc_algo;      /* -1: reference rule (levander iff order==4); 0: robertsson; 1: levander;
                              2: levander without the normal-stress recompute (older generator) */
     int32_t ic_mode;      /* 0: analytic eigenmode (tests/eigenwave3d.py:87-108); 1: all zero */
-    int32_t src_on;       /* 1: inject src_wavelet at src_idx into Txx/Tyy/Tzz */
+    int32_t src_on;       /* 1: one trace for Txx/Tyy/Tzz, 3: x, y, z traces (3*ntsteps values) */
     int32_t src_idx[3];
     int32_t nrec;         /* receivers recorded each step (U,V,W,P) */
     int32_t pad0;
@@ -64,6 +65,7 @@ typedef struct OracleParams {
 int oracle_run(const OracleParams* p, void* fields_out[9], double l2_f64[9],
                double l2_ref[9], double timing[2]);
 int oracle_fd_weights(int half_order, double* w);
+int oracle_resample(const float* src, int snt, float dt, double sdt, float* out, int cap);
 int oracle_norms(const OracleParams* p, void* const fields_in[9], double l2_f64[9], double l2_ref[9]);
 }
 
@@ -864,10 +866,14 @@ struct Sim {
                 o[3] = (float)((atp(TXX, t1, q) + atp(TYY, t1, q) + atp(TZZ, t1, q)) / (R)3);
             }
             if (P.src_on) {
-                R s = (R)P.src_wavelet[ti] / (R)3;
-                atp(TXX, t1, P.src_idx) -= s;
-                atp(TYY, t1, P.src_idx) -= s;
-                atp(TZZ, t1, P.src_idx) -= s;
+                /* test_ref_iso_elastic.cpp:281-283: Txx -= xsrc/3, Tyy -= ysrc/3,
+                 * Tzz -= zsrc/3 (src_on == 1: one trace for all three) */
+                const float* w = P.src_wavelet;
+                const size_t nt = (size_t)P.ntsteps;
+                const size_t o1 = P.src_on == 3 ? nt : 0, o2 = P.src_on == 3 ? 2 * nt : 0;
+                atp(TXX, t1, P.src_idx) -= (R)w[ti] / (R)3;
+                atp(TYY, t1, P.src_idx) -= (R)w[o1 + ti] / (R)3;
+                atp(TZZ, t1, P.src_idx) -= (R)w[o2 + ti] / (R)3;
             }
         }
     }
@@ -993,4 +999,57 @@ extern "C" int oracle_fd_weights(int half_order, double* w) {
     if (half_order < 1 || half_order > 4) return 1;
     for (int k = 0; k < half_order; ++k) w[k] = FDW[half_order][k];
     return 0;
+}
+
+/* Spectral resampling of a trace from interval sdt to dt
+ * (opesci_resample_timeseries, src/opesciHandy.cpp:117-195): unitary float
+ * DFT (angles in double, sums in float, as opesci_dft :100-114 computes
+ * them), spectrum middle padded with zeros (dt < sdt) or cut (dt > sdt),
+ * inverse by conjugating and transforming forward again.  Writes
+ * round(snt*sdt/dt) samples (or a copy when dt == sdt to float precision);
+ * returns that count, or -1 when cap is too small. */
+namespace {
+void dft_float(const float* re, const float* im, float* ore, float* oim, int n) {
+    for (int k = 0; k < n; ++k) {
+        float sr = 0, si = 0;
+        for (int t = 0; t < n; ++t) {
+            const double ang = 2 * M_PI * t * k / n;
+            sr += re[t] * cos(ang) + im[t] * sin(ang);
+            si += -re[t] * sin(ang) + im[t] * cos(ang);
+        }
+        ore[k] = sr;
+        oim[k] = si;
+    }
+}
+}  // namespace
+
+extern "C" int oracle_resample(const float* src, int snt, float dt, double sdt, float* out, int cap) {
+    if (std::fabs(dt - sdt) < std::numeric_limits<float>::epsilon() * (dt + sdt)) {
+        if (cap < snt) return -1;
+        for (int i = 0; i < snt; ++i) out[i] = src[i];
+        return snt;
+    }
+    const int snt2 = (int)std::round(snt * sdt / dt);
+    if (cap < snt2) return -1;
+    std::vector<float> zero(snt, 0.f), fr(snt), fi(snt), gr(snt2, 0.f), gi(snt2, 0.f), oi(snt2);
+    dft_float(src, zero.data(), fr.data(), fi.data(), snt);
+    const float nrm = 1.f / std::sqrt((float)snt);
+    for (int i = 0; i < snt; ++i) {
+        fr[i] *= nrm;
+        fi[i] *= nrm;
+    }
+    if (dt < sdt) {
+        const int mid = snt / 2, off = snt2 - snt;
+        for (int i = 0; i < mid; ++i) { gr[i] = fr[i]; gi[i] = fi[i]; }
+        for (int i = mid; i < snt; ++i) { gr[off + i] = fr[i]; gi[off + i] = fi[i]; }
+    } else {
+        const int mid = snt2 / 2, off = snt - snt2;
+        for (int i = 0; i < mid; ++i) { gr[i] = fr[i]; gi[i] = fi[i]; }
+        for (int i = mid; i < snt2; ++i) { gr[i] = fr[off + i]; gi[i] = fi[off + i]; }
+    }
+    for (int i = 0; i < snt2; ++i) gi[i] = -gi[i];
+    dft_float(gr.data(), gi.data(), out, oi.data(), snt2);
+    const float nrm2 = 1.f / std::sqrt((float)snt2);
+    for (int i = 0; i < snt2; ++i) out[i] *= nrm2;
+    return snt2;
 }

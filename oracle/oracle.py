@@ -83,6 +83,22 @@ def fd_weights(half_order):
     return list(w)
 
 
+def resample(src, dt, sdt):
+    """opesci_resample_timeseries restated in C (float DFT), see
+    oracle_resample."""
+    src = np.ascontiguousarray(src, dtype=np.float32)
+    cap = int(round(src.size * sdt / float(np.float32(dt)))) + src.size + 8
+    out = np.zeros(cap, dtype=np.float32)
+    lib = load()
+    lib.oracle_resample.restype = ctypes.c_int
+    lib.oracle_resample.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_int, ctypes.c_float, ctypes.c_double,
+                                    ctypes.POINTER(ctypes.c_float), ctypes.c_int]
+    n = lib.oracle_resample(src.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), src.size, float(dt), float(sdt),
+                            out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), cap)
+    assert n >= 0
+    return out[:n].copy()
+
+
 def media_from_velocities(rho, vp, vs):
     """StaggeredGrid.set_media_params(read=False) (opesci/staggeredgrid.py:279-282)."""
     return rho, rho * (vp ** 2 - 2 * vs ** 2), rho * vs ** 2
@@ -168,10 +184,12 @@ def run(order=4, n=(16, 16, 16), ntsteps=10, dt=None, L=(1.0, 1.0, 1.0), double=
         p.rho_arr, p.vp_arr, p.vs_arr = [a.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) for a in arrs]
     rec_out = None
     if src is not None:
-        s = np.ascontiguousarray(src, dtype=np.float32)
-        assert s.size >= ntsteps
+        # one trace (Txx, Tyy, Tzz alike) or three (x, y, z), each >= ntsteps long
+        s = np.atleast_2d(np.asarray(src, dtype=np.float32))
+        assert s.shape[0] in (1, 3) and s.shape[1] >= ntsteps
+        s = np.ascontiguousarray(s[:, :max(ntsteps, 1)])
         keep.append(s)
-        p.src_on = 1
+        p.src_on = s.shape[0]
         p.src_wavelet = s.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
         p.src_idx[:] = src_idx
     if rec_idx is not None:
