@@ -15,17 +15,31 @@ from .staggeredgrid import StaggeredGrid
 __all__ = ['eigenwave3d']
 
 
+class _NoSolution(object):
+    """Field proxy that ignores set_analytic_solution (fields start at rest)."""
+
+    def __init__(self, field):
+        self._f = field
+
+    def set_analytic_solution(self, expr):
+        pass
+
+    def __getattr__(self, name):
+        return getattr(self._f, name)
+
+
 def eigenwave3d(domain_size, grid_size, dt, tmax, output_vts=False, o_converge=True,
                 accuracy_order=(2, 4, 4, 4), omp=True, simd=False, ivdep=True, double=False,
                 pluto=False, filename='test.cpp', read=False, expand=True, eval_const=True,
                 rho_file='', vp_file='', vs_file='', fission=False, rho=1.0, vp=1.0, vs=0.5,
                 lam=None, mu=None, media_arrays=None, storage='native', legacy_levander=False,
-                verbose=False):
+                verbose=False, initial='eigenmode'):
     """Build and return the eigenwave StaggeredGrid (same arguments as the
     reference, plus: rho/vp/vs or lam/mu for constant media, media_arrays =
     (rho, vp, vs) numpy arrays for the read path, storage = 'native'|'fp16'|'bf16',
     legacy_levander = the older generator's surfaces without the normal-stress
-    recompute, as in benchmarks/test3d.cpp)."""
+    recompute, as in benchmarks/test3d.cpp; initial = 'zero' starts from rest
+    instead of the eigenmode, for source-driven runs)."""
     if verbose:
         print('domain size: ' + str(domain_size))
         print('grid size: ' + str(grid_size))
@@ -66,6 +80,11 @@ def eigenwave3d(domain_size, grid_size, dt, tmax, output_vts=False, o_converge=T
         print('require dt < ' + str(grid.get_time_step_limit()))
     Omega = pi * sqrt(2 * mu_s * beta)
     A = sqrt(2 * mu_s / beta)
+    if initial not in ('eigenmode', 'zero'):
+        raise ValueError("initial must be 'eigenmode' or 'zero'")
+    if initial == 'zero':
+        U = _NoSolution(U)
+        V, W, Txx, Tyy, Tzz, Txy, Tyz, Txz = (_NoSolution(f) for f in (V, W, Txx, Tyy, Tzz, Txy, Tyz, Txz))
     U.set_analytic_solution(cos(pi * x) * (sin(pi * y) - sin(pi * z)) * cos(Omega * t))
     V.set_analytic_solution(cos(pi * y) * (sin(pi * z) - sin(pi * x)) * cos(Omega * t))
     W.set_analytic_solution(cos(pi * z) * (sin(pi * x) - sin(pi * y)) * cos(Omega * t))

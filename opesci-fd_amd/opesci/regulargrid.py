@@ -105,6 +105,16 @@ class RegularGrid(Grid):
         self.dt.value = dt
         self.ntsteps.value = int(tmax / dt)
 
+    def set_papi_events(self, events=()):
+        """Hardware events to report with profiling (opesci/regulargrid.py:
+        346-371).  The names are kept; on this backend counters come from
+        rocprofv3 (e.g. --pmc FETCH_SIZE), not from PAPI."""
+        self._papi_events = list(events)
+
+    @property
+    def numevents_papi(self):
+        return len(self._papi_events)
+
     def set_switches(self, **kwargs):
         for switch, value in kwargs.items():
             if switch in self._extra_switches:

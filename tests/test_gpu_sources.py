@@ -117,3 +117,23 @@ def test_execute_with_vts_output(tmp_path, monkeypatch, capsys):
     for k in conv:
         assert conv[k] == pytest.approx(conv_plain[k], rel=1e-12, abs=1e-15)
     g.free()
+
+
+def test_precision_sweep_small(tmp_path):
+    """16-bit field storage against FP64 on a short 8th-order run (64^3,
+    300 steps, Ricker source, 48 x 48 receiver plane): FP32 storage within
+    1e-5, FP16 within 2e-2 and BF16 within 1e-1 relative L2 (per field and
+    on the receivers).  Longer, finer runs stagnate in 16 bits (increments
+    below half an ulp are lost)."""
+    import json
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), 'opesci-fd_amd'))
+    import precision_sweep
+    out = tmp_path / 'p.json'
+    precision_sweep.main(['--size', '64', '--order', '8', '--steps', '300', '--plane', '48', '--json', str(out)])
+    res = json.loads(out.read_text())['storages']
+    for kind, bound in (('fp32', 1e-5), ('fp16', 2e-2), ('bf16', 1e-1)):
+        errs = list(res[kind]['field_rel_l2'].values()) + [res[kind]['receiver_rel_l2']]
+        assert max(errs) < bound, (kind, max(errs))
+        assert min(errs) > 0.0
