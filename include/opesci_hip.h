@@ -220,6 +220,22 @@ int opesci_ctx_kernel_times(OpesciContext* ctx, int32_t nsteps, double out_ms[3]
  * counterpart. */
 int opesci_ctx_stats(OpesciContext* ctx, int32_t out[4]);
 
+/* Launch configuration of the interior kernels (kernel_variant 0 = sweep,
+ * 1 = gather; tile_y 0 (default) | 4 | 8; x_chunk 0 = default), used by the
+ * autotuner.  Results do not depend on it (same arithmetic in every
+ * configuration).  No reference counterpart. */
+int opesci_ctx_configure(OpesciContext* ctx, int32_t kernel_variant, int32_t tile_y, int32_t x_chunk);
+
+/* Batches: independent contexts (any shapes, orders, media) stepped together
+ * by one graph launch per two time steps, their kernels running side by
+ * side.  Every context must be initialised and at the same step.  The
+ * contexts stay owned by the caller and must outlive the batch.  No
+ * reference counterpart (the reference runs one model per process). */
+typedef struct OpesciBatch OpesciBatch;
+int opesci_batch_create(OpesciContext** ctxs, int32_t n, OpesciBatch** out);
+int opesci_batch_run(OpesciBatch* batch, int32_t nsteps, double* seconds);
+int opesci_batch_destroy(OpesciBatch* batch);
+
 const char* opesci_last_error(void);
 int opesci_abi_version(void);
 /* device name and architecture of device 0 ("" if none) */

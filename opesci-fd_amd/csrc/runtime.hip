@@ -644,6 +644,150 @@ int opesci_ctx_stats(OpesciContext* ctx, int32_t out[4]) {
     return 0;
 }
 
+int opesci_ctx_configure(OpesciContext* ctx, int32_t kernel_variant, int32_t tile_y, int32_t x_chunk) {
+    if (!ctx) return fail(OPESCI_EINVAL, "null context");
+    if ((kernel_variant != 0 && kernel_variant != 1) || (tile_y != 0 && tile_y != 4 && tile_y != 8) || x_chunk < 0)
+        return fail(OPESCI_EINVAL, "kernel_variant 0|1, tile_y 0|4|8, x_chunk >= 0");
+    ctx->cfg = InteriorLaunch{kernel_variant, tile_y, x_chunk};
+    ctx->desc.kernel_variant = kernel_variant;
+    ctx->desc.tile_y = tile_y;
+    ctx->desc.x_chunk = x_chunk;
+    if (ctx->graph) {
+        OPESCI_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        (void)hipGraphExecDestroy(ctx->graph);
+        ctx->graph = nullptr;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------- batches
+// Independent contexts stepped together: one graph holds two time steps of
+// every context as parallel branches (fork from and join into the batch's
+// own stream), so small grids of different shapes and orders fill the GPU
+// side by side and one graph launch advances them all.
+struct OpesciBatch {
+    std::vector<OpesciContext*> ctx;
+    hipStream_t origin = nullptr;
+    std::vector<hipEvent_t> ev;  // one join event per context + the fork event
+    hipGraphExec_t graph = nullptr;
+};
+
+static void batch_free(OpesciBatch* b) {
+    if (!b) return;
+    if (b->origin) (void)hipStreamSynchronize(b->origin);
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    for (hipEvent_t e : b->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (b->origin) (void)hipStreamDestroy(b->origin);
+    delete b;
+}
+
+static int batch_graph(OpesciBatch* b) {
+    if (b->graph) return 0;
+    const int n = (int)b->ctx.size();
+    hipGraph_t gr = nullptr;
+    OPESCI_HIP_TRY(hipStreamBeginCapture(b->origin, hipStreamCaptureModeThreadLocal));
+    int e = 0;
+    hipError_t he = hipEventRecord(b->ev[n], b->origin);
+    for (int i = 0; i < n && !e && he == hipSuccess; ++i) {
+        OpesciContext* c = b->ctx[i];
+        he = hipStreamWaitEvent(c->stream, b->ev[n], 0);
+        if (he == hipSuccess) e = step_once(c, 0);
+        if (!e && he == hipSuccess) e = step_once(c, 1);
+        if (!e && he == hipSuccess) he = hipEventRecord(b->ev[i], c->stream);
+        if (!e && he == hipSuccess) he = hipStreamWaitEvent(b->origin, b->ev[i], 0);
+    }
+    hipError_t he2 = hipStreamEndCapture(b->origin, &gr);
+    if (e || he != hipSuccess || he2 != hipSuccess) {
+        if (gr) (void)hipGraphDestroy(gr);
+        if (e) return e;
+        return fail(OPESCI_EHIP, std::string("batch capture: ") + hipGetErrorString(he != hipSuccess ? he : he2));
+    }
+    he = hipGraphInstantiate(&b->graph, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (he != hipSuccess) {
+        b->graph = nullptr;
+        return fail(OPESCI_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(he));
+    }
+    return 0;
+}
+
+int opesci_batch_create(OpesciContext** ctxs, int32_t n, OpesciBatch** out) {
+    if (!ctxs || !out || n <= 0) return fail(OPESCI_EINVAL, "batch needs at least one context");
+    *out = nullptr;
+    for (int i = 0; i < n; ++i) {
+        if (!ctxs[i]) return fail(OPESCI_EINVAL, "null context in batch");
+        for (int j = 0; j < i; ++j)
+            if (ctxs[j] == ctxs[i]) return fail(OPESCI_EINVAL, "context listed twice in batch");
+    }
+    OpesciBatch* b = new OpesciBatch();
+    b->ctx.assign(ctxs, ctxs + n);
+    b->ev.assign(n + 1, nullptr);
+    hipError_t he = hipStreamCreateWithFlags(&b->origin, hipStreamNonBlocking);
+    for (int i = 0; i <= n && he == hipSuccess; ++i) he = hipEventCreateWithFlags(&b->ev[i], hipEventDisableTiming);
+    if (he != hipSuccess) {
+        batch_free(b);
+        return fail(OPESCI_EHIP, std::string("batch setup: ") + hipGetErrorString(he));
+    }
+    *out = b;
+    return 0;
+}
+
+int opesci_batch_destroy(OpesciBatch* b) {
+    batch_free(b);
+    return 0;
+}
+
+int opesci_batch_run(OpesciBatch* b, int32_t nsteps, double* seconds) {
+    if (!b) return fail(OPESCI_EINVAL, "null batch");
+    if (nsteps < 0) return fail(OPESCI_EINVAL, "negative step count");
+    const int s0 = b->ctx[0]->steps;
+    for (OpesciContext* c : b->ctx) {
+        if (!c->initialised) return fail(OPESCI_ESTATE, "every context of the batch must be initialised");
+        if (c->steps != s0) return fail(OPESCI_ESTATE, "contexts of a batch must be at the same step");
+    }
+    // the graph holds an even/odd step pair: bring an odd start to even first
+    int left = nsteps;
+    double extra = 0;
+    if (s0 % 2 == 1 && left > 0) {
+        for (OpesciContext* c : b->ctx) {
+            double s = 0;
+            if (int e = ctx_run(c, 1, &s)) return e;
+            extra += s;
+        }
+        --left;
+    }
+    if (left >= 2)
+        if (int e = batch_graph(b)) return e;
+    hipEvent_t e0, e1;
+    OPESCI_HIP_TRY(hipEventCreate(&e0));
+    OPESCI_HIP_TRY(hipEventCreate(&e1));
+    for (OpesciContext* c : b->ctx) OPESCI_HIP_TRY(hipStreamSynchronize(c->stream));
+    OPESCI_HIP_TRY(hipEventRecord(e0, b->origin));
+    int rc = 0;
+    const int reps = left / 2;
+    for (int r = 0; r < reps && !rc; ++r) {
+        hipError_t he = hipGraphLaunch(b->graph, b->origin);
+        if (he != hipSuccess) rc = fail(OPESCI_EHIP, std::string("hipGraphLaunch: ") + hipGetErrorString(he));
+    }
+    OPESCI_HIP_TRY(hipEventRecord(e1, b->origin));
+    OPESCI_HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0;
+    OPESCI_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    for (OpesciContext* c : b->ctx) c->steps += 2 * reps;
+    if (left % 2 == 1)
+        for (OpesciContext* c : b->ctx) {
+            double s = 0;
+            if (int e = ctx_run(c, 1, &s)) return e;
+            extra += s;
+        }
+    if (seconds) *seconds = ms * 1e-3 + extra;
+    return 0;
+}
+
 // ---------------------------------------------------------------- reference-named ABI
 int opesci_model_load(const OpesciModelDesc* desc) {
     if (int e = validate(desc)) return e;

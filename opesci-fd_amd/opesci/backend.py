@@ -52,7 +52,8 @@ _LIB = {}
 EXPORTS = ('opesci_model_load', 'opesci_execute', 'opesci_convergence', 'opesci_free',
            'opesci_ctx_create', 'opesci_ctx_destroy', 'opesci_ctx_init', 'opesci_ctx_run',
            'opesci_ctx_l2', 'opesci_ctx_download', 'opesci_ctx_upload', 'opesci_ctx_receivers',
-           'opesci_ctx_steps', 'opesci_ctx_kernel_times', 'opesci_ctx_stats', 'opesci_last_error',
+           'opesci_ctx_steps', 'opesci_ctx_kernel_times', 'opesci_ctx_stats', 'opesci_ctx_configure',
+           'opesci_batch_create', 'opesci_batch_run', 'opesci_batch_destroy', 'opesci_last_error',
            'opesci_abi_version', 'opesci_device_info')
 
 
@@ -79,6 +80,10 @@ def load(path=None):
         lib.opesci_ctx_steps.argtypes = [_vp, ctypes.POINTER(_i32)]
         lib.opesci_ctx_kernel_times.argtypes = [_vp, _i32, ctypes.POINTER(_f64)]
         lib.opesci_ctx_stats.argtypes = [_vp, ctypes.POINTER(_i32)]
+        lib.opesci_ctx_configure.argtypes = [_vp, _i32, _i32, _i32]
+        lib.opesci_batch_create.argtypes = [ctypes.POINTER(_vp), _i32, ctypes.POINTER(_vp)]
+        lib.opesci_batch_run.argtypes = [_vp, _i32, ctypes.POINTER(_f64)]
+        lib.opesci_batch_destroy.argtypes = [_vp]
         lib.opesci_model_load.argtypes = [ctypes.POINTER(OpesciModelDesc)]
         lib.opesci_execute.argtypes = [_vp, _vp]
         lib.opesci_convergence.argtypes = [_vp, _vp]
@@ -258,3 +263,7 @@ class Context(object):
         out = (_i32 * 4)()
         _check(self.lib.opesci_ctx_stats(self.h, out), 'opesci_ctx_stats')
         return tuple(out[:3])
+
+    def configure(self, kernel_variant=0, tile_y=0, x_chunk=0):
+        """Interior launch configuration (results are the same for every one)."""
+        _check(self.lib.opesci_ctx_configure(self.h, kernel_variant, tile_y, x_chunk), 'opesci_ctx_configure')

@@ -35,3 +35,35 @@ def hetero_model(N, order, steps, double=False, storage='native', seed=2016, sig
                            accuracy_order=[2, order, order, order], double=double, storage=storage,
                            media_arrays=(rho, vp, vs))
     return g, dict(order=order, n=(n, n, n), ntsteps=steps, dt=dt, double=double, media=(rho, vp, vs))
+
+
+def box_model(sizes, order, steps, hetero=False, double=False, seed=2016, sigma=4.0):
+    """Eigenwave (or the seeded smooth medium) on the unit cube with a
+    different number of interior points per axis (dx_k = 1/(N_k - 1))."""
+    n = tuple(int(v) - 1 for v in sizes)
+    dmin = min(1.0 / v for v in n)
+    if hetero:
+        dims = tuple(int(v) + order for v in sizes)
+        rho, vp, vs = smooth_random_media(dims, seed=seed, sigma=sigma)
+        dt = stable_dt(dmin, float(vp.max()), order=4)
+        g = opesci.eigenwave3d((1.0, 1.0, 1.0), n, dt, dt * (steps + 0.5), accuracy_order=[2, order, order, order],
+                               double=double, media_arrays=(rho, vp, vs))
+        return g, dict(order=order, n=n, ntsteps=steps, dt=dt, double=double, media=(rho, vp, vs))
+    dt = 0.5 * dmin / np.sqrt(3.0 * 3.0)
+    g = opesci.eigenwave3d((1.0, 1.0, 1.0), n, dt, dt * (steps + 0.5), accuracy_order=[2, order, order, order],
+                           rho=1.0, vp=np.sqrt(3.0), vs=1.0, double=double)
+    g.set_variable('lambda', 1.0, 'float', True)
+    g.set_variable('mu', 1.0, 'float', True)
+    g._invalidate()
+    return g, dict(order=order, n=n, ntsteps=steps, dt=dt, rho=1.0, lam=1.0, mu=1.0, double=double)
+
+
+def batch_specs(count=64, seed=4, lo=48, hi=160, orders=(2, 4, 8)):
+    """Seeded instance list: per-axis interior sizes uniform in [lo, hi],
+    order drawn from `orders`, every other instance heterogeneous."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        sizes = tuple(int(v) for v in rng.integers(lo, hi + 1, size=3))
+        out.append(dict(sizes=sizes, order=int(rng.choice(orders)), hetero=bool(i % 2)))
+    return out
