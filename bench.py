@@ -52,6 +52,8 @@ def main():
     ap.add_argument('--x-chunk', type=int, default=0)
     ap.add_argument('--tile-y', type=int, default=0)
     ap.add_argument('--copy-probe', type=int, default=0, help='MiB copied device-to-device 3x first')
+    ap.add_argument('--hetero', action='store_true',
+                    help='seeded smooth heterogeneous medium through the read path (8 more media streams)')
     ap.add_argument('--split-z', action='store_true',
                     help='run the deep rows of the stress z-face loops as one launch ahead of the others')
     args = ap.parse_args()
@@ -62,13 +64,16 @@ def main():
 
     import numpy as np
     from opesci.backend import Context
-    from workloads import eigen_model
+    from workloads import eigen_model, hetero_model
 
     hip = _hip()
     if args.copy_probe:
         copy_probe(hip, args.copy_probe << 20)
     N, order, K, W = args.size, args.order, args.steps, args.warmup
-    g, kw = eigen_model(N, order, K + W, storage=args.storage)
+    if args.hetero:
+        g, kw = hetero_model(N, order, K + W, storage=args.storage)
+    else:
+        g, kw = eigen_model(N, order, K + W, storage=args.storage)
     if args.split_z:
         g.fuse_boundary = True
     plan = g.lower()
@@ -89,12 +94,13 @@ def main():
         l2 = ctx.l2()
         kt = ctx.kernel_times(10)   # per-launch averages, after the timed region
         bc_stats = ctx.stats()
-    workload_key = 'stress_sweep %d^3 order %d %s' % (N, order, args.storage)
+    workload_key = 'stress_sweep %d^3 order %d %s%s' % (N, order, args.storage, ' hetero' if args.hetero else '')
     ms_step = secs * 1e3 / K
     value = pts * K / secs
-    stress_bytes = pts * 15 * elem            # 9 loads + 6 stores per point
+    media_streams = (5, 3) if args.hetero else (0, 0)   # lambda, mu, mu12/13/23 | beta1..3 (FP32)
+    stress_bytes = pts * (15 * elem + 4 * media_streams[0])   # 9 loads + 6 stores (+ media) per point
     achieved = stress_bytes / (kt[0] * 1e-3)
-    step_bytes = pts * 27 * elem              # reference LOAD/STORE model, both kernels
+    step_bytes = pts * (27 * elem + 4 * sum(media_streams))   # reference LOAD/STORE model, both kernels
     out = {
         'metric': METRIC,
         'value': value,
@@ -109,8 +115,10 @@ def main():
         'dtype': 'f32' if elem == 4 else 'f32 compute, %s storage' % args.storage,
         'data': 'synthetic',
         'config': {
-            'workload': 'eigenwave3d %d^3 interior (dims %d^3), order %d space / 2 time, %s, rho=lambda=mu=1, '
-                        'dt=dx/6, Levander surfaces' % (N, N + order, order, args.storage),
+            'workload': ('eigenwave3d %d^3 interior (dims %d^3), order %d space / 2 time, %s, ' % (N, N + order, order,
+                                                                                         args.storage)) +
+                        ('seeded smooth medium (Vp 1.5-4.5, rho 1-2.5, sigma 8), read path, dt = 0.5 x CFL'
+                         if args.hetero else 'rho=lambda=mu=1, dt=dx/6') + ', Levander surfaces',
             'grid_points': int(pts),
             'l2_error_vs_analytic': dict(zip(('U', 'V', 'W', 'Txx', 'Tyy', 'Tzz', 'Txy', 'Tyz', 'Txz'),
                                              [float('%.6e' % v) for v in l2])),
@@ -132,7 +140,7 @@ def main():
         },
     }
     if not args.no_cpu:
-        out['cpu_baseline'] = cpu_baseline(N, order, args.cpu_steps)
+        out['cpu_baseline'] = cpu_baseline(N, order, args.cpu_steps, args.hetero)
     print(json.dumps(out))
 
 
@@ -176,19 +184,20 @@ def committed_traffic(key):
     return None
 
 
-def cpu_baseline(N, order, steps):
+def cpu_baseline(N, order, steps, hetero=False):
     sys.path.insert(0, os.path.join(ROOT, 'oracle'))
     import oracle
-    from workloads import eigen_model
-    _, kw = eigen_model(N, order, steps)
+    from workloads import eigen_model, hetero_model
+    _, kw = (hetero_model if hetero else eigen_model)(N, order, steps)
     cores = min(16, os.cpu_count() or 1)
     kw['nthreads'] = cores
     kw['want_fields'] = False
     r = oracle.run(**kw)
     pts = float(N) ** 3 * steps
     return {'value': pts / r['seconds'], 'unit': 'grid-pt updates/s', 'cores': cores, 'kind': 'port',
-            'sample': '%d steps of the same %d^3 order-%d FP32 eigenwave, oracle/elastic_oracle.cpp built with the '
-                      'reference GNU flags (-O3 -fno-tree-vectorize -fopenmp), time loop only' % (steps, N, order)}
+            'sample': '%d steps of the same %d^3 order-%d FP32 %s, oracle/elastic_oracle.cpp built with the '
+                      'reference GNU flags (-O3 -fno-tree-vectorize -fopenmp), time loop only'
+                      % (steps, N, order, 'heterogeneous model' if hetero else 'eigenwave')}
 
 
 if __name__ == '__main__':

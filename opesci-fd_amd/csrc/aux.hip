@@ -15,15 +15,35 @@ __device__ __forceinline__ double coord_of(double dx, int i, int m, int stagger)
 // One launch per hazard level, blockIdx.y = statement.  Statements of a
 // level write disjoint cells and never read what another statement of the
 // level writes.  Terms are loaded independently and summed in term order.
-template <typename R, typename S, bool PROG>
-__global__ __launch_bounds__(256) void bc_kernel(Geom g, FieldPtrs<S> F, MediaPtrs<R> md,
-                                                 const BcDesc* __restrict__ desc,
-                                                 const OpesciExprOp* __restrict__ code, int first, int t0, int t1) {
+template <typename R, typename S>
+__global__ __launch_bounds__(256) void bc_kernel(Geom g, FieldPtrs<S> F, const BcDesc* __restrict__ desc,
+                                                 const R* __restrict__ pool, int first, int t0, int t1) {
     const BcDesc* D = desc + first + blockIdx.y;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= D->npts) return;
     const int n1 = D->n1;
-    bc_point<R, S, PROG>(g, F, md, D, code, D->lo0 + p / n1, D->lo1 + p % n1, t0, t1);
+    bc_point<R, S>(g, F, D, pool, p, D->lo0 + p / n1, D->lo1 + p % n1, t0, t1);
+}
+
+// Per-point media-dependent coefficients of the statements [first,
+// first+count): coef * program, rounded to R exactly as bc_point does.
+template <typename R>
+__global__ __launch_bounds__(256) void bc_coef_kernel(Geom g, MediaPtrs<R> md, const BcDesc* __restrict__ desc,
+                                                      const OpesciExprOp* __restrict__ code, R* __restrict__ pool,
+                                                      int first) {
+    const BcDesc* D = desc + first + blockIdx.y;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= D->npts) return;
+    const int n1 = D->n1, axis = D->axis;
+    const int i0 = D->lo0 + p / n1, i1 = D->lo1 + p % n1;
+    const int ax = axis == 0 ? 0 : i0;
+    const int ay = axis == 0 ? i0 : (axis == 1 ? 0 : i1);
+    const int az = axis == 2 ? 0 : i1;
+    const double c0[3] = {0, 0, 0};
+    for (int k = 0; k < D->nterms; ++k)
+        if (D->t[k].cslot >= 0)
+            pool[D->cpool + (int64_t)D->t[k].cslot * D->npts + p] =
+                (R)((double)D->t[k].coef * vm_eval<R>(code + D->t[k].prog, D->prog_len[k], c0, 0.0, md, g, ax, ay, az));
 }
 
 struct SolTable {
@@ -202,30 +222,40 @@ __global__ void source_kernel(Geom g, FieldPtrs<S> F, int t1, int ntr, int sx, i
 
 // ------------------------------------------------------------------ launchers
 template <typename R, typename S>
-static hipError_t bc_t(const Geom& g, void* fields, const void* media, const BcDesc* desc, const OpesciExprOp* code,
-                       const BcLevel& lv, int t0, int t1, hipStream_t s) {
+static hipError_t bc_t(const Geom& g, void* fields, const BcDesc* desc, const void* pool, const BcLevel& lv, int t0,
+                       int t1, hipStream_t s) {
     if (lv.count <= 0 || lv.maxpts <= 0) return hipSuccess;
     dim3 grid((lv.maxpts + 255) / 256, lv.count);
-    if (lv.has_prog)
-        hipLaunchKernelGGL((bc_kernel<R, S, true>), grid, dim3(256), 0, s, g, *static_cast<const FieldPtrs<S>*>(fields),
-                           *static_cast<const MediaPtrs<R>*>(media), desc, code, lv.first, t0, t1);
-    else
-        hipLaunchKernelGGL((bc_kernel<R, S, false>), grid, dim3(256), 0, s, g,
-                           *static_cast<const FieldPtrs<S>*>(fields), *static_cast<const MediaPtrs<R>*>(media), desc,
-                           code, lv.first, t0, t1);
+    hipLaunchKernelGGL((bc_kernel<R, S>), grid, dim3(256), 0, s, g, *static_cast<const FieldPtrs<S>*>(fields), desc,
+                       static_cast<const R*>(pool), lv.first, t0, t1);
     return hipGetLastError();
+}
+
+template <typename R>
+static hipError_t bc_coef_t(const Geom& g, const void* media, const BcDesc* desc, const OpesciExprOp* code,
+                            void* pool, int first, int count, int maxpts, hipStream_t s) {
+    if (count <= 0 || maxpts <= 0) return hipSuccess;
+    dim3 grid((maxpts + 255) / 256, count);
+    hipLaunchKernelGGL((bc_coef_kernel<R>), grid, dim3(256), 0, s, g, *static_cast<const MediaPtrs<R>*>(media), desc,
+                       code, static_cast<R*>(pool), first);
+    return hipGetLastError();
+}
+
+hipError_t launch_bc_coef(const Geom& g, int real_bytes, const void* media, const BcDesc* d_desc,
+                          const OpesciExprOp* d_code, void* pool, int first, int count, int maxpts, hipStream_t s) {
+    return real_bytes == 8 ? bc_coef_t<double>(g, media, d_desc, d_code, pool, first, count, maxpts, s)
+                           : bc_coef_t<float>(g, media, d_desc, d_code, pool, first, count, maxpts, s);
 }
 
 #define OPESCI_TYPED(RB, ST, CALL_F, CALL_D, CALL_H, CALL_B) \
     ((RB) == 8 ? (CALL_D) : (ST) == OPESCI_STORE_F16 ? (CALL_H) : (ST) == OPESCI_STORE_BF16 ? (CALL_B) : (CALL_F))
 
-hipError_t launch_bc(const Geom& g, int real_bytes, int storage, void* fields, const void* media,
-                     const BcDesc* d_desc, const OpesciExprOp* d_code, const BcLevel& lvl, int t0, int t1,
-                     hipStream_t s) {
-    return OPESCI_TYPED(real_bytes, storage, (bc_t<float, float>(g, fields, media, d_desc, d_code, lvl, t0, t1, s)),
-                        (bc_t<double, double>(g, fields, media, d_desc, d_code, lvl, t0, t1, s)),
-                        (bc_t<float, _Float16>(g, fields, media, d_desc, d_code, lvl, t0, t1, s)),
-                        (bc_t<float, bf16_t>(g, fields, media, d_desc, d_code, lvl, t0, t1, s)));
+hipError_t launch_bc(const Geom& g, int real_bytes, int storage, void* fields, const BcDesc* d_desc, const void* pool,
+                     const BcLevel& lvl, int t0, int t1, hipStream_t s) {
+    return OPESCI_TYPED(real_bytes, storage, (bc_t<float, float>(g, fields, d_desc, pool, lvl, t0, t1, s)),
+                        (bc_t<double, double>(g, fields, d_desc, pool, lvl, t0, t1, s)),
+                        (bc_t<float, _Float16>(g, fields, d_desc, pool, lvl, t0, t1, s)),
+                        (bc_t<float, bf16_t>(g, fields, d_desc, pool, lvl, t0, t1, s)));
 }
 
 static SolTable make_tab(const int* off, const int* len, const double* dx) {

@@ -1,5 +1,6 @@
-// Device-side boundary statement evaluation: expression bytecode and the
-// per-point statement used by the boundary launches.
+// Device-side boundary statement evaluation: expression bytecode (media
+// programs, analytic solutions) and the per-point statement of the boundary
+// launches.
 #pragma once
 
 #include "common.h"
@@ -43,15 +44,15 @@ __device__ double vm_eval(const OpesciExprOp* code, int n, const double coord[3]
     return sp > 0 ? st[0] : 0.0;
 }
 
-// One boundary statement at one point of its loop: dst = sum_k coef_k * src_k,
-// terms loaded independently and accumulated with fma in term order.
-// PROG = false skips media programs (for statements that have none).
+// One boundary statement at point p of its loop: dst = sum_k coef_k * src_k,
+// terms loaded independently and accumulated with fma in term order.  A
+// term's coefficient is its literal or, for media-dependent ones (cslot >=
+// 0), the per-point value precomputed in the pool.
 // (i0, i1) are the loop indices of the two axes other than D->axis, in
 // increasing axis order.
-template <typename R, typename S, bool PROG = true>
-__device__ __forceinline__ void bc_point(const Geom& g, const FieldPtrs<S>& F, const MediaPtrs<R>& md,
-                                         const BcDesc* __restrict__ D, const OpesciExprOp* __restrict__ code, int i0,
-                                         int i1, int t0, int t1) {
+template <typename R, typename S>
+__device__ __forceinline__ void bc_point(const Geom& g, const FieldPtrs<S>& F, const BcDesc* __restrict__ D,
+                                         const R* __restrict__ pool, int p, int i0, int i1, int t0, int t1) {
     using C = Conv<S, R>;
     const int axis = D->axis, nt = D->nterms;
     const int o0 = axis == 0 ? 1 : 0, o1 = axis == 2 ? 1 : 2;
@@ -68,16 +69,8 @@ __device__ __forceinline__ void bc_point(const Geom& g, const FieldPtrs<S>& F, c
 #pragma unroll
     for (int k = 0; k < BC_MAXT; ++k)
         if (k < nt) {
-            R coef = (R)D->t[k].coef;
-            if (PROG && D->t[k].prog >= 0) {
-                // media anchor: the loop point, with the plane axis at 0
-                const int ax = axis == 0 ? 0 : i0;
-                const int ay = axis == 0 ? i0 : (axis == 1 ? 0 : i1);
-                const int az = axis == 2 ? 0 : i1;
-                const double c0[3] = {0, 0, 0};
-                coef = (R)((double)D->t[k].coef *
-                           vm_eval<R>(code + D->t[k].prog, D->prog_len[k], c0, 0.0, md, g, ax, ay, az));
-            }
+            const R coef = D->t[k].cslot >= 0 ? pool[D->cpool + (int64_t)D->t[k].cslot * D->npts + p]
+                                              : (R)D->t[k].coef;
             acc = fma(coef, v[k], acc);
         }
     const int fl = D->fl;

@@ -143,6 +143,7 @@ struct BcTermDev {
     int32_t disp;    // source - destination linear offset
     float coef;
     int32_t prog;    // -1 or offset of a media program (length in prog_len)
+    int32_t cslot;   // -1 or the term's slot in the statement's block of the coefficient pool
 };
 struct BcDesc {
     int32_t fl;      // destination field*2 + level
@@ -150,16 +151,20 @@ struct BcDesc {
     int32_t nterms;
     int32_t axis;
     int32_t lo0, lo1, n1, npts;
+    int32_t cpool;   // first pool element of this statement's per-point coefficients
     int32_t prog_len[BC_MAXT];
     BcTermDev t[BC_MAXT];
 };
 struct BcLevel {
     int first, count, maxpts;   // descriptors [first, first+count)
-    int has_prog;               // some term has a media program
 };
-hipError_t launch_bc(const Geom& g, int real_bytes, int storage, void* fields, const void* media,
-                     const BcDesc* d_desc, const OpesciExprOp* d_code, const BcLevel& lvl, int t0, int t1,
-                     hipStream_t s);
+// Media-dependent coefficients are time-invariant: bc_coef evaluates every
+// program once into the pool (pool[cpool + cslot*npts + point], compute
+// type R), and the step launches read them from there.
+hipError_t launch_bc(const Geom& g, int real_bytes, int storage, void* fields, const BcDesc* d_desc, const void* pool,
+                     const BcLevel& lvl, int t0, int t1, hipStream_t s);
+hipError_t launch_bc_coef(const Geom& g, int real_bytes, const void* media, const BcDesc* d_desc,
+                          const OpesciExprOp* d_code, void* pool, int first, int count, int maxpts, hipStream_t s);
 hipError_t launch_initial(const Geom& g, int real_bytes, int storage, void* fields, const void* media,
                           const OpesciExprOp* d_sol, const int* sol_off, const int* sol_len, const double* spacing,
                           double dt, hipStream_t s);

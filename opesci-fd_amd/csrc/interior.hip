@@ -478,10 +478,11 @@ static Tiling tiling(const Geom& g, const InteriorLaunch& cfg, int ty) {
     t.tyn = (ni[1] + ty - 1) / ty;
     int xc = cfg.x_chunk;
     if (xc <= 0) {
-        // about 2048 blocks: enough to cover 256 CUs several times over
+        // about 512 blocks (two per CU): long x sweeps load fewer x halos and
+        // measured fastest at 256^3 (x chunks 8..128 timed); 8..64 planes
         const int cols = t.tz * t.tyn;
-        xc = (ni[0] * cols + 2047) / 2048;
-        xc = xc < 8 ? 8 : xc;
+        xc = (ni[0] * cols + 511) / 512;
+        xc = xc < 8 ? 8 : (xc > 64 ? 64 : xc);
     }
     t.xchunk = xc;
     t.tx = (ni[0] + xc - 1) / xc;

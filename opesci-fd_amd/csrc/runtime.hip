@@ -47,6 +47,8 @@ struct OpesciContext {
     std::vector<OpesciBcStmt> stmts;  // sorted by (phase, sched, axis, ranges)
     std::vector<BcLevel> levels[5];    // phases 0..3 + 4 (deep z-face rows of the stress phase, one launch)
     BcDesc* d_desc = nullptr;
+    void* d_pool = nullptr;           // precomputed media-dependent boundary coefficients (compute type)
+    int64_t pool_elems = 0;
     OpesciExprOp* d_code = nullptr;
     OpesciExprOp* d_sol = nullptr;
     int sol_off[NF], sol_len[NF];
@@ -209,7 +211,7 @@ int build_program(OpesciContext* c, const OpesciBcTerm* terms) {
     const int n = (int)c->stmts.size();
     for (int i = 0; i < n;) {
         const OpesciBcStmt& L = c->stmts[i];
-        BcLevel lv{(int)descs.size(), 0, 0, 0};
+        BcLevel lv{(int)descs.size(), 0, 0};
         int j = i;
         for (; j < n && c->stmts[j].phase == L.phase && c->stmts[j].sched == L.sched; ++j) {
             const OpesciBcStmt& a = c->stmts[j];
@@ -226,6 +228,8 @@ int build_program(OpesciContext* c, const OpesciBcTerm* terms) {
             d.lo1 = a.lo[1];
             d.n1 = n1;
             d.npts = n0 * n1;
+            d.cpool = 0;
+            int nslot = 0;
             const int o[2] = {a.axis == 0 ? 1 : 0, a.axis == 2 ? 1 : 2};
             for (int k = 0; k < a.nterms; ++k) {
                 const OpesciBcTerm& tm = terms[a.term0 + k];
@@ -237,7 +241,13 @@ int build_program(OpesciContext* c, const OpesciBcTerm* terms) {
                     return fail(OPESCI_EINVAL, "boundary coefficient is not a float literal");
                 d.t[k].prog = tm.prog;
                 d.prog_len[k] = tm.prog_len;
-                if (tm.prog >= 0) lv.has_prog = 1;
+                d.t[k].cslot = tm.prog >= 0 ? nslot++ : -1;
+            }
+            if (nslot > 0) {
+                if (c->pool_elems + (int64_t)nslot * d.npts >= ((int64_t)1 << 31))
+                    return fail(OPESCI_EUNSUPPORTED, "boundary coefficient pool beyond int32 offsets");
+                d.cpool = (int32_t)c->pool_elems;
+                c->pool_elems += (int64_t)nslot * d.npts;
             }
             descs.push_back(d);
             lv.maxpts = std::max(lv.maxpts, d.npts);
@@ -259,8 +269,7 @@ int alloc_zero(OpesciContext* c, void** p, size_t bytes) {
 
 int run_bc_phase(OpesciContext* c, int phase, int t0, int t1) {
     for (const BcLevel& lv : c->levels[phase])
-        OPESCI_HIP_TRY(launch_bc(c->g, c->real_bytes, c->storage, c->fptr, c->mptr, c->d_desc, c->d_code, lv, t0,
-                                 t1, c->stream));
+        OPESCI_HIP_TRY(launch_bc(c->g, c->real_bytes, c->storage, c->fptr, c->d_desc, c->d_pool, lv, t0, t1, c->stream));
     return 0;
 }
 
@@ -310,8 +319,8 @@ void destroy(OpesciContext* c) {
             if (c->base[f][l]) (void)hipFree(c->base[f][l]);
     for (int i = 0; i < OPESCI_M_COUNT; ++i)
         if (c->mbase[i]) (void)hipFree(c->mbase[i]);
-    void* ptrs[] = {c->d_desc, c->d_code,    c->d_sol,  c->d_partial, c->d_stage, c->d_rec,
-                    c->d_trace, c->d_wavelet, c->d_step};
+    void* ptrs[] = {c->d_desc,  c->d_code,    c->d_sol,  c->d_partial, c->d_stage, c->d_rec,
+                    c->d_trace, c->d_wavelet, c->d_step, c->d_pool};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -380,6 +389,15 @@ int create(const OpesciModelDesc* d, OpesciContext** out) {
     c->stmts.assign(d->stmts, d->stmts + d->nstmts);
     if (int e = build_program(c.get(), d->terms)) return e;
     if (int e = upload(&c->d_code, d->bc_code, (size_t)d->ncode, c->stream)) return e;
+    // time-invariant media-dependent coefficients, once (the media are ready)
+    if (c->pool_elems > 0) {
+        OPESCI_HIP_TRY(hipMalloc(&c->d_pool, (size_t)c->pool_elems * c->real_bytes));
+        for (int ph = 0; ph < 5; ++ph)
+            for (const BcLevel& lv : c->levels[ph])
+                OPESCI_HIP_TRY(launch_bc_coef(c->g, c->real_bytes, c->mptr, c->d_desc, c->d_code, c->d_pool, lv.first,
+                                              lv.count, lv.maxpts, c->stream));
+        OPESCI_HIP_TRY(hipStreamSynchronize(c->stream));
+    }
     if (int e = upload(&c->d_sol, d->sol_code, (size_t)d->nsol, c->stream)) return e;
     for (int f = 0; f < NF; ++f) {
         c->sol_off[f] = d->sol_off[f];
