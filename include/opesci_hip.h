@@ -106,6 +106,20 @@ typedef struct OpesciBcStmt {
     int32_t sched;   /* launch index within the phase */
 } OpesciBcStmt;
 
+/* Derivative of a media-dependent boundary coefficient with respect to one
+ * media cell it reads (for the adjoint): term = global term index, media id
+ * and offsets as in OPESCI_OP_MEDIA, prog = bytecode of the partial
+ * derivative (without the term's literal factor).  Entries of one term are
+ * contiguous and ordered by term. */
+typedef struct OpesciBcDTerm {
+    int32_t term;
+    int32_t media;
+    int32_t off[3];
+    int32_t prog;
+    int32_t prog_len;
+    int32_t pad;
+} OpesciBcDTerm;
+
 /* Everything the generator bakes into the reference's emitted TU. */
 typedef struct OpesciModelDesc {
     int32_t abi_version;
@@ -160,6 +174,11 @@ typedef struct OpesciModelDesc {
     int32_t x_chunk;
     int32_t kernel_variant;     /* 0 default (LDS/register-queue), 1 plain gather */
     int32_t use_graph;          /* capture the time loop in a hipGraph */
+    /* adjoint: derivative programs of the media-dependent boundary
+     * coefficients (may be 0 / null when no gradient is asked for) */
+    int32_t ndterms;
+    int32_t pad3;
+    const OpesciBcDTerm* dterms;
 } OpesciModelDesc;
 
 /* Reference ABI structs.  Field pointers are real_t* (float or double). */
@@ -207,8 +226,13 @@ int opesci_ctx_l2(OpesciContext* ctx, double out[OPESCI_NFIELDS]);
 /* copy one field, one time level (0/1, or -1 = latest) to/from a dense host array real_t[dim1][dim2][dim3] */
 int opesci_ctx_download(OpesciContext* ctx, int32_t field, int32_t level, void* host);
 int opesci_ctx_upload(OpesciContext* ctx, int32_t field, int32_t level, const void* host);
-/* receiver traces recorded so far: float[nsteps_done][nrec][4] (U, V, W, pressure) */
+/* receiver traces recorded so far: float[nsteps_done][nrec][4] (U, V, W, pressure);
+   replaces the receiver arrays that tests/src/test_ref_iso_elastic.cpp:227-272
+   fills.  FP64 contexts record FP64 traces and round them here. */
 int opesci_ctx_receivers(OpesciContext* ctx, float* host, int64_t capacity_floats);
+/* the same traces as double[nsteps_done][nrec][4], unrounded for FP64 contexts
+   (no counterpart in the reference, which records float) */
+int opesci_ctx_receivers_f64(OpesciContext* ctx, double* host, int64_t capacity);
 /* steps completed since init */
 int opesci_ctx_steps(OpesciContext* ctx, int32_t* steps);
 /* average device time per launch of each hot kernel over the last run
@@ -231,6 +255,28 @@ int opesci_ctx_configure(OpesciContext* ctx, int32_t kernel_variant, int32_t til
  * side.  Every context must be initialised and at the same step.  The
  * contexts stay owned by the caller and must outlive the batch.  No
  * reference counterpart (the reference runs one model per process). */
+/* Gradient of the receiver misfit J = 1/2 sum_t sum_r sum_{c in U,V,W}
+ * (rec_t[r][c] - observed[t][r][c])^2 with respect to the context's media
+ * arrays (read path, native storage, receivers set), by the discrete adjoint
+ * of the whole time loop (interior updates and boundary program), with the
+ * forward states recomputed from checkpoints taken every `checkpoint_every`
+ * steps.  observed: ntsteps * nrec * 3 floats.  Runs the forward from the
+ * initial state; afterwards the context holds a recomputed early state
+ * (call opesci_ctx_init before stepping again).  The gradients stay on the
+ * device: opesci_ctx_download_gradient(ctx, OPESCI_M_*, host) gives one
+ * media array's, dense [dim1][dim2][dim3] in the compute type; nonzero for
+ * lambda, mu, mu12, mu13, mu23, beta1, beta2, beta3.  The initial state is
+ * taken as independent of the media (exact for a start from rest).  No
+ * reference counterpart. */
+int opesci_ctx_gradient(OpesciContext* ctx, const float* observed, int64_t count, int32_t checkpoint_every,
+                        double* misfit);
+int opesci_ctx_download_gradient(OpesciContext* ctx, int32_t media, void* host);
+/* Replace one media array (dense [dim1][dim2][dim3], compute type) and
+ * recompute the media-dependent boundary coefficients: the hook through
+ * which effective media computed elsewhere (e.g. by autograd) drive the
+ * propagator.  No reference counterpart. */
+int opesci_ctx_upload_media(OpesciContext* ctx, int32_t media, const void* host);
+
 typedef struct OpesciBatch OpesciBatch;
 int opesci_batch_create(OpesciContext** ctxs, int32_t n, OpesciBatch** out);
 int opesci_batch_run(OpesciBatch* batch, int32_t nsteps, double* seconds);

@@ -1,0 +1,459 @@
+// Discrete adjoint of the velocity-stress leapfrog step (read path, native
+// storage), for the gradient of a receiver misfit with respect to the media.
+//
+// One forward step (runtime.hip step_once) is: stress interior update, stress
+// boundary launches, velocity interior update, velocity boundary launches,
+// receivers, source.  Its adjoint runs the transposed pieces in reverse:
+// receiver residual, velocity boundary levels (last first), velocity interior,
+// stress boundary levels, stress interior.  Adjoint fields `bar` mirror the
+// forward fields (both time levels, same padded layout); a forward write of a
+// cell zeroes its adjoint after propagating it, so cells a step never writes
+// carry their adjoint through unchanged.
+//
+// Interior transposes are gathers (no atomics): the adjoint of
+//   T_ii(p) += m_ij(p) sum_k cn[i][j][k] (V_j(p + k e_j) - V_j(p - (1+k) e_j))
+// collects, at each velocity cell q, the terms of every interior p whose
+// stencil reads q; likewise for the shear and velocity updates (stencils in
+// interior.hip).  Boundary statements dst = sum c_k src_k are transposed per
+// point: a = bar[dst]; bar[dst] = 0; bar[src_k] += c_k a (atomic: points of a
+// statement may share a source cell; a level's sources are never its
+// destinations).  Their forward operands are the state before the level, which
+// an undo log restores level by level.
+//
+// Media gradients: lambda, mu (normal stresses), mu12/mu23/mu13 (shear),
+// beta1..3 (velocities) from the interior updates; media-dependent boundary
+// coefficients accumulate d misfit / d coefficient per point in `gpool`, mapped
+// at the end to the media cells through the generator's derivative programs.
+#include "bcdev.h"
+
+namespace opesci {
+
+template <typename R>
+struct AdjPtrs {
+    R* f[NF][2];
+};
+
+template <typename R>
+struct GradPtrs {
+    R* m[OPESCI_M_COUNT];   // media gradients, same layout as the media
+};
+
+__device__ __forceinline__ bool inside(const Geom& g, int x, int y, int z) {
+    const int M = g.m;
+    return x >= M && x < g.nx - M && y >= M && y < g.ny - M && z >= M && z < g.nz - M;
+}
+
+__device__ __forceinline__ void cell_of(const Geom& g, int64_t p, int& x, int& y, int& z) {
+    z = (int)(p % g.nz);
+    y = (int)((p / g.nz) % g.ny);
+    x = (int)(p / ((int64_t)g.nz * g.ny));
+}
+
+// ------------------------------------------------------------------ stress interior
+// bar V_j(t0) += transposed stencils of the stress update (reads bar T(t1)).
+template <typename R, int M>
+__global__ void adj_stress_to_v(Geom g, AdjPtrs<R> B, MediaPtrs<R> md, Coefs<R> c, int t0, int t1) {
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    const R* lam = md.m[OPESCI_M_LAMBDA];
+    const R* mu = md.m[OPESCI_M_MU];
+    const R* ms[3] = {md.m[OPESCI_M_MU12], md.m[OPESCI_M_MU23], md.m[OPESCI_M_MU13]};
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < 3 * n; e += (int64_t)gridDim.x * blockDim.x) {
+        const int j = (int)(e / n);
+        int x, y, z;
+        cell_of(g, e % n, x, y, z);
+        const int q[3] = {x, y, z};
+        R acc = 0;
+        // normal stresses i read V_j along j at +k and -(1+k)
+        for (int i = 0; i < 3; ++i)
+            for (int k = 0; k < M; ++k) {
+                for (int sgn = 0; sgn < 2; ++sgn) {
+                    int p[3] = {q[0], q[1], q[2]};
+                    p[j] += sgn == 0 ? -k : 1 + k;
+                    if (!inside(g, p[0], p[1], p[2])) continue;
+                    const int64_t a = g.at(p[0], p[1], p[2]);
+                    const R m = i == j ? lam[a] + (R)2 * mu[a] : lam[a];
+                    const R v = m * c.cn[i][j][k] * B.f[3 + i][t1][a];
+                    acc += sgn == 0 ? v : -v;
+                }
+            }
+        // shear: (shear index, axis, coefficient set) pairs that read V_j,
+        // each at +(1+k) and -k along its axis
+        int sh[2], ax[2], cc[2];
+        if (j == 0) {
+            sh[0] = 0; ax[0] = 1; cc[0] = 0;   // Txy, dU/dy
+            sh[1] = 2; ax[1] = 2; cc[1] = 0;   // Txz, dU/dz
+        } else if (j == 1) {
+            sh[0] = 0; ax[0] = 0; cc[0] = 1;   // Txy, dV/dx
+            sh[1] = 1; ax[1] = 2; cc[1] = 0;   // Tyz, dV/dz
+        } else {
+            sh[0] = 1; ax[0] = 1; cc[0] = 1;   // Tyz, dW/dy
+            sh[1] = 2; ax[1] = 0; cc[1] = 1;   // Txz, dW/dx
+        }
+        for (int u = 0; u < 2; ++u)
+            for (int k = 0; k < M; ++k)
+                for (int sgn = 0; sgn < 2; ++sgn) {
+                    int p[3] = {q[0], q[1], q[2]};
+                    p[ax[u]] += sgn == 0 ? -(1 + k) : k;
+                    if (!inside(g, p[0], p[1], p[2])) continue;
+                    const int64_t a = g.at(p[0], p[1], p[2]);
+                    const R v = ms[sh[u]][a] * c.cs[sh[u]][cc[u]][k] * B.f[6 + sh[u]][t1][a];
+                    acc += sgn == 0 ? v : -v;
+                }
+        B.f[j][t0][g.at(x, y, z)] += acc;
+    }
+}
+
+// bar T(t0) += bar T(t1), bar T(t1) = 0 on the interior; media gradients of
+// the stress update from the forward velocities at t0.
+template <typename R, int M>
+__global__ void adj_stress_point(Geom g, FieldPtrs<R> F, AdjPtrs<R> B, GradPtrs<R> gm, Coefs<R> c, int t0, int t1) {
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    const int64_t st[3] = {g.sx, g.sy, 1};
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        int x, y, z;
+        cell_of(g, e, x, y, z);
+        if (!inside(g, x, y, z)) continue;
+        const int64_t a = g.at(x, y, z);
+        R bt[6];
+        for (int f = 0; f < 6; ++f) {
+            bt[f] = B.f[3 + f][t1][a];
+            B.f[3 + f][t0][a] += bt[f];
+            B.f[3 + f][t1][a] = 0;
+        }
+        // normal: T_ii += (i == j ? lambda + 2 mu : lambda) Sn[i][j]
+        R glam = 0, gmu = 0;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                R s = 0;
+                for (int k = 0; k < M; ++k)
+                    s += c.cn[i][j][k] * (F.f[j][t0][a + k * st[j]] - F.f[j][t0][a - (1 + k) * st[j]]);
+                glam += bt[i] * s;
+                if (i == j) gmu += (R)2 * bt[i] * s;
+            }
+        gm.m[OPESCI_M_LAMBDA][a] += glam;
+        gm.m[OPESCI_M_MU][a] += gmu;
+        // shear s: mu_s (sum_k cs[s][0][k] (A(+(1+k)) - A(-k)) along ax0 + cs[s][1] B along ax1)
+        const int fa[3] = {0, 1, 0}, fb[3] = {1, 2, 2};   // differentiated fields
+        const int aa[3] = {1, 2, 2}, ab[3] = {0, 1, 0};   // their axes
+        const int mid[3] = {OPESCI_M_MU12, OPESCI_M_MU23, OPESCI_M_MU13};
+        for (int s = 0; s < 3; ++s) {
+            R v = 0;
+            for (int k = 0; k < M; ++k) {
+                v += c.cs[s][0][k] * (F.f[fa[s]][t0][a + (1 + k) * st[aa[s]]] - F.f[fa[s]][t0][a - k * st[aa[s]]]);
+                v += c.cs[s][1][k] * (F.f[fb[s]][t0][a + (1 + k) * st[ab[s]]] - F.f[fb[s]][t0][a - k * st[ab[s]]]);
+            }
+            gm.m[mid[s]][a] += bt[3 + s] * v;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ velocity interior
+// bar T(t1) += transposed stencils of the velocity update (reads bar V(t1)).
+template <typename R, int M>
+__global__ void adj_vel_to_t(Geom g, AdjPtrs<R> B, MediaPtrs<R> md, Coefs<R> c, int t1) {
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    const R* beta[3] = {md.m[OPESCI_M_BETA1], md.m[OPESCI_M_BETA2], md.m[OPESCI_M_BETA3]};
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < 6 * n; e += (int64_t)gridDim.x * blockDim.x) {
+        const int f = (int)(e / n);   // stress 0..5 = Txx, Tyy, Tzz, Txy, Tyz, Txz
+        int x, y, z;
+        cell_of(g, e % n, x, y, z);
+        const int q[3] = {x, y, z};
+        // (velocity i, axis j) pairs reading this stress; a = 1 for T_ii along i
+        int vi[2], vj[2], cnt;
+        if (f < 3) {
+            vi[0] = f; vj[0] = f; cnt = 1;
+        } else if (f == 3) {
+            vi[0] = 0; vj[0] = 1; vi[1] = 1; vj[1] = 0; cnt = 2;
+        } else if (f == 4) {
+            vi[0] = 1; vj[0] = 2; vi[1] = 2; vj[1] = 1; cnt = 2;
+        } else {
+            vi[0] = 0; vj[0] = 2; vi[1] = 2; vj[1] = 0; cnt = 2;
+        }
+        R acc = 0;
+        for (int u = 0; u < cnt; ++u) {
+            const int i = vi[u], j = vj[u], off = (f < 3) ? 1 : 0;
+            for (int k = 0; k < M; ++k)
+                for (int sgn = 0; sgn < 2; ++sgn) {
+                    // V_i(p) reads T(p + (off + k) e_j) (+) and T(p + (off - 1 - k) e_j) (-)
+                    int p[3] = {q[0], q[1], q[2]};
+                    p[j] -= sgn == 0 ? off + k : off - 1 - k;
+                    if (!inside(g, p[0], p[1], p[2])) continue;
+                    const int64_t a = g.at(p[0], p[1], p[2]);
+                    const R v = beta[i][a] * c.cv[i][j][k] * B.f[i][t1][a];
+                    acc += sgn == 0 ? v : -v;
+                }
+        }
+        B.f[3 + f][t1][g.at(x, y, z)] += acc;
+    }
+}
+
+// bar V(t0) += bar V(t1), bar V(t1) = 0 on the interior; beta gradients from
+// the forward stresses at t1.
+template <typename R, int M>
+__global__ void adj_vel_point(Geom g, FieldPtrs<R> F, AdjPtrs<R> B, GradPtrs<R> gm, Coefs<R> c, int t0, int t1) {
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    const int64_t st[3] = {g.sx, g.sy, 1};
+    const int mid[3] = {OPESCI_M_BETA1, OPESCI_M_BETA2, OPESCI_M_BETA3};
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        int x, y, z;
+        cell_of(g, e, x, y, z);
+        if (!inside(g, x, y, z)) continue;
+        const int64_t a = g.at(x, y, z);
+        for (int i = 0; i < 3; ++i) {
+            const R b = B.f[i][t1][a];
+            B.f[i][t0][a] += b;
+            B.f[i][t1][a] = 0;
+            R s = 0;
+            for (int j = 0; j < 3; ++j) {
+                const int fld = j == i ? 3 + i : 6 + ((i + j == 1) ? 0 : (i + j == 3 ? 1 : 2));
+                const int off = j == i ? 1 : 0;
+                for (int k = 0; k < M; ++k)
+                    s += c.cv[i][j][k] *
+                         (F.f[fld][t1][a + (off + k) * st[j]] - F.f[fld][t1][a + (off - 1 - k) * st[j]]);
+            }
+            gm.m[mid[i]][a] += b * s;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ boundary levels
+// Forward statement with the destination's previous value logged
+// (log[(descriptor - first) * maxpts + point]).
+template <typename R>
+__global__ __launch_bounds__(256) void bc_log_kernel(Geom g, FieldPtrs<R> F, const BcDesc* __restrict__ desc,
+                                                     const R* __restrict__ pool, int first, int maxpts, R* log,
+                                                     int t0, int t1) {
+    const BcDesc* D = desc + first + blockIdx.y;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= D->npts) return;
+    const int n1 = D->n1, i0 = D->lo0 + p / n1, i1 = D->lo1 + p % n1;
+    const int o0 = D->axis == 0 ? 1 : 0, o1 = D->axis == 2 ? 1 : 2;
+    const int64_t so0 = o0 == 0 ? g.sx : g.sy, so1 = o1 == 1 ? g.sy : 1;
+    const int64_t dst = (int64_t)i0 * so0 + (int64_t)i1 * so1 + D->disp;
+    const int fl = D->fl;
+    log[(int64_t)blockIdx.y * maxpts + p] = F.f[fl >> 1][(fl & 1) ? t1 : t0][dst];
+    bc_point<R, R>(g, F, D, pool, p, i0, i1, t0, t1);
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void bc_undo_kernel(Geom g, FieldPtrs<R> F, const BcDesc* __restrict__ desc,
+                                                      int first, int maxpts, const R* __restrict__ log, int t0,
+                                                      int t1) {
+    const BcDesc* D = desc + first + blockIdx.y;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= D->npts) return;
+    const int n1 = D->n1, i0 = D->lo0 + p / n1, i1 = D->lo1 + p % n1;
+    const int o0 = D->axis == 0 ? 1 : 0, o1 = D->axis == 2 ? 1 : 2;
+    const int64_t so0 = o0 == 0 ? g.sx : g.sy, so1 = o1 == 1 ? g.sy : 1;
+    const int64_t dst = (int64_t)i0 * so0 + (int64_t)i1 * so1 + D->disp;
+    const int fl = D->fl;
+    F.f[fl >> 1][(fl & 1) ? t1 : t0][dst] = log[(int64_t)blockIdx.y * maxpts + p];
+}
+
+// Transposed statements of one level; coefficient gradients into gpool.
+template <typename R>
+__global__ __launch_bounds__(256) void adj_bc_kernel(Geom g, FieldPtrs<R> F, AdjPtrs<R> B,
+                                                     const BcDesc* __restrict__ desc, const R* __restrict__ pool,
+                                                     R* gpool, int first, int t0, int t1) {
+    const BcDesc* D = desc + first + blockIdx.y;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= D->npts) return;
+    const int n1 = D->n1, i0 = D->lo0 + p / n1, i1 = D->lo1 + p % n1;
+    const int o0 = D->axis == 0 ? 1 : 0, o1 = D->axis == 2 ? 1 : 2;
+    const int64_t so0 = o0 == 0 ? g.sx : g.sy, so1 = o1 == 1 ? g.sy : 1;
+    const int64_t dst = (int64_t)i0 * so0 + (int64_t)i1 * so1 + D->disp;
+    const int fl = D->fl;
+    R* bd = B.f[fl >> 1][(fl & 1) ? t1 : t0] + dst;
+    const R a = *bd;
+    *bd = 0;
+    if (a == (R)0) return;
+    for (int k = 0; k < D->nterms; ++k) {
+        const int sfl = D->t[k].fl;
+        const int lv = (sfl & 1) ? t1 : t0;
+        const int64_t src = dst + D->t[k].disp;
+        R coef = (R)D->t[k].coef;
+        if (D->t[k].cslot >= 0) {
+            const int64_t ci = D->cpool + (int64_t)D->t[k].cslot * D->npts + p;
+            coef = pool[ci];
+            gpool[ci] += a * F.f[sfl >> 1][lv][src];
+        }
+        atomicAdd(B.f[sfl >> 1][lv] + src, coef * a);
+    }
+}
+
+// gpool -> media gradients through the derivative programs of each term.
+template <typename R>
+__global__ __launch_bounds__(256) void adj_coef_media_kernel(Geom g, MediaPtrs<R> md, GradPtrs<R> gm,
+                                                             const BcDesc* __restrict__ desc,
+                                                             const OpesciExprOp* __restrict__ code,
+                                                             const OpesciBcDTerm* __restrict__ dterms,
+                                                             const R* __restrict__ gpool, int first) {
+    const BcDesc* D = desc + first + blockIdx.y;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= D->npts) return;
+    const int n1 = D->n1, axis = D->axis;
+    const int i0 = D->lo0 + p / n1, i1 = D->lo1 + p % n1;
+    const int ax = axis == 0 ? 0 : i0;
+    const int ay = axis == 0 ? i0 : (axis == 1 ? 0 : i1);
+    const int az = axis == 2 ? 0 : i1;
+    const double c0[3] = {0, 0, 0};
+    for (int k = 0; k < D->nterms; ++k) {
+        if (D->t[k].cslot < 0) continue;
+        const R gc = gpool[D->cpool + (int64_t)D->t[k].cslot * D->npts + p];
+        if (gc == (R)0) continue;
+        for (int d = D->t[k].dterm0; d < D->t[k].dterm0 + D->t[k].ndterm; ++d) {
+            const OpesciBcDTerm& T = dterms[d];
+            const double dv = (double)D->t[k].coef * vm_eval<R>(code + T.prog, T.prog_len, c0, 0.0, md, g, ax, ay, az);
+            atomicAdd(gm.m[T.media] + g.at(ax + T.off[0], ay + T.off[1], az + T.off[2]), (R)(gc * (R)dv));
+        }
+    }
+}
+
+// Receiver misfit residual: bar U,V,W(t1) += field - observed at each receiver.
+template <typename R>
+__global__ void adj_receiver_kernel(Geom g, FieldPtrs<R> F, AdjPtrs<R> B, int t1, int nrec,
+                                    const int32_t* __restrict__ rec, const float* __restrict__ obs) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrec) return;
+    const int64_t q = g.at(rec[3 * r], rec[3 * r + 1], rec[3 * r + 2]);
+    for (int c = 0; c < 3; ++c) atomicAdd(B.f[c][t1] + q, F.f[c][t1][q] - (R)obs[3 * r + c]);
+}
+
+__global__ void set_step_kernel(int* step, int value) { *step = value; }
+
+// ------------------------------------------------------------------ launchers
+static dim3 grid_for(int64_t n) {
+    const int64_t b = (n + 255) / 256;
+    return dim3((unsigned)std::min<int64_t>(b, 65536));
+}
+
+template <typename R, int M>
+static hipError_t adj_stress_t(const Geom& g, void* fields, void* bar, const void* media, void* grad,
+                               const void* coefs, int t0, int t1, hipStream_t s) {
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    const FieldPtrs<R>& F = *static_cast<const FieldPtrs<R>*>(fields);
+    const AdjPtrs<R>& B = *static_cast<const AdjPtrs<R>*>(bar);
+    const MediaPtrs<R>& md = *static_cast<const MediaPtrs<R>*>(media);
+    const GradPtrs<R>& gm = *static_cast<const GradPtrs<R>*>(grad);
+    const Coefs<R>& c = *static_cast<const Coefs<R>*>(coefs);
+    hipLaunchKernelGGL((adj_stress_to_v<R, M>), grid_for(3 * n), dim3(256), 0, s, g, B, md, c, t0, t1);
+    hipLaunchKernelGGL((adj_stress_point<R, M>), grid_for(n), dim3(256), 0, s, g, F, B, gm, c, t0, t1);
+    return hipGetLastError();
+}
+
+template <typename R, int M>
+static hipError_t adj_vel_t(const Geom& g, void* fields, void* bar, const void* media, void* grad, const void* coefs,
+                            int t0, int t1, hipStream_t s) {
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    const FieldPtrs<R>& F = *static_cast<const FieldPtrs<R>*>(fields);
+    const AdjPtrs<R>& B = *static_cast<const AdjPtrs<R>*>(bar);
+    const MediaPtrs<R>& md = *static_cast<const MediaPtrs<R>*>(media);
+    const GradPtrs<R>& gm = *static_cast<const GradPtrs<R>*>(grad);
+    const Coefs<R>& c = *static_cast<const Coefs<R>*>(coefs);
+    hipLaunchKernelGGL((adj_vel_to_t<R, M>), grid_for(6 * n), dim3(256), 0, s, g, B, md, c, t1);
+    hipLaunchKernelGGL((adj_vel_point<R, M>), grid_for(n), dim3(256), 0, s, g, F, B, gm, c, t0, t1);
+    return hipGetLastError();
+}
+
+#define OPESCI_ADJ_M(FN, R)                                                                  \
+    switch (g.m) {                                                                           \
+        case 1: return FN<R, 1>(g, fields, bar, media, grad, coefs, t0, t1, s);               \
+        case 2: return FN<R, 2>(g, fields, bar, media, grad, coefs, t0, t1, s);               \
+        case 3: return FN<R, 3>(g, fields, bar, media, grad, coefs, t0, t1, s);               \
+        case 4: return FN<R, 4>(g, fields, bar, media, grad, coefs, t0, t1, s);               \
+        default: return hipErrorInvalidValue;                                                \
+    }
+
+hipError_t launch_adj_stress(const Geom& g, int real_bytes, void* fields, void* bar, const void* media, void* grad,
+                             const void* coefs, int t0, int t1, hipStream_t s) {
+    if (real_bytes == 8) {
+        OPESCI_ADJ_M(adj_stress_t, double)
+    }
+    OPESCI_ADJ_M(adj_stress_t, float)
+}
+
+hipError_t launch_adj_velocity(const Geom& g, int real_bytes, void* fields, void* bar, const void* media, void* grad,
+                               const void* coefs, int t0, int t1, hipStream_t s) {
+    if (real_bytes == 8) {
+        OPESCI_ADJ_M(adj_vel_t, double)
+    }
+    OPESCI_ADJ_M(adj_vel_t, float)
+}
+
+template <typename R>
+static hipError_t bc_log_t(const Geom& g, void* fields, const BcDesc* desc, const void* pool, const BcLevel& lv,
+                           void* log, int t0, int t1, int undo, hipStream_t s) {
+    if (lv.count <= 0 || lv.maxpts <= 0) return hipSuccess;
+    dim3 grid((lv.maxpts + 255) / 256, lv.count);
+    const FieldPtrs<R>& F = *static_cast<const FieldPtrs<R>*>(fields);
+    if (undo)
+        hipLaunchKernelGGL((bc_undo_kernel<R>), grid, dim3(256), 0, s, g, F, desc, lv.first, lv.maxpts,
+                           static_cast<const R*>(log), t0, t1);
+    else
+        hipLaunchKernelGGL((bc_log_kernel<R>), grid, dim3(256), 0, s, g, F, desc, static_cast<const R*>(pool),
+                           lv.first, lv.maxpts, static_cast<R*>(log), t0, t1);
+    return hipGetLastError();
+}
+
+hipError_t launch_bc_logged(const Geom& g, int real_bytes, void* fields, const BcDesc* desc, const void* pool,
+                            const BcLevel& lv, void* log, int t0, int t1, int undo, hipStream_t s) {
+    return real_bytes == 8 ? bc_log_t<double>(g, fields, desc, pool, lv, log, t0, t1, undo, s)
+                           : bc_log_t<float>(g, fields, desc, pool, lv, log, t0, t1, undo, s);
+}
+
+template <typename R>
+static hipError_t adj_bc_t(const Geom& g, void* fields, void* bar, const BcDesc* desc, const void* pool, void* gpool,
+                           const BcLevel& lv, int t0, int t1, hipStream_t s) {
+    if (lv.count <= 0 || lv.maxpts <= 0) return hipSuccess;
+    dim3 grid((lv.maxpts + 255) / 256, lv.count);
+    hipLaunchKernelGGL((adj_bc_kernel<R>), grid, dim3(256), 0, s, g, *static_cast<const FieldPtrs<R>*>(fields),
+                       *static_cast<const AdjPtrs<R>*>(bar), desc, static_cast<const R*>(pool),
+                       static_cast<R*>(gpool), lv.first, t0, t1);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_bc(const Geom& g, int real_bytes, void* fields, void* bar, const BcDesc* desc, const void* pool,
+                         void* gpool, const BcLevel& lv, int t0, int t1, hipStream_t s) {
+    return real_bytes == 8 ? adj_bc_t<double>(g, fields, bar, desc, pool, gpool, lv, t0, t1, s)
+                           : adj_bc_t<float>(g, fields, bar, desc, pool, gpool, lv, t0, t1, s);
+}
+
+template <typename R>
+static hipError_t adj_coef_t(const Geom& g, const void* media, void* grad, const BcDesc* desc,
+                             const OpesciExprOp* code, const OpesciBcDTerm* dterms, const void* gpool,
+                             const BcLevel& lv, hipStream_t s) {
+    if (lv.count <= 0 || lv.maxpts <= 0) return hipSuccess;
+    dim3 grid((lv.maxpts + 255) / 256, lv.count);
+    hipLaunchKernelGGL((adj_coef_media_kernel<R>), grid, dim3(256), 0, s, g, *static_cast<const MediaPtrs<R>*>(media),
+                       *static_cast<const GradPtrs<R>*>(grad), desc, code, dterms, static_cast<const R*>(gpool),
+                       lv.first);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_coef_media(const Geom& g, int real_bytes, const void* media, void* grad, const BcDesc* desc,
+                                 const OpesciExprOp* code, const OpesciBcDTerm* dterms, const void* gpool,
+                                 const BcLevel& lv, hipStream_t s) {
+    return real_bytes == 8 ? adj_coef_t<double>(g, media, grad, desc, code, dterms, gpool, lv, s)
+                           : adj_coef_t<float>(g, media, grad, desc, code, dterms, gpool, lv, s);
+}
+
+hipError_t launch_adj_receivers(const Geom& g, int real_bytes, void* fields, void* bar, int t1, int nrec,
+                                const int32_t* rec, const float* obs, hipStream_t s) {
+    if (nrec <= 0) return hipSuccess;
+    if (real_bytes == 8)
+        hipLaunchKernelGGL((adj_receiver_kernel<double>), dim3((nrec + 255) / 256), dim3(256), 0, s, g,
+                           *static_cast<const FieldPtrs<double>*>(fields), *static_cast<const AdjPtrs<double>*>(bar),
+                           t1, nrec, rec, obs);
+    else
+        hipLaunchKernelGGL((adj_receiver_kernel<float>), dim3((nrec + 255) / 256), dim3(256), 0, s, g,
+                           *static_cast<const FieldPtrs<float>*>(fields), *static_cast<const AdjPtrs<float>*>(bar),
+                           t1, nrec, rec, obs);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_step(int* step, int value, hipStream_t s) {
+    hipLaunchKernelGGL(set_step_kernel, dim3(1), dim3(1), 0, s, step, value);
+    return hipGetLastError();
+}
+
+}  // namespace opesci

@@ -184,22 +184,33 @@ __global__ void zero_kernel(uint4* __restrict__ p, size_t n16) {
 // replayed.
 // Receivers (tests/src/test_ref_iso_elastic.cpp:227-272): U, V, W and
 // (Txx+Tyy+Tzz)/3 at each receiver's grid point after the velocity update,
-// one thread per receiver, trace[step][receiver][4].
+// one thread per receiver, trace[step][receiver][4] in the arithmetic type R
+// (FP64 contexts keep FP64 traces, so misfits of FP64 runs are not rounded).
+template <typename R> struct Trace4;
+template <> struct Trace4<float> {
+    static __device__ void put(void* t, size_t i, float a, float b, float c, float d) {
+        reinterpret_cast<float4*>(t)[i] = make_float4(a, b, c, d);
+    }
+};
+template <> struct Trace4<double> {
+    static __device__ void put(void* t, size_t i, double a, double b, double c, double d) {
+        double2* p = reinterpret_cast<double2*>(t) + 2 * i;
+        p[0] = make_double2(a, b);
+        p[1] = make_double2(c, d);
+    }
+};
+
 template <typename R, typename S>
 __global__ __launch_bounds__(256) void receiver_kernel(Geom g, FieldPtrs<S> F, int t1, int nrec,
-                                                       const int32_t* __restrict__ rec, float* __restrict__ trace,
+                                                       const int32_t* __restrict__ rec, void* __restrict__ trace,
                                                        const int* __restrict__ step, int ntsteps) {
     using C = Conv<S, R>;
     const int s = *step;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ntsteps || r >= nrec) return;
     const int64_t q = g.at(rec[3 * r], rec[3 * r + 1], rec[3 * r + 2]);
-    float4 o;
-    o.x = (float)C::ld(F.f[0][t1] + q);
-    o.y = (float)C::ld(F.f[1][t1] + q);
-    o.z = (float)C::ld(F.f[2][t1] + q);
-    o.w = (float)((C::ld(F.f[3][t1] + q) + C::ld(F.f[4][t1] + q) + C::ld(F.f[5][t1] + q)) / (R)3);
-    reinterpret_cast<float4*>(trace)[(size_t)s * nrec + r] = o;
+    Trace4<R>::put(trace, (size_t)s * nrec + r, C::ld(F.f[0][t1] + q), C::ld(F.f[1][t1] + q), C::ld(F.f[2][t1] + q),
+                   (C::ld(F.f[3][t1] + q) + C::ld(F.f[4][t1] + q) + C::ld(F.f[5][t1] + q)) / (R)3);
 }
 
 // Point source (test_ref_iso_elastic.cpp:274-290), after the receivers:
@@ -373,7 +384,7 @@ hipError_t launch_zero(void* p, size_t bytes, hipStream_t s) {
 }
 
 template <typename R, typename S>
-static hipError_t rec_t(const Geom& g, void* fields, int t1, int nrec, const int32_t* rec, float* trace, int ntr,
+static hipError_t rec_t(const Geom& g, void* fields, int t1, int nrec, const int32_t* rec, void* trace, int ntr,
                         const int32_t* si, const float* w, int* step, int ntsteps, hipStream_t s) {
     const FieldPtrs<S>& F = *static_cast<const FieldPtrs<S>*>(fields);
     if (nrec > 0 && trace)
@@ -385,7 +396,7 @@ static hipError_t rec_t(const Geom& g, void* fields, int t1, int nrec, const int
 }
 
 hipError_t launch_record_source(const Geom& g, int real_bytes, int storage, void* fields, int t1, int nrec,
-                                const int32_t* d_rec, float* d_trace, int src_on, const int32_t* src_idx,
+                                const int32_t* d_rec, void* d_trace, int src_on, const int32_t* src_idx,
                                 const float* d_wavelet, int* d_step, int ntsteps, hipStream_t s) {
     return OPESCI_TYPED(
         real_bytes, storage,

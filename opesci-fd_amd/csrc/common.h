@@ -144,6 +144,8 @@ struct BcTermDev {
     float coef;
     int32_t prog;    // -1 or offset of a media program (length in prog_len)
     int32_t cslot;   // -1 or the term's slot in the statement's block of the coefficient pool
+    int32_t dterm0;  // adjoint: the term's derivative programs [dterm0, dterm0 + ndterm)
+    int32_t ndterm;
 };
 struct BcDesc {
     int32_t fl;      // destination field*2 + level
@@ -165,6 +167,22 @@ hipError_t launch_bc(const Geom& g, int real_bytes, int storage, void* fields, c
                      const BcLevel& lvl, int t0, int t1, hipStream_t s);
 hipError_t launch_bc_coef(const Geom& g, int real_bytes, const void* media, const BcDesc* d_desc,
                           const OpesciExprOp* d_code, void* pool, int first, int count, int maxpts, hipStream_t s);
+// adjoint (adjoint.hip); native storage, compute type given by real_bytes
+hipError_t launch_adj_stress(const Geom& g, int real_bytes, void* fields, void* bar, const void* media, void* grad,
+                             const void* coefs, int t0, int t1, hipStream_t s);
+hipError_t launch_adj_velocity(const Geom& g, int real_bytes, void* fields, void* bar, const void* media, void* grad,
+                               const void* coefs, int t0, int t1, hipStream_t s);
+hipError_t launch_bc_logged(const Geom& g, int real_bytes, void* fields, const BcDesc* desc, const void* pool,
+                            const BcLevel& lv, void* log, int t0, int t1, int undo, hipStream_t s);
+hipError_t launch_adj_bc(const Geom& g, int real_bytes, void* fields, void* bar, const BcDesc* desc, const void* pool,
+                         void* gpool, const BcLevel& lv, int t0, int t1, hipStream_t s);
+hipError_t launch_adj_coef_media(const Geom& g, int real_bytes, const void* media, void* grad, const BcDesc* desc,
+                                 const OpesciExprOp* code, const OpesciBcDTerm* dterms, const void* gpool,
+                                 const BcLevel& lv, hipStream_t s);
+hipError_t launch_adj_receivers(const Geom& g, int real_bytes, void* fields, void* bar, int t1, int nrec,
+                                const int32_t* rec, const float* obs, hipStream_t s);
+hipError_t launch_set_step(int* step, int value, hipStream_t s);
+
 hipError_t launch_initial(const Geom& g, int real_bytes, int storage, void* fields, const void* media,
                           const OpesciExprOp* d_sol, const int* sol_off, const int* sol_len, const double* spacing,
                           double dt, hipStream_t s);
@@ -179,7 +197,7 @@ hipError_t launch_unpack(const Geom& g, int real_bytes, int storage, const void*
 hipError_t launch_pack_media(const Geom& g, int real_bytes, const void* dense, void* media, hipStream_t s);
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
 hipError_t launch_record_source(const Geom& g, int real_bytes, int storage, void* fields, int t1, int nrec,
-                                const int32_t* d_rec, float* d_trace, int src_on, const int32_t* src_idx,
+                                const int32_t* d_rec, void* d_trace, int src_on, const int32_t* src_idx,
                                 const float* d_wavelet, int* d_step, int ntsteps, hipStream_t s);
 
 }  // namespace opesci

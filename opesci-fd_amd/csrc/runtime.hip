@@ -49,6 +49,19 @@ struct OpesciContext {
     BcDesc* d_desc = nullptr;
     void* d_pool = nullptr;           // precomputed media-dependent boundary coefficients (compute type)
     int64_t pool_elems = 0;
+    OpesciBcDTerm* d_dterms = nullptr;  // derivative programs of those coefficients (adjoint)
+    // adjoint state (allocated by the first opesci_ctx_gradient)
+    void* abase[NF][2] = {};
+    void* aptr[NF][2] = {};           // AdjPtrs<R> image
+    void* gbase[OPESCI_M_COUNT] = {};
+    void* gptr[OPESCI_M_COUNT] = {};  // MediaPtrs<R> image of the media gradients
+    void* d_gpool = nullptr;
+    void* d_log = nullptr;            // undo log of one step's boundary levels
+    std::vector<int64_t> log_off[5];  // per phase and level
+    char* d_ckpt = nullptr;           // checkpoint states, 18 field levels each
+    char* d_seg = nullptr;            // states of one checkpoint segment
+    int ckpt_cap = 0, seg_cap = 0;
+    float* d_obs = nullptr;
     OpesciExprOp* d_code = nullptr;
     OpesciExprOp* d_sol = nullptr;
     int sol_off[NF], sol_len[NF];
@@ -56,7 +69,7 @@ struct OpesciContext {
     int partial_cap = 0;
     void* d_stage = nullptr;          // dense staging buffer (one field level, real_t)
     int32_t* d_rec = nullptr;
-    float* d_trace = nullptr;
+    void* d_trace = nullptr;          // [ntsteps][nrec][4] of the arithmetic type
     float* d_wavelet = nullptr;
     int* d_step = nullptr;
     int nrec = 0;
@@ -168,6 +181,36 @@ int validate(const OpesciModelDesc* d) {
             }
         }
     }
+    // adjoint derivative programs: ordered by term, valid media references
+    // (their cells are among those the term's own program reads)
+    if (d->ndterms < 0 || (d->ndterms && !d->dterms)) return fail(OPESCI_EINVAL, "bad derivative table");
+    for (int i = 0; i < d->ndterms; ++i) {
+        const OpesciBcDTerm& t = d->dterms[i];
+        if (t.term < 0 || t.term >= d->nterms || (i > 0 && t.term < d->dterms[i - 1].term) || t.media < 0 ||
+            t.media >= OPESCI_M_COUNT || !d->media_arrays || d->terms[t.term].prog < 0)
+            return fail(OPESCI_EINVAL, "malformed derivative entry");
+        if (t.prog < 0 || t.prog_len < 0 || t.prog + t.prog_len > d->ncode || !program_ok(d->bc_code + t.prog, t.prog_len))
+            return fail(OPESCI_EINVAL, "derivative program out of range or malformed");
+        bool found = false;
+        const OpesciBcTerm& tm = d->terms[t.term];
+        for (int c = tm.prog; c < tm.prog + tm.prog_len && !found; ++c) {
+            const OpesciExprOp& op = d->bc_code[c];
+            found = op.op == OPESCI_OP_MEDIA && op.a[0] == t.media && op.a[1] == t.off[0] && op.a[2] == t.off[1] &&
+                    op.a[3] == t.off[2];
+        }
+        if (!found) return fail(OPESCI_EINVAL, "derivative of a media cell the coefficient does not read");
+        for (int c = t.prog; c < t.prog + t.prog_len; ++c) {
+            const OpesciExprOp& op = d->bc_code[c];
+            if (op.op != OPESCI_OP_MEDIA) continue;
+            bool known = false;
+            for (int c2 = tm.prog; c2 < tm.prog + tm.prog_len && !known; ++c2) {
+                const OpesciExprOp& o2 = d->bc_code[c2];
+                known = o2.op == OPESCI_OP_MEDIA && o2.a[0] == op.a[0] && o2.a[1] == op.a[1] && o2.a[2] == op.a[2] &&
+                        o2.a[3] == op.a[3];
+            }
+            if (!known) return fail(OPESCI_EINVAL, "derivative program reads a cell the coefficient does not");
+        }
+    }
     for (int f = 0; f < NF; ++f) {
         if (d->sol_off[f] < 0 || d->sol_len[f] < 0 || d->sol_off[f] + d->sol_len[f] > d->nsol)
             return fail(OPESCI_EINVAL, "analytic solution table out of range");
@@ -200,7 +243,14 @@ int validate(const OpesciModelDesc* d) {
 // Sort statements by (phase, launch), build one self-contained descriptor
 // per statement with the linear displacements the device uses (all indices
 // were validated against the grid; the grid is checked to fit int32 offsets).
-int build_program(OpesciContext* c, const OpesciBcTerm* terms) {
+int build_program(OpesciContext* c, const OpesciBcTerm* terms, const OpesciBcDTerm* dterms, int ndterms,
+                  int nterms) {
+    // derivative programs per term (contiguous, ordered by term)
+    std::vector<int> dfirst(std::max(nterms, 1), 0), dcount(std::max(nterms, 1), 0);
+    for (int i = 0; i < ndterms; ++i) {
+        if (dcount[dterms[i].term] == 0) dfirst[dterms[i].term] = i;
+        ++dcount[dterms[i].term];
+    }
     std::stable_sort(c->stmts.begin(), c->stmts.end(), [](const OpesciBcStmt& a, const OpesciBcStmt& b) {
         return a.phase != b.phase ? a.phase < b.phase : a.sched < b.sched;
     });
@@ -242,6 +292,8 @@ int build_program(OpesciContext* c, const OpesciBcTerm* terms) {
                 d.t[k].prog = tm.prog;
                 d.prog_len[k] = tm.prog_len;
                 d.t[k].cslot = tm.prog >= 0 ? nslot++ : -1;
+                d.t[k].dterm0 = dfirst[a.term0 + k];
+                d.t[k].ndterm = dcount[a.term0 + k];
             }
             if (nslot > 0) {
                 if (c->pool_elems + (int64_t)nslot * d.npts >= ((int64_t)1 << 31))
@@ -320,7 +372,13 @@ void destroy(OpesciContext* c) {
     for (int i = 0; i < OPESCI_M_COUNT; ++i)
         if (c->mbase[i]) (void)hipFree(c->mbase[i]);
     void* ptrs[] = {c->d_desc,  c->d_code,    c->d_sol,  c->d_partial, c->d_stage, c->d_rec,
-                    c->d_trace, c->d_wavelet, c->d_step, c->d_pool};
+                    c->d_trace, c->d_wavelet, c->d_step, c->d_pool,    c->d_dterms, c->d_gpool,
+                    c->d_log,   c->d_ckpt,    c->d_seg,  c->d_obs};
+    for (int f = 0; f < NF; ++f)
+        for (int l = 0; l < 2; ++l)
+            if (c->abase[f][l]) (void)hipFree(c->abase[f][l]);
+    for (int i = 0; i < OPESCI_M_COUNT; ++i)
+        if (c->gbase[i]) (void)hipFree(c->gbase[i]);
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -387,7 +445,8 @@ int create(const OpesciModelDesc* d, OpesciContext** out) {
                 c->cf.cs[s][gq][k] = (float)d->cs[s][gq][k];
             }
     c->stmts.assign(d->stmts, d->stmts + d->nstmts);
-    if (int e = build_program(c.get(), d->terms)) return e;
+    if (int e = build_program(c.get(), d->terms, d->dterms, d->ndterms, d->nterms)) return e;
+    if (int e = upload(&c->d_dterms, d->dterms, (size_t)d->ndterms, c->stream)) return e;
     if (int e = upload(&c->d_code, d->bc_code, (size_t)d->ncode, c->stream)) return e;
     // time-invariant media-dependent coefficients, once (the media are ready)
     if (c->pool_elems > 0) {
@@ -411,7 +470,7 @@ int create(const OpesciModelDesc* d, OpesciContext** out) {
         if (int e = alloc_zero(c.get(), (void**)&c->d_step, 256)) return e;
         if (c->nrec) {
             if (int e = upload(&c->d_rec, d->rec_idx, (size_t)3 * c->nrec, c->stream)) return e;
-            const size_t tb = (size_t)std::max(d->ntsteps, 1) * c->nrec * 4 * sizeof(float);
+            const size_t tb = (size_t)std::max(d->ntsteps, 1) * c->nrec * 4 * d->real_bytes;
             if (int e = alloc_zero(c.get(), (void**)&c->d_trace, tb)) return e;
         }
         if (c->src_ntr) {
@@ -596,14 +655,37 @@ int opesci_ctx_steps(OpesciContext* ctx, int32_t* steps) {
     return 0;
 }
 
-int opesci_ctx_receivers(OpesciContext* ctx, float* host, int64_t capacity) {
+}  // extern "C"
+
+// traces recorded so far, copied out and converted to T
+template <typename T>
+static int receivers_as(OpesciContext* ctx, T* host, int64_t capacity) {
     if (!ctx) return fail(OPESCI_EINVAL, "null context");
     if (ctx->nrec == 0) return 0;
     const int64_t n = (int64_t)std::min(ctx->steps, ctx->desc.ntsteps) * ctx->nrec * 4;
     if (!host || capacity < n) return fail(OPESCI_EINVAL, "receiver buffer too small");
-    OPESCI_HIP_TRY(hipMemcpyAsync(host, ctx->d_trace, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    if ((size_t)ctx->real_bytes == sizeof(T)) {
+        OPESCI_HIP_TRY(hipMemcpyAsync(host, ctx->d_trace, n * sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+        OPESCI_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        return 0;
+    }
+    std::vector<char> tmp((size_t)n * ctx->real_bytes);
+    OPESCI_HIP_TRY(hipMemcpyAsync(tmp.data(), ctx->d_trace, tmp.size(), hipMemcpyDeviceToHost, ctx->stream));
     OPESCI_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (int64_t i = 0; i < n; ++i)
+        host[i] = ctx->real_bytes == 8 ? (T) reinterpret_cast<const double*>(tmp.data())[i]
+                                       : (T) reinterpret_cast<const float*>(tmp.data())[i];
     return 0;
+}
+
+extern "C" {
+
+int opesci_ctx_receivers(OpesciContext* ctx, float* host, int64_t capacity) {
+    return receivers_as(ctx, host, capacity);
+}
+
+int opesci_ctx_receivers_f64(OpesciContext* ctx, double* host, int64_t capacity) {
+    return receivers_as(ctx, host, capacity);
 }
 
 int opesci_ctx_kernel_times(OpesciContext* ctx, int32_t nsteps, double out_ms[3]) {
@@ -675,6 +757,200 @@ int opesci_ctx_configure(OpesciContext* ctx, int32_t kernel_variant, int32_t til
         (void)hipGraphExecDestroy(ctx->graph);
         ctx->graph = nullptr;
     }
+    return 0;
+}
+
+// ---------------------------------------------------------------- adjoint
+static int adj_alloc(OpesciContext* c, int ckpt, int seg) {
+    const Geom& g = c->g;
+    const size_t rb = c->real_bytes;
+    if (!c->abase[0][0]) {
+        for (int f = 0; f < NF; ++f)
+            for (int l = 0; l < 2; ++l) {
+                if (int e = alloc_zero(c, &c->abase[f][l], c->field_bytes)) return e;
+                c->aptr[f][l] = (char*)c->abase[f][l] + (size_t)g.zoff * rb;
+            }
+        const size_t mbytes = (size_t)g.nx * g.sx * rb;
+        for (int i = 0; i < OPESCI_M_COUNT; ++i) {
+            if (int e = alloc_zero(c, &c->gbase[i], mbytes)) return e;
+            c->gptr[i] = (char*)c->gbase[i] + (size_t)g.zoff * rb;
+        }
+        if (c->pool_elems > 0) {
+            const size_t pb = ((size_t)c->pool_elems * rb + 15) & ~size_t(15);
+            if (int e = alloc_zero(c, &c->d_gpool, pb)) return e;
+        }
+        int64_t off = 0;
+        for (int ph = 2; ph < 5; ++ph) {
+            c->log_off[ph].clear();
+            for (const BcLevel& lv : c->levels[ph]) {
+                c->log_off[ph].push_back(off);
+                off += (int64_t)lv.count * lv.maxpts;
+            }
+        }
+        if (off > 0) OPESCI_HIP_TRY(hipMalloc(&c->d_log, (size_t)off * rb));
+    }
+    const size_t state = (size_t)2 * NF * c->field_bytes;
+    if (ckpt > c->ckpt_cap) {
+        if (c->d_ckpt) OPESCI_HIP_TRY(hipFree(c->d_ckpt));
+        c->d_ckpt = nullptr;
+        OPESCI_HIP_TRY(hipMalloc((void**)&c->d_ckpt, state * ckpt));
+        c->ckpt_cap = ckpt;
+    }
+    if (seg > c->seg_cap) {
+        if (c->d_seg) OPESCI_HIP_TRY(hipFree(c->d_seg));
+        c->d_seg = nullptr;
+        OPESCI_HIP_TRY(hipMalloc((void**)&c->d_seg, state * seg));
+        c->seg_cap = seg;
+    }
+    return 0;
+}
+
+// copy the 18 field levels to / from a state slot
+static int state_copy(OpesciContext* c, char* slot, bool save) {
+    for (int f = 0; f < NF; ++f)
+        for (int l = 0; l < 2; ++l) {
+            char* s = slot + (size_t)(2 * f + l) * c->field_bytes;
+            OPESCI_HIP_TRY(hipMemcpyAsync(save ? s : (char*)c->base[f][l], save ? c->base[f][l] : s, c->field_bytes,
+                                          hipMemcpyDeviceToDevice, c->stream));
+        }
+    return 0;
+}
+
+// one forward step up to its velocity boundary levels, with every boundary
+// level's overwritten values logged
+static int step_logged(OpesciContext* c, int t0) {
+    const int t1 = 1 - t0;
+    auto levels = [&](int ph) -> int {
+        for (size_t L = 0; L < c->levels[ph].size(); ++L)
+            OPESCI_HIP_TRY(launch_bc_logged(c->g, c->real_bytes, c->fptr, c->d_desc, c->d_pool, c->levels[ph][L],
+                                            (char*)c->d_log + c->log_off[ph][L] * c->real_bytes, t0, t1, 0,
+                                            c->stream));
+        return 0;
+    };
+    OPESCI_HIP_TRY(launch_stress(c->g, c->real_bytes, c->storage, c->het, c->fptr, c->mptr, c->coefs(), t0, t1,
+                                 c->cfg, c->stream));
+    if (int e = levels(4)) return e;
+    if (int e = levels(2)) return e;
+    OPESCI_HIP_TRY(launch_velocity(c->g, c->real_bytes, c->storage, c->het, c->fptr, c->mptr, c->coefs(), t0, t1,
+                                   c->cfg, c->stream));
+    // no receivers / source here: the adjoint reads the state the velocity
+    // boundary left (the source, added after it, is a constant forcing with
+    // an identity adjoint, and must not show in the stresses it reads)
+    return levels(3);
+}
+
+// adjoint of step n, the fields holding that step's logged forward result
+static int step_adjoint(OpesciContext* c, int n) {
+    const int t0 = n % 2, t1 = 1 - t0;
+    auto levels = [&](int ph) -> int {
+        for (int L = (int)c->levels[ph].size() - 1; L >= 0; --L) {
+            const BcLevel& lv = c->levels[ph][L];
+            OPESCI_HIP_TRY(launch_adj_bc(c->g, c->real_bytes, c->fptr, c->aptr, c->d_desc, c->d_pool, c->d_gpool, lv,
+                                         t0, t1, c->stream));
+            OPESCI_HIP_TRY(launch_bc_logged(c->g, c->real_bytes, c->fptr, c->d_desc, c->d_pool, lv,
+                                            (char*)c->d_log + c->log_off[ph][L] * c->real_bytes, t0, t1, 1,
+                                            c->stream));
+        }
+        return 0;
+    };
+    OPESCI_HIP_TRY(launch_adj_receivers(c->g, c->real_bytes, c->fptr, c->aptr, t1, c->nrec, c->d_rec,
+                                        c->d_obs + (size_t)n * c->nrec * 3, c->stream));
+    if (int e = levels(3)) return e;
+    OPESCI_HIP_TRY(launch_adj_velocity(c->g, c->real_bytes, c->fptr, c->aptr, c->mptr, c->gptr, c->coefs(), t0, t1,
+                                       c->stream));
+    if (int e = levels(2)) return e;
+    if (int e = levels(4)) return e;
+    OPESCI_HIP_TRY(launch_adj_stress(c->g, c->real_bytes, c->fptr, c->aptr, c->mptr, c->gptr, c->coefs(), t0, t1,
+                                     c->stream));
+    return 0;
+}
+
+int opesci_ctx_gradient(OpesciContext* c, const float* observed, int64_t count, int32_t every, double* misfit) {
+    if (!c || !observed || every <= 0) return fail(OPESCI_EINVAL, "bad arguments");
+    if (!c->het || c->storage != OPESCI_STORE_NATIVE)
+        return fail(OPESCI_EUNSUPPORTED, "gradients need the read path (media arrays) and native storage");
+    if (c->nrec <= 0) return fail(OPESCI_EINVAL, "gradients need receivers");
+    const int N = c->desc.ntsteps;
+    if (count != (int64_t)N * c->nrec * 3) return fail(OPESCI_EINVAL, "observed must hold ntsteps * nrec * 3 values");
+    const int nck = (N + every - 1) / every;
+    if (int e = adj_alloc(c, std::max(nck, 1), every)) return e;
+    if (!c->d_obs) OPESCI_HIP_TRY(hipMalloc((void**)&c->d_obs, (size_t)count * sizeof(float)));
+    OPESCI_HIP_TRY(hipMemcpyAsync(c->d_obs, observed, (size_t)count * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    const size_t state = (size_t)2 * NF * c->field_bytes;
+    // forward with checkpoints
+    if (int e = ctx_init(c)) return e;
+    for (int n = 0; n < N; ++n) {
+        if (n % every == 0)
+            if (int e = state_copy(c, c->d_ckpt + (size_t)(n / every) * state, true)) return e;
+        if (int e = step_once(c, n % 2)) return e;
+    }
+    c->steps = N;
+    // misfit from the recorded U, V, W traces
+    std::vector<double> tr((size_t)N * c->nrec * 4);
+    if (int e = receivers_as(c, tr.data(), (int64_t)tr.size())) return e;
+    double J = 0;
+    for (int64_t i = 0; i < (int64_t)N * c->nrec; ++i)
+        for (int k = 0; k < 3; ++k) {
+            const double r = tr[(size_t)i * 4 + k] - (double)observed[(size_t)i * 3 + k];
+            J += 0.5 * r * r;
+        }
+    if (misfit) *misfit = J;
+    // reverse sweep, segment by segment
+    for (int f = 0; f < NF; ++f)
+        for (int l = 0; l < 2; ++l) OPESCI_HIP_TRY(launch_zero(c->abase[f][l], c->field_bytes, c->stream));
+    const size_t mbytes = (size_t)c->g.nx * c->g.sx * c->real_bytes;
+    for (int i = 0; i < OPESCI_M_COUNT; ++i) OPESCI_HIP_TRY(launch_zero(c->gbase[i], mbytes, c->stream));
+    if (c->d_gpool)
+        OPESCI_HIP_TRY(launch_zero(c->d_gpool, ((size_t)c->pool_elems * c->real_bytes + 15) & ~size_t(15), c->stream));
+    for (int seg = nck - 1; seg >= 0; --seg) {
+        const int k0 = seg * every, k1 = std::min(N, k0 + every);
+        if (int e = state_copy(c, c->d_ckpt + (size_t)seg * state, false)) return e;
+        OPESCI_HIP_TRY(launch_set_step(c->d_step, k0, c->stream));
+        for (int n = k0; n < k1; ++n) {
+            if (int e = state_copy(c, c->d_seg + (size_t)(n - k0) * state, true)) return e;
+            if (int e = step_once(c, n % 2)) return e;
+        }
+        for (int n = k1 - 1; n >= k0; --n) {
+            if (int e = state_copy(c, c->d_seg + (size_t)(n - k0) * state, false)) return e;
+            OPESCI_HIP_TRY(launch_set_step(c->d_step, n, c->stream));
+            if (int e = step_logged(c, n % 2)) return e;
+            if (int e = step_adjoint(c, n)) return e;
+        }
+    }
+    // media-dependent boundary coefficients -> media cells
+    for (int ph = 0; ph < 5; ++ph)
+        for (const BcLevel& lv : c->levels[ph])
+            OPESCI_HIP_TRY(launch_adj_coef_media(c->g, c->real_bytes, c->mptr, c->gptr, c->d_desc, c->d_code,
+                                                 c->d_dterms, c->d_gpool, lv, c->stream));
+    OPESCI_HIP_TRY(hipStreamSynchronize(c->stream));
+    OPESCI_HIP_TRY(hipGetLastError());
+    c->initialised = false;   // the fields hold a recomputed state; opesci_ctx_init before stepping
+    c->steps = 0;
+    return 0;
+}
+
+int opesci_ctx_download_gradient(OpesciContext* c, int32_t media, void* host) {
+    if (!c || !host || media < 0 || media >= OPESCI_M_COUNT) return fail(OPESCI_EINVAL, "bad arguments");
+    if (!c->gbase[media]) return fail(OPESCI_ESTATE, "no gradient: call opesci_ctx_gradient first");
+    const size_t dense = (size_t)c->g.nx * c->g.ny * c->g.nz;
+    OPESCI_HIP_TRY(launch_unpack(c->g, c->real_bytes, OPESCI_STORE_NATIVE, c->gptr[media], c->d_stage, c->stream));
+    OPESCI_HIP_TRY(hipMemcpyAsync(host, c->d_stage, dense * c->real_bytes, hipMemcpyDeviceToHost, c->stream));
+    OPESCI_HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int opesci_ctx_upload_media(OpesciContext* c, int32_t media, const void* host) {
+    if (!c || !host || media < 0 || media >= OPESCI_M_COUNT) return fail(OPESCI_EINVAL, "bad arguments");
+    if (!c->het || !c->mptr[media]) return fail(OPESCI_EUNSUPPORTED, "media arrays exist on the read path only");
+    const size_t dense = (size_t)c->g.nx * c->g.ny * c->g.nz;
+    OPESCI_HIP_TRY(hipMemcpyAsync(c->d_stage, host, dense * c->real_bytes, hipMemcpyHostToDevice, c->stream));
+    OPESCI_HIP_TRY(launch_pack_media(c->g, c->real_bytes, c->d_stage, c->mptr[media], c->stream));
+    if (c->d_pool)
+        for (int ph = 0; ph < 5; ++ph)
+            for (const BcLevel& lv : c->levels[ph])
+                OPESCI_HIP_TRY(launch_bc_coef(c->g, c->real_bytes, c->mptr, c->d_desc, c->d_code, c->d_pool, lv.first,
+                                              lv.count, lv.maxpts, c->stream));
+    OPESCI_HIP_TRY(hipStreamSynchronize(c->stream));
     return 0;
 }
 
@@ -820,7 +1096,8 @@ int opesci_model_load(const OpesciModelDesc* desc) {
     const size_t sz_me = desc->media_arrays ? 3 * dense * sizeof(float) : 0;
     const size_t sz_w = (size_t)std::max(desc->ntsteps, 1) * desc->src_on * sizeof(float);
     const size_t sz_r = (size_t)desc->nrec * 3 * sizeof(int32_t);
-    g_loaded_blob.assign(sz_st + sz_te + sz_co + sz_so + sz_me + sz_w + sz_r + 64, 0);
+    const size_t sz_dt = (size_t)desc->ndterms * sizeof(OpesciBcDTerm);
+    g_loaded_blob.assign(sz_st + sz_te + sz_co + sz_so + sz_me + sz_w + sz_r + sz_dt + 72, 0);
     char* p = g_loaded_blob.data();
     g_loaded = *desc;
     auto take = [&](const void* src, size_t n) -> void* {
@@ -841,6 +1118,7 @@ int opesci_model_load(const OpesciModelDesc* desc) {
     }
     g_loaded.src_wavelet = desc->src_on ? (const float*)take(desc->src_wavelet, sz_w) : nullptr;
     g_loaded.rec_idx = desc->nrec ? (const int32_t*)take(desc->rec_idx, sz_r) : nullptr;
+    g_loaded.dterms = desc->ndterms ? (const OpesciBcDTerm*)take(desc->dterms, sz_dt) : nullptr;
     g_have_loaded = true;
     return 0;
 }

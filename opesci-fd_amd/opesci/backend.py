@@ -17,6 +17,8 @@ from .compilation import library_path
 
 ABI_VERSION = 1
 FIELDS = ('U', 'V', 'W', 'Txx', 'Tyy', 'Tzz', 'Txy', 'Tyz', 'Txz')
+MEDIA_IDS = {'rho': 0, 'vp': 1, 'vs': 2, 'beta': 3, 'lambda': 4, 'mu': 5, 'beta1': 6, 'beta2': 7, 'beta3': 8,
+             'mu12': 9, 'mu13': 10, 'mu23': 11}
 
 _i32 = ctypes.c_int32
 _f64 = ctypes.c_double
@@ -37,6 +39,7 @@ class OpesciModelDesc(ctypes.Structure):
         ('src_on', _i32), ('src_idx', _i32 * 3), ('src_wavelet', _vp),
         ('nrec', _i32), ('pad2', _i32), ('rec_idx', _vp),
         ('tile_y', _i32), ('x_chunk', _i32), ('kernel_variant', _i32), ('use_graph', _i32),
+        ('ndterms', _i32), ('pad3', _i32), ('dterms', _vp),
     ]
 
 
@@ -51,9 +54,10 @@ class OpesciProfilingC(ctypes.Structure):
 _LIB = {}
 EXPORTS = ('opesci_model_load', 'opesci_execute', 'opesci_convergence', 'opesci_free',
            'opesci_ctx_create', 'opesci_ctx_destroy', 'opesci_ctx_init', 'opesci_ctx_run',
-           'opesci_ctx_l2', 'opesci_ctx_download', 'opesci_ctx_upload', 'opesci_ctx_receivers',
+           'opesci_ctx_l2', 'opesci_ctx_download', 'opesci_ctx_upload', 'opesci_ctx_receivers', 'opesci_ctx_receivers_f64',
            'opesci_ctx_steps', 'opesci_ctx_kernel_times', 'opesci_ctx_stats', 'opesci_ctx_configure',
-           'opesci_batch_create', 'opesci_batch_run', 'opesci_batch_destroy', 'opesci_last_error',
+           'opesci_batch_create', 'opesci_batch_run', 'opesci_batch_destroy', 'opesci_ctx_gradient',
+           'opesci_ctx_download_gradient', 'opesci_ctx_upload_media', 'opesci_last_error',
            'opesci_abi_version', 'opesci_device_info')
 
 
@@ -77,6 +81,7 @@ def load(path=None):
         lib.opesci_ctx_download.argtypes = [_vp, _i32, _i32, _vp]
         lib.opesci_ctx_upload.argtypes = [_vp, _i32, _i32, _vp]
         lib.opesci_ctx_receivers.argtypes = [_vp, _vp, ctypes.c_int64]
+        lib.opesci_ctx_receivers_f64.argtypes = [_vp, _vp, ctypes.c_int64]
         lib.opesci_ctx_steps.argtypes = [_vp, ctypes.POINTER(_i32)]
         lib.opesci_ctx_kernel_times.argtypes = [_vp, _i32, ctypes.POINTER(_f64)]
         lib.opesci_ctx_stats.argtypes = [_vp, ctypes.POINTER(_i32)]
@@ -84,6 +89,9 @@ def load(path=None):
         lib.opesci_batch_create.argtypes = [ctypes.POINTER(_vp), _i32, ctypes.POINTER(_vp)]
         lib.opesci_batch_run.argtypes = [_vp, _i32, ctypes.POINTER(_f64)]
         lib.opesci_batch_destroy.argtypes = [_vp]
+        lib.opesci_ctx_gradient.argtypes = [_vp, _vp, ctypes.c_int64, _i32, ctypes.POINTER(_f64)]
+        lib.opesci_ctx_download_gradient.argtypes = [_vp, _i32, _vp]
+        lib.opesci_ctx_upload_media.argtypes = [_vp, _i32, _vp]
         lib.opesci_model_load.argtypes = [ctypes.POINTER(OpesciModelDesc)]
         lib.opesci_execute.argtypes = [_vp, _vp]
         lib.opesci_convergence.argtypes = [_vp, _vp]
@@ -160,6 +168,10 @@ def make_desc(plan):
     d.x_chunk = int(plan.x_chunk or 0)
     d.kernel_variant = int(plan.kernel_variant or 0)
     d.use_graph = int(plan.use_graph)
+    dterms = getattr(plan, 'dterms', None)
+    if dterms is not None and len(dterms):
+        d.ndterms = len(dterms)
+        d.dterms = ptr(dterms)
     return d, keep
 
 
@@ -246,10 +258,14 @@ class Context(object):
         _check(self.lib.opesci_ctx_steps(self.h, ctypes.byref(s)), 'opesci_ctx_steps')
         return s.value
 
-    def receivers(self):
+    def receivers(self, dtype=np.float32):
+        """Traces [step][receiver][U, V, W, pressure]; dtype float64 returns
+        the unrounded traces of an FP64 context."""
         n = self.steps() * max(int(self.desc.nrec), 0) * 4
-        out = np.zeros(max(n, 1), dtype=np.float32)
-        _check(self.lib.opesci_ctx_receivers(self.h, out.ctypes.data, n), 'opesci_ctx_receivers')
+        f64 = np.dtype(dtype) == np.float64
+        out = np.zeros(max(n, 1), dtype=np.float64 if f64 else np.float32)
+        fn = 'opesci_ctx_receivers_f64' if f64 else 'opesci_ctx_receivers'
+        _check(getattr(self.lib, fn)(self.h, out.ctypes.data, n), fn)
         return out[:n].reshape(-1, int(self.desc.nrec), 4) if n else out[:0]
 
     def kernel_times(self, nsteps):
@@ -267,3 +283,29 @@ class Context(object):
     def configure(self, kernel_variant=0, tile_y=0, x_chunk=0):
         """Interior launch configuration (results are the same for every one)."""
         _check(self.lib.opesci_ctx_configure(self.h, kernel_variant, tile_y, x_chunk), 'opesci_ctx_configure')
+
+    def gradient(self, observed, checkpoint_every=32):
+        """Misfit J = 1/2 sum (U,V,W receiver traces - observed)^2 and, left on
+        the device, dJ/d(media arrays) by the adjoint (see download_gradient).
+        observed: (ntsteps, nrec, 3) float32."""
+        obs = np.ascontiguousarray(observed, dtype=np.float32)
+        j = _f64()
+        _check(self.lib.opesci_ctx_gradient(self.h, obs.ctypes.data, obs.size, int(checkpoint_every), ctypes.byref(j)),
+               'opesci_ctx_gradient')
+        return j.value
+
+    def download_gradient(self, media):
+        """dJ/d(media array) of the last gradient() call, dense (dim1, dim2, dim3)."""
+        mid = MEDIA_IDS[media] if isinstance(media, str) else int(media)
+        a = np.zeros(self.plan.dims, dtype=self.dtype)
+        _check(self.lib.opesci_ctx_download_gradient(self.h, mid, a.ctypes.data), 'opesci_ctx_download_gradient')
+        return a
+
+    def upload_media(self, media, arr):
+        """Replace one media array (dense, compute type) and refresh the
+        media-dependent boundary coefficients."""
+        mid = MEDIA_IDS[media] if isinstance(media, str) else int(media)
+        a = np.ascontiguousarray(arr, dtype=self.dtype)
+        if a.shape != tuple(self.plan.dims):
+            raise ValueError('media array shape %s, grid %s' % (a.shape, self.plan.dims))
+        _check(self.lib.opesci_ctx_upload_media(self.h, mid, a.ctypes.data), 'opesci_ctx_upload_media')

@@ -37,6 +37,8 @@ STMT_DTYPE = np.dtype([('field', '<i4'), ('level', '<i4'), ('axis', '<i4'), ('pl
                        ('nterms', '<i4'), ('phase', '<i4'), ('sched', '<i4')])
 TERM_DTYPE = np.dtype([('field', '<i4'), ('level', '<i4'), ('plane', '<i4'), ('off', '<i4', (2,)),
                        ('prog', '<i4'), ('prog_len', '<i4'), ('pad', '<i4'), ('coef', '<f8')])
+DTERM_DTYPE = np.dtype([('term', '<i4'), ('media', '<i4'), ('off', '<i4', (3,)), ('prog', '<i4'),
+                        ('prog_len', '<i4'), ('pad', '<i4')])
 
 
 class PlanError(NotImplementedError):
@@ -234,6 +236,7 @@ class _BcBuilder(object):
         self.stmts = []
         self.terms = []
         self.code = []
+        self.dterms = []
         self.op_of_stmt = []
 
     def media_anchor_for(self, axis):
@@ -293,22 +296,42 @@ class _BcBuilder(object):
                     prog, plen, coef = -1, 0, f_lit(sum(f_lit(n) for n, _ in parts))
                 elif len(parts) == 1:
                     num, other = parts[0]
-                    code = exprvm.compile_expr(Mul(*other), coords={}, consts={},
+                    expr = Mul(*other)
+                    code = exprvm.compile_expr(expr, coords={}, consts={},
                                                media_anchor=self.media_anchor_for(axis))
                     code = [(op, a, f_lit(v) if op == exprvm.OP_CONST else v) for op, a, v in code]
                     prog, plen, coef = len(self.code), len(code), f_lit(num)
                     self.code += code
+                    self._derivatives(expr, axis)
                 else:
                     expr = Add(*[Float(f_lit(n)) * Mul(*o) for n, o in parts], evaluate=False)
                     code = exprvm.compile_expr(expr, coords={}, consts={},
                                                media_anchor=self.media_anchor_for(axis))
                     prog, plen, coef = len(self.code), len(code), 1.0
                     self.code += code
+                    self._derivatives(expr, axis)
                 self.terms.append((src, lvl, p, list(offs), prog, plen, coef))
             self.stmts.append(dict(field=dst, level=_time_level(lhs.indices[0], g), axis=axis,
                                    plane=plane, lo=[r[0] for r in ranges], hi=[r[1] for r in ranges],
                                    term0=t0, nterms=len(self.terms) - t0, phase=phase, sched=0))
             self.op_of_stmt.append(op_id)
+
+    def _derivatives(self, expr, axis):
+        """Partial derivatives of a media-dependent coefficient program with
+        respect to each media cell it reads (for the adjoint's media
+        gradient): one (term, media id, offsets, program) entry per cell; the
+        term is the one about to be appended."""
+        term = len(self.terms)
+        anchor = self.media_anchor_for(axis)
+        cells = sorted((e for e in expr.atoms(Indexed) if str(e.base.label) in exprvm.MEDIA_IDS), key=str)
+        for cell in cells:
+            d = expr.diff(cell)
+            if d == 0:
+                continue
+            mid, offs = anchor(cell)
+            code = exprvm.compile_expr(d, coords={}, consts={}, media_anchor=anchor)
+            self.dterms.append((term, mid, list(offs), len(self.code), len(code)))
+            self.code += code
 
     # -------- hazard analysis
     def _boxes(self, s, lo=None, hi=None):
@@ -477,7 +500,14 @@ class _BcBuilder(object):
             te[i]['prog'] = prog
             te[i]['prog_len'] = plen
             te[i]['coef'] = coef
-        return st, te, exprvm.to_array(self.code)
+        dt = np.zeros(len(self.dterms), dtype=DTERM_DTYPE)
+        for i, (term, mid, offs, prog, plen) in enumerate(self.dterms):
+            dt[i]['term'] = term
+            dt[i]['media'] = mid
+            dt[i]['off'] = offs
+            dt[i]['prog'] = prog
+            dt[i]['prog_len'] = plen
+        return st, te, exprvm.to_array(self.code), dt
 
 
 def _build_bc(grid, slots):
@@ -620,7 +650,7 @@ def build_plan(grid):
         p.media = _media_arrays(grid, p.dims)
     p.cn, p.cs, p.cv = _extract_interior(grid, slots)
     b = _build_bc(grid, slots)
-    p.stmts, p.terms, p.bc_code = b.arrays()
+    p.stmts, p.terms, p.bc_code, p.dterms = b.arrays()
     p.stmt_seq = np.asarray(b.seq, dtype=np.int64)
     p.fused_frame = b.fused_frame
     p.nlaunches_step = int(sum(len(set(p.stmts['sched'][p.stmts['phase'] == ph])) for ph in (2, 3)))

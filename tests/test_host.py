@@ -205,3 +205,53 @@ def test_malformed_description_rejected_before_device():
     desc, keep = backend.make_desc(plan)
     desc.space_order = 10
     assert lib.opesci_ctx_create(ctypes.byref(desc), ctypes.byref(h)) == 6
+
+
+def test_boundary_coefficient_derivatives():
+    """The adjoint's derivative programs (plan.dterms) equal central finite
+    differences of each media-dependent boundary coefficient with respect to
+    the media cell they name (FP64, relative step 1e-6), and every media
+    cell a coefficient reads has one."""
+    from opesci.plan import DTERM_DTYPE
+    from planvm import _code, eval_code
+    g, _ = hetero_model(14, 4, 2, double=True, sigma=2.0)
+    p = g.lower()
+    assert p.dterms.dtype == DTERM_DTYPE and len(p.dterms) > 0
+    rng = np.random.default_rng(1)
+    cells = {}
+    checked = 0
+    for i in rng.choice(len(p.dterms), size=min(40, len(p.dterms)), replace=False):
+        d = p.dterms[i]
+        t = p.terms[d['term']]
+        prog = _code(p.bc_code, int(t['prog']), int(t['prog_len']))
+        dprog = _code(p.bc_code, int(d['prog']), int(d['prog_len']))
+        vals = {}
+
+        def media_at(mid, offs):
+            key = (int(mid), tuple(int(v) for v in offs))
+            if key not in vals:
+                vals[key] = 1.0 + rng.uniform(0.2, 2.0)
+            return vals[key]
+        eval_code(prog, None, 0.0, media_at)
+        key = (int(d['media']), tuple(int(v) for v in d['off']))
+        assert key in vals
+        base = vals[key]
+        h = 1e-6 * base
+
+        def f(v):
+            vals[key] = v
+            return eval_code(prog, None, 0.0, media_at)
+        fd = (f(base + h) - f(base - h)) / (2 * h)
+        vals[key] = base
+        an = eval_code(dprog, None, 0.0, media_at)
+        assert abs(an - fd) <= 1e-6 * max(1.0, abs(fd)), (i, an, fd)
+        checked += 1
+        cells.setdefault(int(d['term']), set()).add(key)
+    assert checked > 10
+    # completeness: every media cell of a sampled term has a derivative entry
+    for term, keys in list(cells.items())[:10]:
+        t = p.terms[term]
+        prog = _code(p.bc_code, int(t['prog']), int(t['prog_len']))
+        reads = set((a[0], tuple(a[1:4])) for op, a, v in prog if op == 3)
+        have = set((int(d['media']), tuple(int(v) for v in d['off'])) for d in p.dterms if d['term'] == term)
+        assert reads == have
