@@ -216,6 +216,217 @@ __global__ void adj_vel_point(Geom g, FieldPtrs<R> F, AdjPtrs<R> B, GradPtrs<R> 
     }
 }
 
+// ------------------------------------------------------------------ fast interior transposes
+// The transposed interior updates have the forward stencils' shapes:
+//   bar T(t1) += stress stencil of Q_i = beta_i bar V_i(t1)
+//                (cn'[i][i] = -cv[i][i], cs'[s] = -cv of the two velocities
+//                 that read T_s)
+//   bar V(t0) += velocity stencil of P_j = lambda (bar Txx + bar Tyy + bar Tzz)
+//                + 2 mu bar T_jj and W_s = mu_s bar T_s
+//                (cv'[j][j] = -cn[.][j], cv'[i][j] = -cs of the stress term
+//                 that reads V_i along j)
+// The P form needs cn[i][j][k] independent of i, which the launcher checks.
+// A point kernel forms Q (or P, W) in scratch arrays that are zero outside
+// the interior, moves bar t1 into bar t0 and accumulates the media gradients;
+// the forward sweep kernels (homogeneous form) then update interior targets
+// in place, and a rim kernel gathers the targets in the M-thick face slabs
+// (halo cells that interior stencils read along one axis).
+template <typename R>
+struct Scratch {
+    R* a[6];
+};
+
+__device__ __forceinline__ bool interior_cell(const Geom& g, int& x, int& y, int& z) {
+    const int M = g.m;
+    z = M + (int)blockIdx.x * 64 + (int)(threadIdx.x & 63);
+    y = M + (int)blockIdx.y * 4 + (int)(threadIdx.x >> 6);
+    x = M + (int)blockIdx.z;
+    return z < g.nz - M && y < g.ny - M;
+}
+
+// Q_i = beta_i bar V_i(t1); bar V(t0) += bar V(t1); bar V(t1) = 0; beta
+// gradients from the forward stresses at t1.
+template <typename R, int M>
+__global__ __launch_bounds__(256) void adj_vel_prep(Geom g, FieldPtrs<R> F, AdjPtrs<R> B, MediaPtrs<R> md,
+                                                    GradPtrs<R> gm, Coefs<R> c, Scratch<R> S, int t0, int t1) {
+    int x, y, z;
+    if (!interior_cell(g, x, y, z)) return;
+    const int64_t a = g.at(x, y, z);
+    const int64_t st[3] = {g.sx, g.sy, 1};
+    const int src[3][3] = {{3, 6, 8}, {6, 4, 7}, {8, 7, 5}};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const R b = B.f[i][t1][a];
+        S.a[i][a] = md.m[OPESCI_M_BETA1 + i][a] * b;
+        B.f[i][t0][a] += b;
+        B.f[i][t1][a] = 0;
+        R s = 0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const int o = i == j ? 1 : 0;
+            const R* t = F.f[src[i][j]][t1] + a;
+#pragma unroll
+            for (int k = 0; k < M; ++k) s = fma(c.cv[i][j][k], t[(o + k) * st[j]] - t[(o - 1 - k) * st[j]], s);
+        }
+        gm.m[OPESCI_M_BETA1 + i][a] += b * s;
+    }
+}
+
+// P_j, W_s from bar T(t1); bar T(t0) += bar T(t1); bar T(t1) = 0; lambda, mu,
+// mu12/mu23/mu13 gradients from the forward velocities at t0.
+template <typename R, int M>
+__global__ __launch_bounds__(256) void adj_stress_prep(Geom g, FieldPtrs<R> F, AdjPtrs<R> B, MediaPtrs<R> md,
+                                                       GradPtrs<R> gm, Coefs<R> c, Scratch<R> S, int t0, int t1) {
+    int x, y, z;
+    if (!interior_cell(g, x, y, z)) return;
+    const int64_t a = g.at(x, y, z);
+    const int64_t st[3] = {g.sx, g.sy, 1};
+    R bt[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        bt[f] = B.f[3 + f][t1][a];
+        B.f[3 + f][t0][a] += bt[f];
+        B.f[3 + f][t1][a] = 0;
+    }
+    const R lam = md.m[OPESCI_M_LAMBDA][a], mu = md.m[OPESCI_M_MU][a];
+    const R tr = bt[0] + bt[1] + bt[2];
+    const int mid[3] = {OPESCI_M_MU12, OPESCI_M_MU23, OPESCI_M_MU13};
+#pragma unroll
+    for (int j = 0; j < 3; ++j) S.a[j][a] = lam * tr + (R)2 * mu * bt[j];
+#pragma unroll
+    for (int s = 0; s < 3; ++s) S.a[3 + s][a] = md.m[mid[s]][a] * bt[3 + s];
+    R glam = 0, gmu = 0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const R* v = F.f[j][t0] + a;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            R d = 0;
+#pragma unroll
+            for (int k = 0; k < M; ++k) d = fma(c.cn[i][j][k], v[k * st[j]] - v[-(1 + k) * st[j]], d);
+            glam += bt[i] * d;
+            if (i == j) gmu += (R)2 * bt[i] * d;
+        }
+    }
+    gm.m[OPESCI_M_LAMBDA][a] += glam;
+    gm.m[OPESCI_M_MU][a] += gmu;
+    const int fa[3] = {0, 1, 0}, fb[3] = {1, 2, 2};   // differentiated velocities
+    const int aa[3] = {1, 2, 2}, ab[3] = {0, 1, 0};   // their axes
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        const R* va = F.f[fa[s]][t0] + a;
+        const R* vb = F.f[fb[s]][t0] + a;
+        R v = 0;
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            v = fma(c.cs[s][0][k], va[(1 + k) * st[aa[s]]] - va[-k * st[aa[s]]], v);
+            v = fma(c.cs[s][1][k], vb[(1 + k) * st[ab[s]]] - vb[-k * st[ab[s]]], v);
+        }
+        gm.m[mid[s]][a] += bt[3 + s] * v;
+    }
+}
+
+// Face-slab targets: slab blockIdx.y = 2 * axis + side, cells with that
+// coordinate in the M halo planes of the side and the other two interior.
+__device__ __forceinline__ bool rim_cell(const Geom& g, int& x, int& y, int& z) {
+    const int M = g.m;
+    const int sl = (int)blockIdx.y, ax = sl >> 1, side = sl & 1;
+    const int n[3] = {g.nx, g.ny, g.nz};
+    const int b = ax == 0 ? 1 : 0, cc = ax == 2 ? 1 : 2;
+    const int nb = n[b] - 2 * M, nc = n[cc] - 2 * M;
+    const int e = (int)blockIdx.x * (int)blockDim.x + (int)threadIdx.x;
+    if (e >= M * nb * nc) return false;
+    const int d = e / (nb * nc), r = e % (nb * nc);
+    int q[3];
+    q[ax] = side ? n[ax] - M + d : d;
+    q[b] = M + r / nc;
+    q[cc] = M + r % nc;
+    x = q[0];
+    y = q[1];
+    z = q[2];
+    return true;
+}
+
+// bar T(t1) at face-slab cells from Q (velocity-update transpose).
+template <typename R, int M>
+__global__ __launch_bounds__(256) void adj_rim_t(Geom g, AdjPtrs<R> B, Scratch<R> S, Coefs<R> c, int t1) {
+    int x, y, z;
+    if (!rim_cell(g, x, y, z)) return;
+    const int q[3] = {x, y, z};
+    const int64_t at = g.at(x, y, z);
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        // (velocity i, axis j) pairs reading stress f
+        int vi[2], vj[2], cnt;
+        if (f < 3) {
+            vi[0] = f; vj[0] = f; cnt = 1;
+        } else if (f == 3) {
+            vi[0] = 0; vj[0] = 1; vi[1] = 1; vj[1] = 0; cnt = 2;
+        } else if (f == 4) {
+            vi[0] = 1; vj[0] = 2; vi[1] = 2; vj[1] = 1; cnt = 2;
+        } else {
+            vi[0] = 0; vj[0] = 2; vi[1] = 2; vj[1] = 0; cnt = 2;
+        }
+        const int off = f < 3 ? 1 : 0;
+        R acc = 0;
+        for (int u = 0; u < cnt; ++u) {
+            const int i = vi[u], j = vj[u];
+            for (int k = 0; k < M; ++k)
+                for (int sgn = 0; sgn < 2; ++sgn) {
+                    int p[3] = {q[0], q[1], q[2]};
+                    p[j] -= sgn == 0 ? off + k : off - 1 - k;
+                    if (!inside(g, p[0], p[1], p[2])) continue;
+                    const R v = c.cv[i][j][k] * S.a[i][g.at(p[0], p[1], p[2])];
+                    acc += sgn == 0 ? v : -v;
+                }
+        }
+        B.f[3 + f][t1][at] += acc;
+    }
+}
+
+// bar V(t0) at face-slab cells from P, W (stress-update transpose).
+template <typename R, int M>
+__global__ __launch_bounds__(256) void adj_rim_v(Geom g, AdjPtrs<R> B, Scratch<R> S, Coefs<R> c, int t0) {
+    int x, y, z;
+    if (!rim_cell(g, x, y, z)) return;
+    const int q[3] = {x, y, z};
+    const int64_t at = g.at(x, y, z);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        R acc = 0;
+        for (int k = 0; k < M; ++k)
+            for (int sgn = 0; sgn < 2; ++sgn) {
+                int p[3] = {q[0], q[1], q[2]};
+                p[j] += sgn == 0 ? -k : 1 + k;
+                if (!inside(g, p[0], p[1], p[2])) continue;
+                const R v = c.cn[0][j][k] * S.a[j][g.at(p[0], p[1], p[2])];
+                acc += sgn == 0 ? v : -v;
+            }
+        // (shear slot, axis, coefficient set) of the stress terms reading V_j
+        int sh[2], ax[2], cc[2];
+        if (j == 0) {
+            sh[0] = 0; ax[0] = 1; cc[0] = 0;
+            sh[1] = 2; ax[1] = 2; cc[1] = 0;
+        } else if (j == 1) {
+            sh[0] = 0; ax[0] = 0; cc[0] = 1;
+            sh[1] = 1; ax[1] = 2; cc[1] = 0;
+        } else {
+            sh[0] = 1; ax[0] = 1; cc[0] = 1;
+            sh[1] = 2; ax[1] = 0; cc[1] = 1;
+        }
+        for (int u = 0; u < 2; ++u)
+            for (int k = 0; k < M; ++k)
+                for (int sgn = 0; sgn < 2; ++sgn) {
+                    int p[3] = {q[0], q[1], q[2]};
+                    p[ax[u]] += sgn == 0 ? -(1 + k) : k;
+                    if (!inside(g, p[0], p[1], p[2])) continue;
+                    const R v = c.cs[sh[u]][cc[u]][k] * S.a[3 + sh[u]][g.at(p[0], p[1], p[2])];
+                    acc += sgn == 0 ? v : -v;
+                }
+        B.f[j][t0][at] += acc;
+    }
+}
+
 // ------------------------------------------------------------------ boundary levels
 // Forward statement with the destination's previous value logged
 // (log[(descriptor - first) * maxpts + point]).
@@ -327,45 +538,116 @@ static dim3 grid_for(int64_t n) {
     return dim3((unsigned)std::min<int64_t>(b, 65536));
 }
 
+// cn[i][j][k] independent of i: the P form of the stress transpose applies
+template <typename R>
+static bool normal_coefs_shared(const Coefs<R>& c, int M) {
+    for (int j = 0; j < 3; ++j)
+        for (int k = 0; k < M; ++k)
+            if (c.cn[1][j][k] != c.cn[0][j][k] || c.cn[2][j][k] != c.cn[0][j][k]) return false;
+    return true;
+}
+
+static dim3 interior_grid(const Geom& g) {
+    const int M = g.m;
+    return dim3((unsigned)((g.nz - 2 * M + 63) / 64), (unsigned)((g.ny - 2 * M + 3) / 4), (unsigned)(g.nx - 2 * M));
+}
+
+static dim3 rim_grid(const Geom& g) {
+    const int M = g.m;
+    const int ni[3] = {g.nx - 2 * M, g.ny - 2 * M, g.nz - 2 * M};
+    const int slab = M * std::max(ni[1] * ni[2], std::max(ni[0] * ni[2], ni[0] * ni[1]));
+    return dim3((unsigned)((slab + 255) / 256), 6);
+}
+
 template <typename R, int M>
 static hipError_t adj_stress_t(const Geom& g, void* fields, void* bar, const void* media, void* grad,
-                               const void* coefs, int t0, int t1, hipStream_t s) {
+                               const void* coefs, void* const* scratch, const InteriorLaunch& cfg, int t0, int t1,
+                               hipStream_t s) {
     const int64_t n = (int64_t)g.nx * g.ny * g.nz;
     const FieldPtrs<R>& F = *static_cast<const FieldPtrs<R>*>(fields);
     const AdjPtrs<R>& B = *static_cast<const AdjPtrs<R>*>(bar);
     const MediaPtrs<R>& md = *static_cast<const MediaPtrs<R>*>(media);
     const GradPtrs<R>& gm = *static_cast<const GradPtrs<R>*>(grad);
     const Coefs<R>& c = *static_cast<const Coefs<R>*>(coefs);
-    hipLaunchKernelGGL((adj_stress_to_v<R, M>), grid_for(3 * n), dim3(256), 0, s, g, B, md, c, t0, t1);
-    hipLaunchKernelGGL((adj_stress_point<R, M>), grid_for(n), dim3(256), 0, s, g, F, B, gm, c, t0, t1);
+    if (!scratch || cfg.variant == 1 || !normal_coefs_shared(c, M)) {
+        hipLaunchKernelGGL((adj_stress_to_v<R, M>), grid_for(3 * n), dim3(256), 0, s, g, B, md, c, t0, t1);
+        hipLaunchKernelGGL((adj_stress_point<R, M>), grid_for(n), dim3(256), 0, s, g, F, B, gm, c, t0, t1);
+        return hipGetLastError();
+    }
+    Scratch<R> S;
+    for (int i = 0; i < 6; ++i) S.a[i] = static_cast<R*>(scratch[i]);
+    // the weighted stresses must be read from bar T(t1) before the rim and
+    // sweep kernels touch bar V(t0); the prep kernel zeroes bar T(t1)
+    hipLaunchKernelGGL((adj_stress_prep<R, M>), interior_grid(g), dim3(256), 0, s, g, F, B, md, gm, c, S, t0, t1);
+    // velocity-shaped sweep: level 0 = level 1 = bar V(t0), stresses = P, W
+    FieldPtrs<R> A;
+    for (int i = 0; i < 3; ++i) A.f[i][0] = A.f[i][1] = B.f[i][t0];
+    for (int f = 0; f < 6; ++f) A.f[3 + f][0] = A.f[3 + f][1] = S.a[f];
+    Coefs<R> cb = {};
+    const int pr[3][2] = {{0, 1}, {1, 2}, {0, 2}};
+    for (int k = 0; k < M; ++k) {
+        for (int j = 0; j < 3; ++j) cb.cv[j][j][k] = -c.cn[0][j][k];
+        for (int sl = 0; sl < 3; ++sl) {
+            const int ia = pr[sl][0], ja = pr[sl][1];
+            cb.cv[ia][ja][k] = -c.cs[sl][0][k];   // T_s reads V_ia along ja
+            cb.cv[ja][ia][k] = -c.cs[sl][1][k];   // and V_ja along ia
+        }
+    }
+    if (hipError_t e = launch_velocity(g, sizeof(R), OPESCI_STORE_NATIVE, false, &A, media, &cb, 0, 1, cfg, s))
+        return e;
+    hipLaunchKernelGGL((adj_rim_v<R, M>), rim_grid(g), dim3(256), 0, s, g, B, S, c, t0);
     return hipGetLastError();
 }
 
 template <typename R, int M>
 static hipError_t adj_vel_t(const Geom& g, void* fields, void* bar, const void* media, void* grad, const void* coefs,
-                            int t0, int t1, hipStream_t s) {
+                            void* const* scratch, const InteriorLaunch& cfg, int t0, int t1, hipStream_t s) {
     const int64_t n = (int64_t)g.nx * g.ny * g.nz;
     const FieldPtrs<R>& F = *static_cast<const FieldPtrs<R>*>(fields);
     const AdjPtrs<R>& B = *static_cast<const AdjPtrs<R>*>(bar);
     const MediaPtrs<R>& md = *static_cast<const MediaPtrs<R>*>(media);
     const GradPtrs<R>& gm = *static_cast<const GradPtrs<R>*>(grad);
     const Coefs<R>& c = *static_cast<const Coefs<R>*>(coefs);
-    hipLaunchKernelGGL((adj_vel_to_t<R, M>), grid_for(6 * n), dim3(256), 0, s, g, B, md, c, t1);
-    hipLaunchKernelGGL((adj_vel_point<R, M>), grid_for(n), dim3(256), 0, s, g, F, B, gm, c, t0, t1);
+    if (!scratch || cfg.variant == 1) {
+        hipLaunchKernelGGL((adj_vel_to_t<R, M>), grid_for(6 * n), dim3(256), 0, s, g, B, md, c, t1);
+        hipLaunchKernelGGL((adj_vel_point<R, M>), grid_for(n), dim3(256), 0, s, g, F, B, gm, c, t0, t1);
+        return hipGetLastError();
+    }
+    Scratch<R> S;
+    for (int i = 0; i < 6; ++i) S.a[i] = static_cast<R*>(scratch[i]);
+    hipLaunchKernelGGL((adj_vel_prep<R, M>), interior_grid(g), dim3(256), 0, s, g, F, B, md, gm, c, S, t0, t1);
+    // stress-shaped sweep: velocities = Q, level 0 = level 1 = bar T(t1)
+    FieldPtrs<R> A;
+    for (int i = 0; i < 3; ++i) A.f[i][0] = A.f[i][1] = S.a[i];
+    for (int f = 3; f < NF; ++f) A.f[f][0] = A.f[f][1] = B.f[f][t1];
+    Coefs<R> ca = {};
+    const int pr[3][2] = {{0, 1}, {1, 2}, {0, 2}};
+    for (int k = 0; k < M; ++k) {
+        for (int i = 0; i < 3; ++i) ca.cn[i][i][k] = -c.cv[i][i][k];
+        for (int sl = 0; sl < 3; ++sl) {
+            const int ia = pr[sl][0], ja = pr[sl][1];
+            ca.cs[sl][0][k] = -c.cv[ia][ja][k];   // V_ia reads T_s along ja
+            ca.cs[sl][1][k] = -c.cv[ja][ia][k];   // V_ja reads T_s along ia
+        }
+    }
+    if (hipError_t e = launch_stress(g, sizeof(R), OPESCI_STORE_NATIVE, false, &A, media, &ca, 0, 1, cfg, s))
+        return e;
+    hipLaunchKernelGGL((adj_rim_t<R, M>), rim_grid(g), dim3(256), 0, s, g, B, S, c, t1);
     return hipGetLastError();
 }
 
-#define OPESCI_ADJ_M(FN, R)                                                                  \
-    switch (g.m) {                                                                           \
-        case 1: return FN<R, 1>(g, fields, bar, media, grad, coefs, t0, t1, s);               \
-        case 2: return FN<R, 2>(g, fields, bar, media, grad, coefs, t0, t1, s);               \
-        case 3: return FN<R, 3>(g, fields, bar, media, grad, coefs, t0, t1, s);               \
-        case 4: return FN<R, 4>(g, fields, bar, media, grad, coefs, t0, t1, s);               \
-        default: return hipErrorInvalidValue;                                                \
+#define OPESCI_ADJ_M(FN, R)                                                                     \
+    switch (g.m) {                                                                              \
+        case 1: return FN<R, 1>(g, fields, bar, media, grad, coefs, scratch, cfg, t0, t1, s);    \
+        case 2: return FN<R, 2>(g, fields, bar, media, grad, coefs, scratch, cfg, t0, t1, s);    \
+        case 3: return FN<R, 3>(g, fields, bar, media, grad, coefs, scratch, cfg, t0, t1, s);    \
+        case 4: return FN<R, 4>(g, fields, bar, media, grad, coefs, scratch, cfg, t0, t1, s);    \
+        default: return hipErrorInvalidValue;                                                   \
     }
 
 hipError_t launch_adj_stress(const Geom& g, int real_bytes, void* fields, void* bar, const void* media, void* grad,
-                             const void* coefs, int t0, int t1, hipStream_t s) {
+                             const void* coefs, void* const* scratch, const InteriorLaunch& cfg, int t0, int t1,
+                             hipStream_t s) {
     if (real_bytes == 8) {
         OPESCI_ADJ_M(adj_stress_t, double)
     }
@@ -373,7 +655,8 @@ hipError_t launch_adj_stress(const Geom& g, int real_bytes, void* fields, void* 
 }
 
 hipError_t launch_adj_velocity(const Geom& g, int real_bytes, void* fields, void* bar, const void* media, void* grad,
-                               const void* coefs, int t0, int t1, hipStream_t s) {
+                               const void* coefs, void* const* scratch, const InteriorLaunch& cfg, int t0, int t1,
+                               hipStream_t s) {
     if (real_bytes == 8) {
         OPESCI_ADJ_M(adj_vel_t, double)
     }

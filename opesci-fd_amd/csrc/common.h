@@ -168,10 +168,14 @@ hipError_t launch_bc(const Geom& g, int real_bytes, int storage, void* fields, c
 hipError_t launch_bc_coef(const Geom& g, int real_bytes, const void* media, const BcDesc* d_desc,
                           const OpesciExprOp* d_code, void* pool, int first, int count, int maxpts, hipStream_t s);
 // adjoint (adjoint.hip); native storage, compute type given by real_bytes
+// scratch: six field-sized arrays, zero outside the interior (null selects
+// the simple gather kernels, as does cfg.variant == 1)
 hipError_t launch_adj_stress(const Geom& g, int real_bytes, void* fields, void* bar, const void* media, void* grad,
-                             const void* coefs, int t0, int t1, hipStream_t s);
+                             const void* coefs, void* const* scratch, const InteriorLaunch& cfg, int t0, int t1,
+                             hipStream_t s);
 hipError_t launch_adj_velocity(const Geom& g, int real_bytes, void* fields, void* bar, const void* media, void* grad,
-                               const void* coefs, int t0, int t1, hipStream_t s);
+                               const void* coefs, void* const* scratch, const InteriorLaunch& cfg, int t0, int t1,
+                               hipStream_t s);
 hipError_t launch_bc_logged(const Geom& g, int real_bytes, void* fields, const BcDesc* desc, const void* pool,
                             const BcLevel& lv, void* log, int t0, int t1, int undo, hipStream_t s);
 hipError_t launch_adj_bc(const Geom& g, int real_bytes, void* fields, void* bar, const BcDesc* desc, const void* pool,

@@ -55,6 +55,8 @@ struct OpesciContext {
     void* aptr[NF][2] = {};           // AdjPtrs<R> image
     void* gbase[OPESCI_M_COUNT] = {};
     void* gptr[OPESCI_M_COUNT] = {};  // MediaPtrs<R> image of the media gradients
+    void* sbase[6] = {};              // adjoint scratch (weighted adjoint fields)
+    void* sptr[6] = {};
     void* d_gpool = nullptr;
     void* d_log = nullptr;            // undo log of one step's boundary levels
     std::vector<int64_t> log_off[5];  // per phase and level
@@ -379,6 +381,8 @@ void destroy(OpesciContext* c) {
             if (c->abase[f][l]) (void)hipFree(c->abase[f][l]);
     for (int i = 0; i < OPESCI_M_COUNT; ++i)
         if (c->gbase[i]) (void)hipFree(c->gbase[i]);
+    for (int i = 0; i < 6; ++i)
+        if (c->sbase[i]) (void)hipFree(c->sbase[i]);
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -775,6 +779,11 @@ static int adj_alloc(OpesciContext* c, int ckpt, int seg) {
             if (int e = alloc_zero(c, &c->gbase[i], mbytes)) return e;
             c->gptr[i] = (char*)c->gbase[i] + (size_t)g.zoff * rb;
         }
+        // zero outside the interior for good: the adjoint kernels write interior cells only
+        for (int i = 0; i < 6; ++i) {
+            if (int e = alloc_zero(c, &c->sbase[i], c->field_bytes)) return e;
+            c->sptr[i] = (char*)c->sbase[i] + (size_t)g.zoff * rb;
+        }
         if (c->pool_elems > 0) {
             const size_t pb = ((size_t)c->pool_elems * rb + 15) & ~size_t(15);
             if (int e = alloc_zero(c, &c->d_gpool, pb)) return e;
@@ -856,12 +865,12 @@ static int step_adjoint(OpesciContext* c, int n) {
     OPESCI_HIP_TRY(launch_adj_receivers(c->g, c->real_bytes, c->fptr, c->aptr, t1, c->nrec, c->d_rec,
                                         c->d_obs + (size_t)n * c->nrec * 3, c->stream));
     if (int e = levels(3)) return e;
-    OPESCI_HIP_TRY(launch_adj_velocity(c->g, c->real_bytes, c->fptr, c->aptr, c->mptr, c->gptr, c->coefs(), t0, t1,
-                                       c->stream));
+    OPESCI_HIP_TRY(launch_adj_velocity(c->g, c->real_bytes, c->fptr, c->aptr, c->mptr, c->gptr, c->coefs(), c->sptr,
+                                       c->cfg, t0, t1, c->stream));
     if (int e = levels(2)) return e;
     if (int e = levels(4)) return e;
-    OPESCI_HIP_TRY(launch_adj_stress(c->g, c->real_bytes, c->fptr, c->aptr, c->mptr, c->gptr, c->coefs(), t0, t1,
-                                     c->stream));
+    OPESCI_HIP_TRY(launch_adj_stress(c->g, c->real_bytes, c->fptr, c->aptr, c->mptr, c->gptr, c->coefs(), c->sptr,
+                                     c->cfg, t0, t1, c->stream));
     return 0;
 }
 

@@ -194,51 +194,74 @@ class PlanVM(object):
         phase as listed (statements folded into the interior kernel first),
         launch level, then position.  Sequential order: the reference emission
         order (pieces of one split statement together)."""
-        p, F, R = self.p, self.F, self.dt_real
-        lv = (t0, t1)
+        for i in self.stmt_order(phases):
+            self.apply_stmt(i, (t0, t1))
+
+    def stmt_order(self, phases):
+        p = self.p
         st = p.stmts
         if isinstance(phases, int):
             phases = (phases,)
         idx = [i for i in range(len(st)) if int(st['phase'][i]) in phases]
         if self.order_by_sched:
             rank = {ph: r for r, ph in enumerate(phases)}
-            idx = sorted(idx, key=lambda i: (rank[int(st['phase'][i])], st['sched'][i], i))
-        else:
-            seq = getattr(p, 'stmt_seq', np.arange(len(st)))
-            idx = sorted(idx, key=lambda i: (seq[i], i))
-        for i in idx:
-            s = st[i]
-            ax = int(s['axis'])
-            oth = [a for a in range(3) if a != ax]
-            lo, hi = [int(v) for v in s['lo']], [int(v) for v in s['hi']]
-            n0, n1 = hi[0] - lo[0], hi[1] - lo[1]
-            if n0 <= 0 or n1 <= 0:
-                continue
-            acc = np.zeros((n0, n1), dtype=R)
-            for t in p.terms[s['term0']:s['term0'] + s['nterms']]:
-                sl = [None] * 3
-                sl[ax] = int(t['plane'])
-                for k, a in enumerate(oth):
-                    sl[a] = slice(lo[k] + int(t['off'][k]), hi[k] + int(t['off'][k]))
-                src = F[int(t['field']), lv[int(t['level'])]][tuple(sl)]
-                if t['prog'] >= 0:
-                    code = _code(p.bc_code, int(t['prog']), int(t['prog_len']))
+            return sorted(idx, key=lambda i: (rank[int(st['phase'][i])], st['sched'][i], i))
+        seq = getattr(p, 'stmt_seq', np.arange(len(st)))
+        return sorted(idx, key=lambda i: (seq[i], i))
 
-                    def media_at(mid, offs, lo=lo, hi=hi, oth=oth, ax=ax):
-                        msl = [None] * 3
-                        msl[ax] = int(offs[ax])
-                        for k, a in enumerate(oth):
-                            msl[a] = slice(lo[k] + int(offs[a]), hi[k] + int(offs[a]))
-                        return self.media[mid][tuple(msl)].astype(np.float64)
-                    c = np.asarray(eval_code(code, None, 0.0, media_at)) * float(t['coef'])
-                    acc = acc + c.astype(R) * src
-                else:
-                    acc = acc + R(t['coef']) * src
-            dsl = [None] * 3
-            dsl[ax] = int(s['plane'])
+    def stmt_geometry(self, i):
+        """(axis, in-plane axes, lo, hi) of statement i, or None when empty."""
+        s = self.p.stmts[i]
+        ax = int(s['axis'])
+        oth = [a for a in range(3) if a != ax]
+        lo, hi = [int(v) for v in s['lo']], [int(v) for v in s['hi']]
+        if hi[0] - lo[0] <= 0 or hi[1] - lo[1] <= 0:
+            return None
+        return ax, oth, lo, hi
+
+    def term_slices(self, i, t):
+        """Source slice of term t of statement i, and the media slicer for an
+        offset vector of its coefficient program."""
+        ax, oth, lo, hi = self.stmt_geometry(i)
+        sl = [None] * 3
+        sl[ax] = int(t['plane'])
+        for k, a in enumerate(oth):
+            sl[a] = slice(lo[k] + int(t['off'][k]), hi[k] + int(t['off'][k]))
+
+        def media_slice(offs):
+            msl = [None] * 3
+            msl[ax] = int(offs[ax])
             for k, a in enumerate(oth):
-                dsl[a] = slice(lo[k], hi[k])
-            F[int(s['field']), lv[int(s['level'])]][tuple(dsl)] = acc
+                msl[a] = slice(lo[k] + int(offs[a]), hi[k] + int(offs[a]))
+            return tuple(msl)
+        return tuple(sl), media_slice
+
+    def dst_slice(self, i):
+        ax, oth, lo, hi = self.stmt_geometry(i)
+        dsl = [None] * 3
+        dsl[ax] = int(self.p.stmts[i]['plane'])
+        for k, a in enumerate(oth):
+            dsl[a] = slice(lo[k], hi[k])
+        return tuple(dsl)
+
+    def apply_stmt(self, i, lv):
+        p, F, R = self.p, self.F, self.dt_real
+        if self.stmt_geometry(i) is None:
+            return
+        s = p.stmts[i]
+        ax, oth, lo, hi = self.stmt_geometry(i)
+        acc = np.zeros((hi[0] - lo[0], hi[1] - lo[1]), dtype=R)
+        for t in p.terms[s['term0']:s['term0'] + s['nterms']]:
+            sl, media_slice = self.term_slices(i, t)
+            src = F[int(t['field']), lv[int(t['level'])]][sl]
+            if t['prog'] >= 0:
+                code = _code(p.bc_code, int(t['prog']), int(t['prog_len']))
+                c = np.asarray(eval_code(code, None, 0.0, lambda mid, offs: self.media[mid][media_slice(offs)]
+                                         .astype(np.float64))) * float(t['coef'])
+                acc = acc + c.astype(R) * src
+            else:
+                acc = acc + R(t['coef']) * src
+        F[int(s['field']), lv[int(s['level'])]][self.dst_slice(i)] = acc
 
     def step(self):
         t0 = self.steps % 2

@@ -67,3 +67,36 @@ def batch_specs(count=64, seed=4, lo=48, hi=160, orders=(2, 4, 8)):
         sizes = tuple(int(v) for v in rng.integers(lo, hi + 1, size=3))
         out.append(dict(sizes=sizes, order=int(rng.choice(orders)), hetero=bool(i % 2)))
     return out
+
+
+def adjoint_model(N, order, steps, seed=5):
+    """Small inversion case: seeded smooth medium, fields at rest, FP64,
+    Ricker source (x100) in Txx/Tyy/Tzz at the centre, a receiver patch two
+    cells below the top face plus three interior receivers.  Returns the grid
+    and FP64 rho, lambda, mu."""
+    from opesci.sources import ricker
+    n = N - 1
+    dims = (N + order,) * 3
+    rho, vp, vs = smooth_random_media(dims, seed=seed, sigma=2.0)
+    dx = 1.0 / n
+    dt = 0.4 * 0.495 * dx / float(vp.max())
+    g = opesci.eigenwave3d((1.0, 1.0, 1.0), (n, n, n), dt, dt * (steps + 0.5), accuracy_order=[2, order, order, order],
+                           double=True, media_arrays=(rho, vp, vs), initial='zero', o_converge=False)
+    m = order // 2
+    c = m + n // 2
+    g.set_source(ricker(20.0, dt, steps, t0=6 * dt) * 100.0, index=(c, c, c))
+    rec = [(m + i, m + j, m + 2) for i in range(2, n - 1, 3) for j in range(2, n - 1, 3)]
+    rec += [(c + 2, c - 1, c + 3), (m + 1, c, c), (c, m + 1, c)]
+    g.set_receivers(index=rec)
+    rho64, vp64, vs64 = (a.astype(np.float64) for a in (rho, vp, vs))
+    lam = rho64 * (vp64 ** 2 - 2 * vs64 ** 2)
+    mu = rho64 * vs64 ** 2
+    return g, rho64, lam, mu
+
+
+def lambda_blob(lam, amp=0.1, width=2.5):
+    """lambda raised by `amp` in a Gaussian blob off the centre."""
+    d = np.array(lam.shape)
+    xx, yy, zz = np.meshgrid(*[np.arange(s) for s in d], indexing='ij')
+    blob = np.exp(-((xx - d[0] * 0.4) ** 2 + (yy - d[1] * 0.6) ** 2 + (zz - d[2] * 0.5) ** 2) / (2 * width ** 2))
+    return lam * (1 + amp * blob)
