@@ -20,7 +20,7 @@ from the 20,000 largest gradient entries of each parameter (rho, lambda and
 mu in turn).  Near the source the misfit is far from quadratic in one cell's
 value, so h = 1e-3 carries truncation error there; h = 1e-5 does not.
 
-  python adjoint_run.py [--size 256] [--steps 1000] [--every 32] [--fd 16] [--json out.json]
+  python adjoint_run.py [--size 256] [--steps 1000] [--every 32] [--fd 16] [--no-fp64] [--json out.json]
 """
 import argparse
 import json
@@ -81,6 +81,7 @@ def main(argv=None):
     ap.add_argument('--fd', type=int, default=16, help='finite-difference points')
     ap.add_argument('--fd-steps', type=float, nargs='+', default=[1e-3, 1e-5],
                     help='relative finite-difference steps')
+    ap.add_argument('--no-fp64', action='store_true', help='FP32 gradient only (no FP64 run, no differences)')
     ap.add_argument('--json', default=None)
     args = ap.parse_args(argv)
     N, order, steps = args.size, args.order, args.steps
@@ -125,6 +126,9 @@ def main(argv=None):
     result['fp32'] = dict(J=J32, gradient_s=secs32, device_gradient_s=dev32, forward_s=fwd,
                           gpts_per_s_forward=N ** 3 * steps / fwd)
 
+    if args.no_fp64:
+        print(json.dumps(result))
+        return
     # FP64 gradient and finite differences
     g64, _ = build(N, order, steps, (rho, vp, vs), True, args.f0, src, plane)
     ctx = Context(g64.lower())

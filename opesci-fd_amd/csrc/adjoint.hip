@@ -532,6 +532,18 @@ __global__ void adj_receiver_kernel(Geom g, FieldPtrs<R> F, AdjPtrs<R> B, int t1
 
 __global__ void set_step_kernel(int* step, int value) { *step = value; }
 
+// Txx, Tyy, Tzz of one cell at level t: read into buf (mode 0) or written
+// from it (mode 1); brackets the point source in the reverse sweep.
+template <typename R>
+__global__ void src_cells_kernel(FieldPtrs<R> F, int t, int64_t q, R* buf, int mode) {
+    const int f = (int)threadIdx.x;
+    if (f >= 3) return;
+    if (mode == 0)
+        buf[f] = F.f[3 + f][t][q];
+    else
+        F.f[3 + f][t][q] = buf[f];
+}
+
 // ------------------------------------------------------------------ launchers
 static dim3 grid_for(int64_t n) {
     const int64_t b = (n + 255) / 256;
@@ -731,6 +743,18 @@ hipError_t launch_adj_receivers(const Geom& g, int real_bytes, void* fields, voi
         hipLaunchKernelGGL((adj_receiver_kernel<float>), dim3((nrec + 255) / 256), dim3(256), 0, s, g,
                            *static_cast<const FieldPtrs<float>*>(fields), *static_cast<const AdjPtrs<float>*>(bar),
                            t1, nrec, rec, obs);
+    return hipGetLastError();
+}
+
+hipError_t launch_src_cells(const Geom& g, int real_bytes, void* fields, int t, int x, int y, int z, void* buf,
+                            int mode, hipStream_t s) {
+    const int64_t q = g.at(x, y, z);
+    if (real_bytes == 8)
+        hipLaunchKernelGGL((src_cells_kernel<double>), dim3(1), dim3(64), 0, s,
+                           *static_cast<const FieldPtrs<double>*>(fields), t, q, static_cast<double*>(buf), mode);
+    else
+        hipLaunchKernelGGL((src_cells_kernel<float>), dim3(1), dim3(64), 0, s,
+                           *static_cast<const FieldPtrs<float>*>(fields), t, q, static_cast<float*>(buf), mode);
     return hipGetLastError();
 }
 

@@ -186,6 +186,9 @@ hipError_t launch_adj_coef_media(const Geom& g, int real_bytes, const void* medi
 hipError_t launch_adj_receivers(const Geom& g, int real_bytes, void* fields, void* bar, int t1, int nrec,
                                 const int32_t* rec, const float* obs, hipStream_t s);
 hipError_t launch_set_step(int* step, int value, hipStream_t s);
+// mode 0: buf[0..2] = Txx, Tyy, Tzz at (x, y, z) of level t; mode 1: the reverse
+hipError_t launch_src_cells(const Geom& g, int real_bytes, void* fields, int t, int x, int y, int z, void* buf,
+                            int mode, hipStream_t s);
 
 hipError_t launch_initial(const Geom& g, int real_bytes, int storage, void* fields, const void* media,
                           const OpesciExprOp* d_sol, const int* sol_off, const int* sol_len, const double* spacing,

@@ -62,7 +62,11 @@ struct OpesciContext {
     std::vector<int64_t> log_off[5];  // per phase and level
     char* d_ckpt = nullptr;           // checkpoint states, 18 field levels each
     char* d_seg = nullptr;            // states of one checkpoint segment
-    int ckpt_cap = 0, seg_cap = 0;
+    size_t ckpt_cap = 0, seg_cap = 0;  // bytes
+    int64_t log_step = 0;             // undo-log elements of one step
+    int64_t log_cap = 0;              // undo-log steps allocated
+    int64_t srclog_cap = 0;           // source-cell log steps allocated
+    void* d_srclog = nullptr;         // Txx/Tyy/Tzz at the source before and after injection, per step
     float* d_obs = nullptr;
     OpesciExprOp* d_code = nullptr;
     OpesciExprOp* d_sol = nullptr;
@@ -375,7 +379,7 @@ void destroy(OpesciContext* c) {
         if (c->mbase[i]) (void)hipFree(c->mbase[i]);
     void* ptrs[] = {c->d_desc,  c->d_code,    c->d_sol,  c->d_partial, c->d_stage, c->d_rec,
                     c->d_trace, c->d_wavelet, c->d_step, c->d_pool,    c->d_dterms, c->d_gpool,
-                    c->d_log,   c->d_ckpt,    c->d_seg,  c->d_obs};
+                    c->d_log,   c->d_ckpt,    c->d_seg,  c->d_obs,     c->d_srclog};
     for (int f = 0; f < NF; ++f)
         for (int l = 0; l < 2; ++l)
             if (c->abase[f][l]) (void)hipFree(c->abase[f][l]);
@@ -765,7 +769,7 @@ int opesci_ctx_configure(OpesciContext* ctx, int32_t kernel_variant, int32_t til
 }
 
 // ---------------------------------------------------------------- adjoint
-static int adj_alloc(OpesciContext* c, int ckpt, int seg) {
+static int adj_alloc(OpesciContext* c, int ckpt, size_t seg_bytes, int log_steps) {
     const Geom& g = c->g;
     const size_t rb = c->real_bytes;
     if (!c->abase[0][0]) {
@@ -796,20 +800,32 @@ static int adj_alloc(OpesciContext* c, int ckpt, int seg) {
                 off += (int64_t)lv.count * lv.maxpts;
             }
         }
-        if (off > 0) OPESCI_HIP_TRY(hipMalloc(&c->d_log, (size_t)off * rb));
+        c->log_step = off;
+    }
+    if (c->log_step > 0 && log_steps > c->log_cap) {
+        if (c->d_log) OPESCI_HIP_TRY(hipFree(c->d_log));
+        c->d_log = nullptr;
+        OPESCI_HIP_TRY(hipMalloc(&c->d_log, (size_t)c->log_step * log_steps * rb));
+        c->log_cap = log_steps;
+    }
+    if (log_steps > c->srclog_cap) {
+        if (c->d_srclog) OPESCI_HIP_TRY(hipFree(c->d_srclog));
+        c->d_srclog = nullptr;
+        OPESCI_HIP_TRY(hipMalloc(&c->d_srclog, (size_t)6 * log_steps * rb));
+        c->srclog_cap = log_steps;
     }
     const size_t state = (size_t)2 * NF * c->field_bytes;
-    if (ckpt > c->ckpt_cap) {
+    if (state * ckpt > c->ckpt_cap) {
         if (c->d_ckpt) OPESCI_HIP_TRY(hipFree(c->d_ckpt));
         c->d_ckpt = nullptr;
         OPESCI_HIP_TRY(hipMalloc((void**)&c->d_ckpt, state * ckpt));
-        c->ckpt_cap = ckpt;
+        c->ckpt_cap = state * ckpt;
     }
-    if (seg > c->seg_cap) {
+    if (seg_bytes > c->seg_cap) {
         if (c->d_seg) OPESCI_HIP_TRY(hipFree(c->d_seg));
         c->d_seg = nullptr;
-        OPESCI_HIP_TRY(hipMalloc((void**)&c->d_seg, state * seg));
-        c->seg_cap = seg;
+        OPESCI_HIP_TRY(hipMalloc((void**)&c->d_seg, seg_bytes));
+        c->seg_cap = seg_bytes;
     }
     return 0;
 }
@@ -826,14 +842,13 @@ static int state_copy(OpesciContext* c, char* slot, bool save) {
 }
 
 // one forward step up to its velocity boundary levels, with every boundary
-// level's overwritten values logged
-static int step_logged(OpesciContext* c, int t0) {
+// level's overwritten values logged into `log` (one step's log)
+static int step_logged(OpesciContext* c, int t0, char* log) {
     const int t1 = 1 - t0;
     auto levels = [&](int ph) -> int {
         for (size_t L = 0; L < c->levels[ph].size(); ++L)
             OPESCI_HIP_TRY(launch_bc_logged(c->g, c->real_bytes, c->fptr, c->d_desc, c->d_pool, c->levels[ph][L],
-                                            (char*)c->d_log + c->log_off[ph][L] * c->real_bytes, t0, t1, 0,
-                                            c->stream));
+                                            log + c->log_off[ph][L] * c->real_bytes, t0, t1, 0, c->stream));
         return 0;
     };
     OPESCI_HIP_TRY(launch_stress(c->g, c->real_bytes, c->storage, c->het, c->fptr, c->mptr, c->coefs(), t0, t1,
@@ -848,30 +863,45 @@ static int step_logged(OpesciContext* c, int t0) {
     return levels(3);
 }
 
-// adjoint of step n, the fields holding that step's logged forward result
-static int step_adjoint(OpesciContext* c, int n) {
+// adjoint of step n; `fp` (a FieldPtrs image) holds that step's logged
+// forward result, `log` its undo log
+static int step_adjoint(OpesciContext* c, int n, void* fp, char* log) {
     const int t0 = n % 2, t1 = 1 - t0;
     auto levels = [&](int ph) -> int {
         for (int L = (int)c->levels[ph].size() - 1; L >= 0; --L) {
             const BcLevel& lv = c->levels[ph][L];
-            OPESCI_HIP_TRY(launch_adj_bc(c->g, c->real_bytes, c->fptr, c->aptr, c->d_desc, c->d_pool, c->d_gpool, lv,
-                                         t0, t1, c->stream));
-            OPESCI_HIP_TRY(launch_bc_logged(c->g, c->real_bytes, c->fptr, c->d_desc, c->d_pool, lv,
-                                            (char*)c->d_log + c->log_off[ph][L] * c->real_bytes, t0, t1, 1,
-                                            c->stream));
+            OPESCI_HIP_TRY(launch_adj_bc(c->g, c->real_bytes, fp, c->aptr, c->d_desc, c->d_pool, c->d_gpool, lv, t0,
+                                         t1, c->stream));
+            OPESCI_HIP_TRY(launch_bc_logged(c->g, c->real_bytes, fp, c->d_desc, c->d_pool, lv,
+                                            log + c->log_off[ph][L] * c->real_bytes, t0, t1, 1, c->stream));
         }
         return 0;
     };
-    OPESCI_HIP_TRY(launch_adj_receivers(c->g, c->real_bytes, c->fptr, c->aptr, t1, c->nrec, c->d_rec,
+    OPESCI_HIP_TRY(launch_adj_receivers(c->g, c->real_bytes, fp, c->aptr, t1, c->nrec, c->d_rec,
                                         c->d_obs + (size_t)n * c->nrec * 3, c->stream));
     if (int e = levels(3)) return e;
-    OPESCI_HIP_TRY(launch_adj_velocity(c->g, c->real_bytes, c->fptr, c->aptr, c->mptr, c->gptr, c->coefs(), c->sptr,
+    OPESCI_HIP_TRY(launch_adj_velocity(c->g, c->real_bytes, fp, c->aptr, c->mptr, c->gptr, c->coefs(), c->sptr,
                                        c->cfg, t0, t1, c->stream));
     if (int e = levels(2)) return e;
     if (int e = levels(4)) return e;
-    OPESCI_HIP_TRY(launch_adj_stress(c->g, c->real_bytes, c->fptr, c->aptr, c->mptr, c->gptr, c->coefs(), c->sptr,
+    OPESCI_HIP_TRY(launch_adj_stress(c->g, c->real_bytes, fp, c->aptr, c->mptr, c->gptr, c->coefs(), c->sptr,
                                      c->cfg, t0, t1, c->stream));
     return 0;
+}
+
+// copy level `lvl` of the nine fields to a slot of nine arrays
+static int level_save(OpesciContext* c, int lvl, char* slot) {
+    for (int f = 0; f < NF; ++f)
+        OPESCI_HIP_TRY(hipMemcpyAsync(slot + (size_t)f * c->field_bytes, c->base[f][lvl], c->field_bytes,
+                                      hipMemcpyDeviceToDevice, c->stream));
+    return 0;
+}
+
+// every stepping boundary statement writes level t1 (so a step leaves level t0 as it found it)
+static bool steps_keep_t0(const OpesciContext* c) {
+    for (const OpesciBcStmt& st : c->stmts)
+        if (st.phase >= 2 && st.level == 0) return false;
+    return true;
 }
 
 int opesci_ctx_gradient(OpesciContext* c, const float* observed, int64_t count, int32_t every, double* misfit) {
@@ -882,10 +912,18 @@ int opesci_ctx_gradient(OpesciContext* c, const float* observed, int64_t count, 
     const int N = c->desc.ntsteps;
     if (count != (int64_t)N * c->nrec * 3) return fail(OPESCI_EINVAL, "observed must hold ntsteps * nrec * 3 values");
     const int nck = (N + every - 1) / every;
-    if (int e = adj_alloc(c, std::max(nck, 1), every)) return e;
+    const size_t state = (size_t)2 * NF * c->field_bytes;
+    // Segment scheme.  keep_t0: the segment's forward pass logs every step and
+    // keeps its end state's level t1 (nine arrays); the reverse step runs on
+    // pointers into those slots (level t0 = the previous step's slot with the
+    // source cells set to their injected values, or the checkpoint).
+    // Otherwise: full states before each step, and the step recomputed with
+    // its log before its adjoint.
+    const bool keep_t0 = steps_keep_t0(c);
+    const size_t slot = keep_t0 ? (size_t)NF * c->field_bytes : state;
+    if (int e = adj_alloc(c, std::max(nck, 1), slot * every, keep_t0 ? every : 1)) return e;
     if (!c->d_obs) OPESCI_HIP_TRY(hipMalloc((void**)&c->d_obs, (size_t)count * sizeof(float)));
     OPESCI_HIP_TRY(hipMemcpyAsync(c->d_obs, observed, (size_t)count * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    const size_t state = (size_t)2 * NF * c->field_bytes;
     // forward with checkpoints
     if (int e = ctx_init(c)) return e;
     for (int n = 0; n < N; ++n) {
@@ -911,19 +949,59 @@ int opesci_ctx_gradient(OpesciContext* c, const float* observed, int64_t count, 
     for (int i = 0; i < OPESCI_M_COUNT; ++i) OPESCI_HIP_TRY(launch_zero(c->gbase[i], mbytes, c->stream));
     if (c->d_gpool)
         OPESCI_HIP_TRY(launch_zero(c->d_gpool, ((size_t)c->pool_elems * c->real_bytes + 15) & ~size_t(15), c->stream));
+    const size_t rb = c->real_bytes, zo = (size_t)c->g.zoff * rb;
+    const int* si = c->src_idx;
+    char* srclog = static_cast<char*>(c->d_srclog);
     for (int seg = nck - 1; seg >= 0; --seg) {
         const int k0 = seg * every, k1 = std::min(N, k0 + every);
-        if (int e = state_copy(c, c->d_ckpt + (size_t)seg * state, false)) return e;
+        char* ck = c->d_ckpt + (size_t)seg * state;
+        if (int e = state_copy(c, ck, false)) return e;
         OPESCI_HIP_TRY(launch_set_step(c->d_step, k0, c->stream));
+        if (!keep_t0) {
+            for (int n = k0; n < k1; ++n) {
+                if (int e = state_copy(c, c->d_seg + (size_t)(n - k0) * state, true)) return e;
+                if (int e = step_once(c, n % 2)) return e;
+            }
+            for (int n = k1 - 1; n >= k0; --n) {
+                if (int e = state_copy(c, c->d_seg + (size_t)(n - k0) * state, false)) return e;
+                if (int e = step_logged(c, n % 2, static_cast<char*>(c->d_log))) return e;
+                if (int e = step_adjoint(c, n, c->fptr, static_cast<char*>(c->d_log))) return e;
+            }
+            continue;
+        }
         for (int n = k0; n < k1; ++n) {
-            if (int e = state_copy(c, c->d_seg + (size_t)(n - k0) * state, true)) return e;
-            if (int e = step_once(c, n % 2)) return e;
+            const int t1 = 1 - n % 2;
+            char* log = static_cast<char*>(c->d_log) + (size_t)(n - k0) * c->log_step * rb;
+            char* sl = srclog + (size_t)6 * (n - k0) * rb;
+            if (int e = step_logged(c, n % 2, log)) return e;
+            if (int e = level_save(c, t1, c->d_seg + (size_t)(n - k0) * slot)) return e;
+            if (c->src_ntr)
+                OPESCI_HIP_TRY(launch_src_cells(c->g, c->real_bytes, c->fptr, t1, si[0], si[1], si[2], sl, 0,
+                                                c->stream));
+            // source and step counter only (the traces are already recorded)
+            OPESCI_HIP_TRY(launch_record_source(c->g, c->real_bytes, c->storage, c->fptr, t1, 0, c->d_rec,
+                                                c->d_trace, c->src_ntr, c->src_idx, c->d_wavelet, c->d_step,
+                                                c->desc.ntsteps, c->stream));
+            if (c->src_ntr)
+                OPESCI_HIP_TRY(launch_src_cells(c->g, c->real_bytes, c->fptr, t1, si[0], si[1], si[2], sl + 3 * rb, 0,
+                                                c->stream));
         }
         for (int n = k1 - 1; n >= k0; --n) {
-            if (int e = state_copy(c, c->d_seg + (size_t)(n - k0) * state, false)) return e;
-            OPESCI_HIP_TRY(launch_set_step(c->d_step, n, c->stream));
-            if (int e = step_logged(c, n % 2)) return e;
-            if (int e = step_adjoint(c, n)) return e;
+            const int t0 = n % 2, t1 = 1 - t0;
+            void* fp[NF][2];
+            for (int f = 0; f < NF; ++f) {
+                fp[f][t1] = c->d_seg + (size_t)(n - k0) * slot + (size_t)f * c->field_bytes + zo;
+                fp[f][t0] = n > k0 ? c->d_seg + (size_t)(n - 1 - k0) * slot + (size_t)f * c->field_bytes + zo
+                                   : ck + (size_t)(2 * f + t0) * c->field_bytes + zo;
+            }
+            char* prev = n > k0 ? srclog + (size_t)6 * (n - 1 - k0) * rb : nullptr;
+            if (n > k0 && c->src_ntr)
+                OPESCI_HIP_TRY(launch_src_cells(c->g, c->real_bytes, fp, t0, si[0], si[1], si[2], prev + 3 * rb, 1,
+                                                c->stream));
+            char* log = static_cast<char*>(c->d_log) + (size_t)(n - k0) * c->log_step * rb;
+            if (int e = step_adjoint(c, n, fp, log)) return e;
+            if (n > k0 && c->src_ntr)
+                OPESCI_HIP_TRY(launch_src_cells(c->g, c->real_bytes, fp, t0, si[0], si[1], si[2], prev, 1, c->stream));
         }
     }
     // media-dependent boundary coefficients -> media cells
