@@ -245,7 +245,8 @@ int opesci_ctx_kernel_times(OpesciContext* ctx, int32_t nsteps, double out_ms[3]
 int opesci_ctx_stats(OpesciContext* ctx, int32_t out[4]);
 
 /* Launch configuration of the interior kernels (kernel_variant 0 = sweep,
- * 1 = gather; tile_y 0 (default) | 4 | 8; x_chunk 0 = default), used by the
+ * 1 = gather; tile_y 0 (default: 16 for FP32 constant media, else 8) | 4 |
+ * 8 | 16; x_chunk 0 = default), used by the
  * autotuner.  Results do not depend on it (same arithmetic in every
  * configuration).  No reference counterpart. */
 int opesci_ctx_configure(OpesciContext* ctx, int32_t kernel_variant, int32_t tile_y, int32_t x_chunk);

@@ -37,10 +37,15 @@ struct bf16_t {
 
 template <typename S, typename R>
 struct Conv;
+// ld/st: ordinary accesses.  ldnt/stnt: non-temporal, for streams a sweep
+// kernel touches once (its own cell's previous value, every store), so the
+// caches keep the halo lines that neighbouring tiles re-read.
 template <>
 struct Conv<float, float> {
     __device__ __forceinline__ static float ld(const float* p) { return *p; }
     __device__ __forceinline__ static void st(float* p, float v) { *p = v; }
+    __device__ __forceinline__ static float ldnt(const float* p) { return __builtin_nontemporal_load(p); }
+    __device__ __forceinline__ static void stnt(float* p, float v) { __builtin_nontemporal_store(v, p); }
     __device__ __forceinline__ static float from(float v) { return v; }
     __device__ __forceinline__ static float to(float v) { return v; }
 };
@@ -48,6 +53,8 @@ template <>
 struct Conv<double, double> {
     __device__ __forceinline__ static double ld(const double* p) { return *p; }
     __device__ __forceinline__ static void st(double* p, double v) { *p = v; }
+    __device__ __forceinline__ static double ldnt(const double* p) { return __builtin_nontemporal_load(p); }
+    __device__ __forceinline__ static void stnt(double* p, double v) { __builtin_nontemporal_store(v, p); }
     __device__ __forceinline__ static double from(double v) { return v; }
     __device__ __forceinline__ static double to(double v) { return v; }
 };
@@ -57,6 +64,8 @@ struct Conv<_Float16, float> {
     __device__ __forceinline__ static _Float16 to(float v) { return (_Float16)v; }  // round to nearest even
     __device__ __forceinline__ static float ld(const _Float16* p) { return from(*p); }
     __device__ __forceinline__ static void st(_Float16* p, float v) { *p = to(v); }
+    __device__ __forceinline__ static float ldnt(const _Float16* p) { return from(__builtin_nontemporal_load(p)); }
+    __device__ __forceinline__ static void stnt(_Float16* p, float v) { __builtin_nontemporal_store(to(v), p); }
 };
 template <>
 struct Conv<bf16_t, float> {
@@ -74,6 +83,12 @@ struct Conv<bf16_t, float> {
     }
     __device__ __forceinline__ static float ld(const bf16_t* p) { return from(*p); }
     __device__ __forceinline__ static void st(bf16_t* p, float v) { *p = to(v); }
+    __device__ __forceinline__ static float ldnt(const bf16_t* p) {
+        bf16_t r;
+        r.bits = __builtin_nontemporal_load(&p->bits);
+        return from(r);
+    }
+    __device__ __forceinline__ static void stnt(bf16_t* p, float v) { __builtin_nontemporal_store(to(v).bits, &p->bits); }
 };
 
 // ---------------------------------------------------------------- geometry

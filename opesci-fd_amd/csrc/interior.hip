@@ -21,7 +21,9 @@
 //    current x plane of fields differentiated along y or z is staged with its
 //    halo in LDS, so every field is read from HBM once per sweep.  Blocks are
 //    remapped so that the tiles of one XCD form a contiguous slab and share
-//    halo lines in that XCD's L2.
+//    halo lines in that XCD's L2.  The cell's own previous value and every
+//    store are non-temporal (streamed once); only the stencil operands, whose
+//    halo lines neighbouring tiles re-read, are ordinary cached loads.
 //  * gather: one thread per (y,z) column, every stencil operand loaded from
 //    global memory (caches only); kept as the simple reference form.
 #include "common.h"
@@ -256,7 +258,7 @@ __global__ __launch_bounds__(64 * TY) void stress_sweep(Geom g, FieldPtrs<S> F, 
         nV = C::ld(V0 + px + (int64_t)M * g.sx + col);
         nW = C::ld(W0 + px + (int64_t)M * g.sx + col);
 #pragma unroll
-        for (int f = 0; f < 6; ++f) nT[f] = C::ld(F.f[3 + f][t0] + px + col);
+        for (int f = 0; f < 6; ++f) nT[f] = C::ldnt(F.f[3 + f][t0] + px + col);
         nhU.fetch(U0, px, hs);
         nhV.fetch(V0, px, hs);
         nhW.fetch(W0, px, hs);
@@ -317,7 +319,7 @@ __global__ __launch_bounds__(64 * TY) void stress_sweep(Geom g, FieldPtrs<S> F, 
 #pragma unroll
                 for (int k = 0; k < M; ++k) s = fmac(c.cn[i][2][k], dz[k], s);
                 acc = HET ? fmac(i == 2 ? l2m : lam, s, acc) : acc + s;
-                C::st(F.f[3 + i][t1] + q, acc);
+                C::stnt(F.f[3 + i][t1] + q, acc);
             }
             // Txy: dU/dy then dV/dx;  Tyz: dV/dz then dW/dy;  Txz: dU/dz then dW/dx
             R sxy = 0, syz = 0, sxz = 0;
@@ -334,13 +336,13 @@ __global__ __launch_bounds__(64 * TY) void stress_sweep(Geom g, FieldPtrs<S> F, 
 #pragma unroll
             for (int k = 0; k < M; ++k) sxz = fmac(c.cs[2][1][k], qW[M + k] - qW[M - 1 - k], sxz);
             if (HET) {
-                C::st(F.f[6][t1] + q, fmac(md.m[OPESCI_M_MU12][q], sxy, T0[3]));
-                C::st(F.f[7][t1] + q, fmac(md.m[OPESCI_M_MU23][q], syz, T0[4]));
-                C::st(F.f[8][t1] + q, fmac(md.m[OPESCI_M_MU13][q], sxz, T0[5]));
+                C::stnt(F.f[6][t1] + q, fmac(md.m[OPESCI_M_MU12][q], sxy, T0[3]));
+                C::stnt(F.f[7][t1] + q, fmac(md.m[OPESCI_M_MU23][q], syz, T0[4]));
+                C::stnt(F.f[8][t1] + q, fmac(md.m[OPESCI_M_MU13][q], sxz, T0[5]));
             } else {
-                C::st(F.f[6][t1] + q, T0[3] + sxy);
-                C::st(F.f[7][t1] + q, T0[4] + syz);
-                C::st(F.f[8][t1] + q, T0[5] + sxz);
+                C::stnt(F.f[6][t1] + q, T0[3] + sxy);
+                C::stnt(F.f[7][t1] + q, T0[4] + syz);
+                C::stnt(F.f[8][t1] + q, T0[5] + sxz);
             }
         }
     }
@@ -390,9 +392,9 @@ __global__ __launch_bounds__(64 * TY) void velocity_sweep(Geom g, FieldPtrs<S> F
         nYY = C::ld(TYY + px + col);
         nZZ = C::ld(TZZ + px + col);
         nYZ = C::ld(TYZ + px + col);
-        nU = C::ld(F.f[0][t0] + px + col);
-        nV = C::ld(F.f[1][t0] + px + col);
-        nW = C::ld(F.f[2][t0] + px + col);
+        nU = C::ldnt(F.f[0][t0] + px + col);
+        nV = C::ldnt(F.f[1][t0] + px + col);
+        nW = C::ldnt(F.f[2][t0] + px + col);
         hYY.fetch(TYY, px, hs);
         hXY.fetch(TXY, px, hs);
         hZZ.fetch(TZZ, px, hs);
@@ -452,13 +454,13 @@ __global__ __launch_bounds__(64 * TY) void velocity_sweep(Geom g, FieldPtrs<S> F
 #pragma unroll
             for (int k = 0; k < M; ++k) sw = fmac(c.cv[2][2][k], LZZ[me + 1 + k] - LZZ[me - k], sw);
             if (HET) {
-                C::st(F.f[0][t1] + q, fmac(md.m[OPESCI_M_BETA1][q], su, u0));
-                C::st(F.f[1][t1] + q, fmac(md.m[OPESCI_M_BETA2][q], sv, v0));
-                C::st(F.f[2][t1] + q, fmac(md.m[OPESCI_M_BETA3][q], sw, w0));
+                C::stnt(F.f[0][t1] + q, fmac(md.m[OPESCI_M_BETA1][q], su, u0));
+                C::stnt(F.f[1][t1] + q, fmac(md.m[OPESCI_M_BETA2][q], sv, v0));
+                C::stnt(F.f[2][t1] + q, fmac(md.m[OPESCI_M_BETA3][q], sw, w0));
             } else {
-                C::st(F.f[0][t1] + q, u0 + su);
-                C::st(F.f[1][t1] + q, v0 + sv);
-                C::st(F.f[2][t1] + q, w0 + sw);
+                C::stnt(F.f[0][t1] + q, u0 + su);
+                C::stnt(F.f[1][t1] + q, v0 + sv);
+                C::stnt(F.f[2][t1] + q, w0 + sw);
             }
         }
     }
@@ -478,10 +480,11 @@ static Tiling tiling(const Geom& g, const InteriorLaunch& cfg, int ty) {
     t.tyn = (ni[1] + ty - 1) / ty;
     int xc = cfg.x_chunk;
     if (xc <= 0) {
-        // about 512 blocks (two per CU): long x sweeps load fewer x halos and
-        // measured fastest at 256^3 (x chunks 8..128 timed); 8..64 planes
-        const int cols = t.tz * t.tyn;
-        xc = (ni[0] * cols + 511) / 512;
+        // about 512 blocks of 512 threads (256 of 1024): two per CU.  Long x
+        // sweeps load fewer x halos and measured fastest at 256^3 (x chunks
+        // 8..128 timed); 8..64 planes
+        const int cols = t.tz * t.tyn, target = ty >= 16 ? 256 : 512;
+        xc = (ni[0] * cols + target - 1) / target;
         xc = xc < 8 ? 8 : (xc > 64 ? 64 : xc);
     }
     t.xchunk = xc;
@@ -489,6 +492,14 @@ static Tiling tiling(const Geom& g, const InteriorLaunch& cfg, int ty) {
     t.nb = t.tz * t.tyn * t.tx;
     t.grid = ((t.nb + 7) / 8) * 8;
     return t;
+}
+
+// Rows per tile when the caller leaves it open: 16 for FP32 on constant media
+// (fewer y-halo rows; measured 2-3 % faster at 256^3), else 8 (the media
+// streams and the 16-bit conversions favour the smaller tile).
+template <typename R, typename S, bool HET>
+static int default_tile_y() {
+    return (!HET && sizeof(R) == 4 && sizeof(S) == 4) ? 16 : 8;
 }
 
 template <typename R, typename S, int M, bool HET, bool VEL>
@@ -507,7 +518,18 @@ static hipError_t run_interior(const Geom& g, void* fields, const void* media, c
                                t.tz, t.tyn, t.xchunk, t.nb);
         return hipGetLastError();
     }
-    if (cfg.tile_y == 4) {
+    const int tile_y = cfg.tile_y ? cfg.tile_y : default_tile_y<R, S, HET>();
+    if (tile_y == 16) {
+        const Tiling t = tiling<M>(g, cfg, 16);
+        if (VEL)
+            hipLaunchKernelGGL((velocity_sweep<R, S, M, 16, HET>), dim3(t.grid), dim3(1024), 0, s, g, F, md, c, t0, t1,
+                               t.tz, t.tyn, t.xchunk, t.nb);
+        else
+            hipLaunchKernelGGL((stress_sweep<R, S, M, 16, HET>), dim3(t.grid), dim3(1024), 0, s, g, F, md, c, t0, t1,
+                               t.tz, t.tyn, t.xchunk, t.nb);
+        return hipGetLastError();
+    }
+    if (tile_y == 4) {
         const Tiling t = tiling<M>(g, cfg, 4);
         if (VEL)
             hipLaunchKernelGGL((velocity_sweep<R, S, M, 4, HET>), dim3(t.grid), dim3(256), 0, s, g, F, md, c, t0, t1,

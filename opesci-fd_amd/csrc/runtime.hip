@@ -754,8 +754,9 @@ int opesci_ctx_stats(OpesciContext* ctx, int32_t out[4]) {
 
 int opesci_ctx_configure(OpesciContext* ctx, int32_t kernel_variant, int32_t tile_y, int32_t x_chunk) {
     if (!ctx) return fail(OPESCI_EINVAL, "null context");
-    if ((kernel_variant != 0 && kernel_variant != 1) || (tile_y != 0 && tile_y != 4 && tile_y != 8) || x_chunk < 0)
-        return fail(OPESCI_EINVAL, "kernel_variant 0|1, tile_y 0|4|8, x_chunk >= 0");
+    if ((kernel_variant != 0 && kernel_variant != 1) || (tile_y != 0 && tile_y != 4 && tile_y != 8 && tile_y != 16) ||
+        x_chunk < 0)
+        return fail(OPESCI_EINVAL, "kernel_variant 0|1, tile_y 0|4|8|16, x_chunk >= 0");
     ctx->cfg = InteriorLaunch{kernel_variant, tile_y, x_chunk};
     ctx->desc.kernel_variant = kernel_variant;
     ctx->desc.tile_y = tile_y;

@@ -213,7 +213,8 @@ def test_c1_second_order_fp64_64cube():
     (2, False, 'native', False, 8), (4, False, 'native', False, 8), (4, False, 'native', False, 4),
     (6, True, 'native', False, 8), (8, False, 'native', False, 8), (8, True, 'native', False, 4),
     (4, False, 'fp16', False, 8), (4, False, 'bf16', False, 8), (4, False, 'native', True, 8),
-    (8, True, 'native', True, 8)])
+    (8, True, 'native', True, 8), (4, False, 'native', False, 16), (8, False, 'native', True, 16),
+    (2, True, 'native', False, 16), (4, False, 'native', False, 0)])
 def test_sweep_equals_gather_bitwise(order, double, storage, het, tile_y):
     """The LDS/register-queue sweep kernels and the plain gather kernels keep
     the same accumulation order, so their results agree bit for bit (odd
