@@ -918,9 +918,10 @@ int opesci_ctx_gradient(OpesciContext* c, const float* observed, int64_t count, 
     // keeps its end state's level t1 (nine arrays); the reverse step runs on
     // pointers into those slots (level t0 = the previous step's slot with the
     // source cells set to their injected values, or the checkpoint).
-    // Otherwise: full states before each step, and the step recomputed with
-    // its log before its adjoint.
-    const bool keep_t0 = steps_keep_t0(c);
+    // Otherwise (and with the simple launch variant 1, which selects the
+    // reference forms throughout): full states before each step, and the step
+    // recomputed with its log before its adjoint.
+    const bool keep_t0 = steps_keep_t0(c) && c->cfg.variant != 1;
     const size_t slot = keep_t0 ? (size_t)NF * c->field_bytes : state;
     if (int e = adj_alloc(c, std::max(nck, 1), slot * every, keep_t0 ? every : 1)) return e;
     if (!c->d_obs) OPESCI_HIP_TRY(hipMalloc((void**)&c->d_obs, (size_t)count * sizeof(float)));

@@ -70,3 +70,18 @@ def test_dual_numbers_differentiate_a_free_surface_coefficient():
     assert np.allclose(v, lam / (lam + 2 * mu))
     assert np.allclose(d[(4, (0, 0, 0))], 2 * mu / (lam + 2 * mu) ** 2)
     assert np.allclose(d[(5, (0, 0, 0))], -2 * lam / (lam + 2 * mu) ** 2)
+
+
+def test_torch_effective_media_match_the_plan_interpreter():
+    """opesci.autograd.effective_media (torch, differentiable) forms the same
+    effective arrays as the plan interpreter builds from rho/Vp/Vs (which
+    tests/test_host.py pins against the oracle's read path), FP64."""
+    from opesci.autograd import effective_media
+    from planvm import PlanVM
+    g, rho, lam, mu = adjoint_model(9, 4, 2)
+    vm = PlanVM(g.lower())
+    eff = effective_media(torch.from_numpy(rho), torch.from_numpy(lam), torch.from_numpy(mu))
+    for k, mid in (('lambda', 4), ('mu', 5), ('beta1', 6), ('beta2', 7), ('beta3', 8), ('mu12', 9), ('mu13', 10),
+                   ('mu23', 11)):
+        a, b = eff[k].numpy(), vm.media[mid]
+        assert np.allclose(a, b, rtol=1e-6, atol=0), k   # the interpreter's media start from float32 rho/Vp/Vs

@@ -190,9 +190,10 @@ def test_gradient_matches_fp64_cpu_adjoint(order):
 
 @pytest.mark.parametrize("order", [2, 4, 8])
 def test_fast_interior_transposes_equal_simple_kernels(order):
-    """The sweep-based interior transposes (default) against the per-cell
-    gather kernels (launch variant 1), FP64, on a box of uneven sides:
-    every gradient entry to 1e-12 of the array's largest."""
+    """The sweep-based interior transposes and the slot-based segment
+    scheme (default) against the per-cell gather kernels and full-state
+    segments (launch variant 1), FP64, on a box of uneven sides: every
+    gradient entry to 1e-12 of the array's largest."""
     import opesci
     from opesci.media import smooth_random_media
     from opesci.sources import ricker
