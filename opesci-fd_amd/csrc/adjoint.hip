@@ -256,10 +256,12 @@ __global__ __launch_bounds__(256) void adj_vel_prep(Geom g, FieldPtrs<R> F, AdjP
     const int src[3][3] = {{3, 6, 8}, {6, 4, 7}, {8, 7, 5}};
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        const R b = B.f[i][t1][a];
-        S.a[i][a] = md.m[OPESCI_M_BETA1 + i][a] * b;
-        B.f[i][t0][a] += b;
-        B.f[i][t1][a] = 0;
+        // own-cell streams are touched once: non-temporal (forward-field
+        // stencil reads stay cached)
+        const R b = __builtin_nontemporal_load(B.f[i][t1] + a);
+        __builtin_nontemporal_store(__builtin_nontemporal_load(md.m[OPESCI_M_BETA1 + i] + a) * b, S.a[i] + a);
+        __builtin_nontemporal_store(__builtin_nontemporal_load(B.f[i][t0] + a) + b, B.f[i][t0] + a);
+        __builtin_nontemporal_store((R)0, B.f[i][t1] + a);
         R s = 0;
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
@@ -268,7 +270,8 @@ __global__ __launch_bounds__(256) void adj_vel_prep(Geom g, FieldPtrs<R> F, AdjP
 #pragma unroll
             for (int k = 0; k < M; ++k) s = fma(c.cv[i][j][k], t[(o + k) * st[j]] - t[(o - 1 - k) * st[j]], s);
         }
-        gm.m[OPESCI_M_BETA1 + i][a] += b * s;
+        R* gp = gm.m[OPESCI_M_BETA1 + i] + a;
+        __builtin_nontemporal_store(__builtin_nontemporal_load(gp) + b * s, gp);
     }
 }
 
@@ -281,20 +284,24 @@ __global__ __launch_bounds__(256) void adj_stress_prep(Geom g, FieldPtrs<R> F, A
     if (!interior_cell(g, x, y, z)) return;
     const int64_t a = g.at(x, y, z);
     const int64_t st[3] = {g.sx, g.sy, 1};
+    // own-cell streams are touched once: non-temporal (forward-field stencil
+    // reads stay cached)
     R bt[6];
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
-        bt[f] = B.f[3 + f][t1][a];
-        B.f[3 + f][t0][a] += bt[f];
-        B.f[3 + f][t1][a] = 0;
+        bt[f] = __builtin_nontemporal_load(B.f[3 + f][t1] + a);
+        __builtin_nontemporal_store(__builtin_nontemporal_load(B.f[3 + f][t0] + a) + bt[f], B.f[3 + f][t0] + a);
+        __builtin_nontemporal_store((R)0, B.f[3 + f][t1] + a);
     }
-    const R lam = md.m[OPESCI_M_LAMBDA][a], mu = md.m[OPESCI_M_MU][a];
+    const R lam = __builtin_nontemporal_load(md.m[OPESCI_M_LAMBDA] + a);
+    const R mu = __builtin_nontemporal_load(md.m[OPESCI_M_MU] + a);
     const R tr = bt[0] + bt[1] + bt[2];
     const int mid[3] = {OPESCI_M_MU12, OPESCI_M_MU23, OPESCI_M_MU13};
 #pragma unroll
-    for (int j = 0; j < 3; ++j) S.a[j][a] = lam * tr + (R)2 * mu * bt[j];
+    for (int j = 0; j < 3; ++j) __builtin_nontemporal_store(lam * tr + (R)2 * mu * bt[j], S.a[j] + a);
 #pragma unroll
-    for (int s = 0; s < 3; ++s) S.a[3 + s][a] = md.m[mid[s]][a] * bt[3 + s];
+    for (int s = 0; s < 3; ++s)
+        __builtin_nontemporal_store(__builtin_nontemporal_load(md.m[mid[s]] + a) * bt[3 + s], S.a[3 + s] + a);
     R glam = 0, gmu = 0;
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
@@ -308,8 +315,10 @@ __global__ __launch_bounds__(256) void adj_stress_prep(Geom g, FieldPtrs<R> F, A
             if (i == j) gmu += (R)2 * bt[i] * d;
         }
     }
-    gm.m[OPESCI_M_LAMBDA][a] += glam;
-    gm.m[OPESCI_M_MU][a] += gmu;
+    R* gl = gm.m[OPESCI_M_LAMBDA] + a;
+    R* gu = gm.m[OPESCI_M_MU] + a;
+    __builtin_nontemporal_store(__builtin_nontemporal_load(gl) + glam, gl);
+    __builtin_nontemporal_store(__builtin_nontemporal_load(gu) + gmu, gu);
     const int fa[3] = {0, 1, 0}, fb[3] = {1, 2, 2};   // differentiated velocities
     const int aa[3] = {1, 2, 2}, ab[3] = {0, 1, 0};   // their axes
 #pragma unroll
@@ -322,7 +331,8 @@ __global__ __launch_bounds__(256) void adj_stress_prep(Geom g, FieldPtrs<R> F, A
             v = fma(c.cs[s][0][k], va[(1 + k) * st[aa[s]]] - va[-k * st[aa[s]]], v);
             v = fma(c.cs[s][1][k], vb[(1 + k) * st[ab[s]]] - vb[-k * st[ab[s]]], v);
         }
-        gm.m[mid[s]][a] += bt[3 + s] * v;
+        R* gs = gm.m[mid[s]] + a;
+        __builtin_nontemporal_store(__builtin_nontemporal_load(gs) + bt[3 + s] * v, gs);
     }
 }
 
