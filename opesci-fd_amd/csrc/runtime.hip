@@ -912,6 +912,15 @@ int opesci_ctx_gradient(OpesciContext* c, const float* observed, int64_t count, 
     if (c->nrec <= 0) return fail(OPESCI_EINVAL, "gradients need receivers");
     const int N = c->desc.ntsteps;
     if (count != (int64_t)N * c->nrec * 3) return fail(OPESCI_EINVAL, "observed must hold ntsteps * nrec * 3 values");
+    // the fields end up holding a recomputed state (or, after a failure, a
+    // partial one): on every return, stepping needs opesci_ctx_init again
+    struct NeedsInit {
+        OpesciContext* c;
+        ~NeedsInit() {
+            c->initialised = false;
+            c->steps = 0;
+        }
+    } needs_init{c};
     const int nck = (N + every - 1) / every;
     const size_t state = (size_t)2 * NF * c->field_bytes;
     // Segment scheme.  keep_t0: the segment's forward pass logs every step and
@@ -1013,8 +1022,6 @@ int opesci_ctx_gradient(OpesciContext* c, const float* observed, int64_t count, 
                                                  c->d_dterms, c->d_gpool, lv, c->stream));
     OPESCI_HIP_TRY(hipStreamSynchronize(c->stream));
     OPESCI_HIP_TRY(hipGetLastError());
-    c->initialised = false;   // the fields hold a recomputed state; opesci_ctx_init before stepping
-    c->steps = 0;
     return 0;
 }
 

@@ -222,3 +222,30 @@ def test_fast_interior_transposes_equal_simple_kernels(order):
         scale = float(np.abs(ref).max())
         assert scale > 0, k
         assert float(np.abs(out[0][1][k] - ref).max()) <= 1e-12 * scale, k
+
+
+def test_gradient_errors_and_state():
+    """Refusals leave the context usable; a gradient leaves it needing init."""
+    from workloads import eigen_model
+    g, rho, lam, mu = build(10, 4, 6)
+    with Context(g.lower()) as ctx:
+        ctx.init()
+        ctx.run()
+        obs = ctx.receivers()[:, :, :3].copy()
+        with pytest.raises(RuntimeError, match='observed'):
+            ctx.gradient(obs[:-1])
+        with pytest.raises(RuntimeError):
+            ctx.gradient(obs, checkpoint_every=0)
+        ctx.init()
+        ctx.run()                                  # still steps after the refusals
+        ctx.gradient(obs, checkpoint_every=4)
+        with pytest.raises(RuntimeError, match='init'):
+            ctx.run()
+        ctx.init()
+        ctx.run()
+        assert np.array_equal(ctx.receivers()[:, :, :3], obs)   # the model is unchanged
+    plain, _ = eigen_model(10, 4, 4, double=True)    # constant media: no media arrays
+    with Context(plain.lower()) as ctx:
+        ctx.init()
+        with pytest.raises(RuntimeError, match='read path'):
+            ctx.gradient(np.zeros((4, 0, 3), np.float32))
