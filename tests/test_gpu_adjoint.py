@@ -127,7 +127,7 @@ def test_autograd_rho_lambda_mu_finite_differences():
         J.backward()
         grads = {'rho': r.grad.numpy(), 'lambda': l.grad.numpy(), 'mu': m.grad.numpy()}
         base = {'rho': rho, 'lambda': lam, 'mu': mu}
-        assert float(J) == pytest.approx(misfit(ctx, torch.from_numpy(rho), torch.from_numpy(lam),
+        assert float(J.detach()) == pytest.approx(misfit(ctx, torch.from_numpy(rho), torch.from_numpy(lam),
                                                  torch.from_numpy(mu), obs), rel=1e-12)
         rng = np.random.default_rng(16)
         gmax = max(float(np.abs(v).max()) for v in grads.values())
