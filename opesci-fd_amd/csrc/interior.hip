@@ -376,7 +376,7 @@ __global__ __launch_bounds__(64 * TY) void velocity_sweep(Geom g, FieldPtrs<S> F
     R qXX[2 * M], qXY[2 * M], qXZ[2 * M];
 #pragma unroll
     for (int i = 1; i < 2 * M; ++i) {
-        qXX[i] = C::ld(TXX + (int64_t)(xb - M + i) * g.sx + col);
+        qXX[i] = C::ldnt(TXX + (int64_t)(xb - M + i) * g.sx + col);
         qXY[i] = C::ld(TXY + (int64_t)(xb - M - 1 + i) * g.sx + col);
         qXZ[i] = C::ld(TXZ + (int64_t)(xb - M - 1 + i) * g.sx + col);
     }
@@ -386,7 +386,7 @@ __global__ __launch_bounds__(64 * TY) void velocity_sweep(Geom g, FieldPtrs<S> F
     Staged<R, M, TY, true, true> hYZ;
     auto fetch = [&](int x) {
         const int64_t px = (int64_t)x * g.sx;
-        nXX = C::ld(TXX + px + (int64_t)M * g.sx + col);
+        nXX = C::ldnt(TXX + px + (int64_t)M * g.sx + col);   // read along x only: streamed once
         nXY = C::ld(TXY + px + (int64_t)(M - 1) * g.sx + col);
         nXZ = C::ld(TXZ + px + (int64_t)(M - 1) * g.sx + col);
         nYY = C::ld(TYY + px + col);
