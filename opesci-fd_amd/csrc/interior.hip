@@ -301,8 +301,8 @@ __global__ __launch_bounds__(64 * TY) void stress_sweep(Geom g, FieldPtrs<S> F, 
             }
             R lam = 0, l2m = 0;
             if (HET) {
-                lam = md.m[OPESCI_M_LAMBDA][q];
-                l2m = lam + (R)2 * md.m[OPESCI_M_MU][q];
+                lam = __builtin_nontemporal_load(md.m[OPESCI_M_LAMBDA] + q);
+                l2m = lam + (R)2 * __builtin_nontemporal_load(md.m[OPESCI_M_MU] + q);
             }
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
@@ -336,9 +336,9 @@ __global__ __launch_bounds__(64 * TY) void stress_sweep(Geom g, FieldPtrs<S> F, 
 #pragma unroll
             for (int k = 0; k < M; ++k) sxz = fmac(c.cs[2][1][k], qW[M + k] - qW[M - 1 - k], sxz);
             if (HET) {
-                C::stnt(F.f[6][t1] + q, fmac(md.m[OPESCI_M_MU12][q], sxy, T0[3]));
-                C::stnt(F.f[7][t1] + q, fmac(md.m[OPESCI_M_MU23][q], syz, T0[4]));
-                C::stnt(F.f[8][t1] + q, fmac(md.m[OPESCI_M_MU13][q], sxz, T0[5]));
+                C::stnt(F.f[6][t1] + q, fmac(__builtin_nontemporal_load(md.m[OPESCI_M_MU12] + q), sxy, T0[3]));
+                C::stnt(F.f[7][t1] + q, fmac(__builtin_nontemporal_load(md.m[OPESCI_M_MU23] + q), syz, T0[4]));
+                C::stnt(F.f[8][t1] + q, fmac(__builtin_nontemporal_load(md.m[OPESCI_M_MU13] + q), sxz, T0[5]));
             } else {
                 C::stnt(F.f[6][t1] + q, T0[3] + sxy);
                 C::stnt(F.f[7][t1] + q, T0[4] + syz);
@@ -454,9 +454,9 @@ __global__ __launch_bounds__(64 * TY) void velocity_sweep(Geom g, FieldPtrs<S> F
 #pragma unroll
             for (int k = 0; k < M; ++k) sw = fmac(c.cv[2][2][k], LZZ[me + 1 + k] - LZZ[me - k], sw);
             if (HET) {
-                C::stnt(F.f[0][t1] + q, fmac(md.m[OPESCI_M_BETA1][q], su, u0));
-                C::stnt(F.f[1][t1] + q, fmac(md.m[OPESCI_M_BETA2][q], sv, v0));
-                C::stnt(F.f[2][t1] + q, fmac(md.m[OPESCI_M_BETA3][q], sw, w0));
+                C::stnt(F.f[0][t1] + q, fmac(__builtin_nontemporal_load(md.m[OPESCI_M_BETA1] + q), su, u0));
+                C::stnt(F.f[1][t1] + q, fmac(__builtin_nontemporal_load(md.m[OPESCI_M_BETA2] + q), sv, v0));
+                C::stnt(F.f[2][t1] + q, fmac(__builtin_nontemporal_load(md.m[OPESCI_M_BETA3] + q), sw, w0));
             } else {
                 C::stnt(F.f[0][t1] + q, u0 + su);
                 C::stnt(F.f[1][t1] + q, v0 + sv);
