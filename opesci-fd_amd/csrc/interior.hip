@@ -466,17 +466,405 @@ __global__ __launch_bounds__(64 * TY) void velocity_sweep(Geom g, FieldPtrs<S> F
     }
 }
 
+// ------------------------------------------------------------------ pair sweeps (16-bit storage)
+// The same sweeps for FP16/BF16 fields with two z points per lane: a tile is
+// 128 (z) x TY (y), lane lz owns z0 + 2 lz and z0 + 2 lz + 1, and every
+// own-column value and y-halo row is moved as one 32-bit pair (tiles start
+// at z = m, which the layout aligns to 128 bytes, so pairs are aligned; the
+// row padding holds the partner of the last point).  Per point the
+// arithmetic is the single-point sweeps' and the gather kernels'.
+template <int M, int TY>
+struct Plane2 {
+    static constexpr int W = 128 + 2 * M;
+    static constexpr int H = TY + 2 * M;
+    static constexpr int NT = 64 * TY;
+    static constexpr int NY = 2 * M * 64;   // y-halo pairs
+    static constexpr int NZ = TY * 2 * M;   // z-halo cells
+    static constexpr int JY = (NY + NT - 1) / NT;
+    static_assert(NZ <= NT, "z halo larger than the block");
+};
+
+// first z of the aligned pair holding z (z - m even), kept inside the row
+__device__ __forceinline__ int pair_start(const Geom& g, int z) {
+    z = min(max(z, 0), g.nz - 1);
+    return z - ((z - g.m) & 1);
+}
+
+template <int M, int TY>
+struct HaloSlots2 {
+    using P = Plane2<M, TY>;
+    int ly[P::JY], gy[P::JY];
+    int lz, gz;
+    __device__ void init(const Geom& g, int y0, int z0) {
+        const int t = threadIdx.x;
+#pragma unroll
+        for (int j = 0; j < P::JY; ++j) {
+            const int h = t + j * P::NT;
+            if (h < P::NY) {
+                const int r = h / 64, row = r < M ? r : r + TY, cl = M + 2 * (h % 64);
+                const int yy = min(max(y0 - M + row, 0), g.ny - 1), zz = pair_start(g, z0 - M + cl);
+                ly[j] = row * P::W + cl;
+                gy[j] = yy * (int)g.sy + zz;
+            } else {
+                ly[j] = -1;
+                gy[j] = 0;
+            }
+        }
+        if (t < P::NZ) {
+            const int row = M + t / (2 * M), c = t % (2 * M), cl = c < M ? c : 128 + c;
+            const int yy = min(max(y0 - M + row, 0), g.ny - 1), zz = min(max(z0 - M + cl, 0), g.nz - 1);
+            lz = row * P::W + cl;
+            gz = yy * (int)g.sy + zz;
+        } else {
+            lz = -1;
+            gz = 0;
+        }
+    }
+};
+
+template <int M, int TY, bool HY, bool HZ>
+struct Staged2 {
+    using P = Plane2<M, TY>;
+    float y0[HY ? P::JY : 1], y1[HY ? P::JY : 1];
+    float z;
+    template <typename S>
+    __device__ __forceinline__ void fetch(const S* base, int64_t px, const HaloSlots2<M, TY>& hs) {
+        if (HY) {
+#pragma unroll
+            for (int j = 0; j < P::JY; ++j)
+                if (hs.ly[j] >= 0) ld2(base + px + hs.gy[j], y0[j], y1[j]);
+        }
+        if (HZ && hs.lz >= 0) z = Conv<S, float>::ld(base + px + hs.gz);
+    }
+    __device__ __forceinline__ void put(float* lds, const HaloSlots2<M, TY>& hs) const {
+        if (HY) {
+#pragma unroll
+            for (int j = 0; j < P::JY; ++j)
+                if (hs.ly[j] >= 0) {
+                    lds[hs.ly[j]] = y0[j];
+                    lds[hs.ly[j] + 1] = y1[j];
+                }
+        }
+        if (HZ && hs.lz >= 0) lds[hs.lz] = z;
+    }
+};
+
+// FP32 media of a lane's two points as one 8-byte load (the media use the
+// fields' element layout, so z - m even is 8-byte aligned for them)
+typedef float mpair_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ mpair_t media2(const float* p) {
+    return __builtin_nontemporal_load(reinterpret_cast<const mpair_t*>(p));
+}
+
+// one or two of a lane's points, as one pair or a single value
+template <typename S>
+__device__ __forceinline__ void store_points(S* p, float a, float b, bool act0, bool act1) {
+    if (act1)
+        stnt2(p, a, b);
+    else if (act0)
+        Conv<S, float>::stnt(p, a);
+}
+
+template <typename S, int M, int TY, bool HET>
+__global__ __launch_bounds__(64 * TY) void stress_sweep2(Geom g, FieldPtrs<S> F, MediaPtrs<float> md,
+                                                         Coefs<float> c, int t0, int t1, int tz, int ty, int xchunk,
+                                                         int nblocks) {
+    using R = float;
+    using P = Plane2<M, TY>;
+    __shared__ R sU[2][P::H * P::W], sV[2][P::H * P::W], sW[2][P::H * P::W];
+    const int l = logical_block(nblocks);
+    if (l < 0) return;
+    const int bz = l % tz, by = (l / tz) % ty, bx = l / (tz * ty);
+    const int lz = threadIdx.x & 63, ly = threadIdx.x >> 6;
+    const int z0 = M + bz * 128, y0 = M + by * TY;
+    const int z = z0 + 2 * lz, y = y0 + ly;
+    const bool act0 = z < g.nz - M && y < g.ny - M, act1 = z + 1 < g.nz - M && y < g.ny - M;
+    const int64_t col = (int64_t)min(y, g.ny - 1) * g.sy + pair_start(g, z);
+    const int xb = M + bx * xchunk, xe = min(xb + xchunk, g.nx - M);
+    const S* U0 = F.f[0][t0];
+    const S* V0 = F.f[1][t0];
+    const S* W0 = F.f[2][t0];
+    HaloSlots2<M, TY> hs;
+    hs.init(g, y0, z0);
+    // per point v: qU[v][i] = U(x+i-M), qV[v][i] = V(x+i-M+1), qW[v][i] = W(x+i-M+1)
+    R qU[2][2 * M], qV[2][2 * M], qW[2][2 * M];
+#pragma unroll
+    for (int i = 1; i < 2 * M; ++i) {
+        ld2(U0 + (int64_t)(xb - M - 1 + i) * g.sx + col, qU[0][i], qU[1][i]);
+        ld2(V0 + (int64_t)(xb - M + i) * g.sx + col, qV[0][i], qV[1][i]);
+        ld2(W0 + (int64_t)(xb - M + i) * g.sx + col, qW[0][i], qW[1][i]);
+    }
+    R nU[2], nV[2], nW[2], nT[6][2];
+    Staged2<M, TY, true, true> nhU, nhV, nhW;
+    auto fetch = [&](int x) {
+        const int64_t px = (int64_t)x * g.sx;
+        ld2(U0 + px + (int64_t)(M - 1) * g.sx + col, nU[0], nU[1]);
+        ld2(V0 + px + (int64_t)M * g.sx + col, nV[0], nV[1]);
+        ld2(W0 + px + (int64_t)M * g.sx + col, nW[0], nW[1]);
+#pragma unroll
+        for (int f = 0; f < 6; ++f) ldnt2(F.f[3 + f][t0] + px + col, nT[f][0], nT[f][1]);
+        nhU.fetch(U0, px, hs);
+        nhV.fetch(V0, px, hs);
+        nhW.fetch(W0, px, hs);
+    };
+    fetch(xb);
+    for (int x = xb; x < xe; ++x) {
+        const int b = (x - xb) & 1;
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+#pragma unroll
+            for (int i = 0; i < 2 * M - 1; ++i) {
+                qU[v][i] = qU[v][i + 1];
+                qV[v][i] = qV[v][i + 1];
+                qW[v][i] = qW[v][i + 1];
+            }
+            qU[v][2 * M - 1] = nU[v];
+            qV[v][2 * M - 1] = nV[v];
+            qW[v][2 * M - 1] = nW[v];
+        }
+        R T0[6][2];
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+            T0[f][0] = nT[f][0];
+            T0[f][1] = nT[f][1];
+        }
+        R* LU = sU[b];
+        R* LV = sV[b];
+        R* LW = sW[b];
+        const int me0 = (ly + M) * P::W + 2 * lz + M;
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+            LU[me0 + v] = qU[v][M];
+            LV[me0 + v] = qV[v][M - 1];
+            LW[me0 + v] = qW[v][M - 1];
+        }
+        nhU.put(LU, hs);
+        nhV.put(LV, hs);
+        nhW.put(LW, hs);
+        if (x + 1 < xe) fetch(x + 1);
+        __syncthreads();
+        if (act0) {
+            const int64_t q = (int64_t)x * g.sx + col;
+            R out[6][2];
+            mpair_t mlam, mmu, m12, m23, m13;
+            if (HET) {
+                mlam = media2(md.m[OPESCI_M_LAMBDA] + q);
+                mmu = media2(md.m[OPESCI_M_MU] + q);
+                m12 = media2(md.m[OPESCI_M_MU12] + q);
+                m23 = media2(md.m[OPESCI_M_MU23] + q);
+                m13 = media2(md.m[OPESCI_M_MU13] + q);
+            }
+#pragma unroll
+            for (int v = 0; v < 2; ++v) {
+                const int me = me0 + v;
+                R dx[M], dy[M], dz[M];
+#pragma unroll
+                for (int k = 0; k < M; ++k) {
+                    dx[k] = qU[v][M + k] - qU[v][M - 1 - k];
+                    dy[k] = LV[me + k * P::W] - LV[me - (1 + k) * P::W];
+                    dz[k] = LW[me + k] - LW[me - 1 - k];
+                }
+                R lam = 0, l2m = 0;
+                if (HET) {
+                    lam = mlam[v];
+                    l2m = lam + (R)2 * mmu[v];
+                }
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    R acc = T0[i][v];
+                    R s = 0;
+#pragma unroll
+                    for (int k = 0; k < M; ++k) s = fmac(c.cn[i][0][k], dx[k], s);
+                    acc = HET ? fmac(i == 0 ? l2m : lam, s, acc) : acc + s;
+                    s = 0;
+#pragma unroll
+                    for (int k = 0; k < M; ++k) s = fmac(c.cn[i][1][k], dy[k], s);
+                    acc = HET ? fmac(i == 1 ? l2m : lam, s, acc) : acc + s;
+                    s = 0;
+#pragma unroll
+                    for (int k = 0; k < M; ++k) s = fmac(c.cn[i][2][k], dz[k], s);
+                    acc = HET ? fmac(i == 2 ? l2m : lam, s, acc) : acc + s;
+                    out[i][v] = acc;
+                }
+                R sxy = 0, syz = 0, sxz = 0;
+#pragma unroll
+                for (int k = 0; k < M; ++k) sxy = fmac(c.cs[0][0][k], LU[me + (1 + k) * P::W] - LU[me - k * P::W], sxy);
+#pragma unroll
+                for (int k = 0; k < M; ++k) sxy = fmac(c.cs[0][1][k], qV[v][M + k] - qV[v][M - 1 - k], sxy);
+#pragma unroll
+                for (int k = 0; k < M; ++k) syz = fmac(c.cs[1][0][k], LV[me + 1 + k] - LV[me - k], syz);
+#pragma unroll
+                for (int k = 0; k < M; ++k) syz = fmac(c.cs[1][1][k], LW[me + (1 + k) * P::W] - LW[me - k * P::W], syz);
+#pragma unroll
+                for (int k = 0; k < M; ++k) sxz = fmac(c.cs[2][0][k], LU[me + 1 + k] - LU[me - k], sxz);
+#pragma unroll
+                for (int k = 0; k < M; ++k) sxz = fmac(c.cs[2][1][k], qW[v][M + k] - qW[v][M - 1 - k], sxz);
+                if (HET) {
+                    out[3][v] = fmac(m12[v], sxy, T0[3][v]);
+                    out[4][v] = fmac(m23[v], syz, T0[4][v]);
+                    out[5][v] = fmac(m13[v], sxz, T0[5][v]);
+                } else {
+                    out[3][v] = T0[3][v] + sxy;
+                    out[4][v] = T0[4][v] + syz;
+                    out[5][v] = T0[5][v] + sxz;
+                }
+            }
+#pragma unroll
+            for (int f = 0; f < 6; ++f) store_points(F.f[3 + f][t1] + q, out[f][0], out[f][1], act0, act1);
+        }
+    }
+}
+
+template <typename S, int M, int TY, bool HET>
+__global__ __launch_bounds__(64 * TY) void velocity_sweep2(Geom g, FieldPtrs<S> F, MediaPtrs<float> md,
+                                                           Coefs<float> c, int t0, int t1, int tz, int ty, int xchunk,
+                                                           int nblocks) {
+    using R = float;
+    using P = Plane2<M, TY>;
+    __shared__ R sYY[2][P::H * P::W], sZZ[2][P::H * P::W], sXY[2][P::H * P::W], sYZ[2][P::H * P::W],
+        sXZ[2][P::H * P::W];
+    const int l = logical_block(nblocks);
+    if (l < 0) return;
+    const int bz = l % tz, by = (l / tz) % ty, bx = l / (tz * ty);
+    const int lz = threadIdx.x & 63, ly = threadIdx.x >> 6;
+    const int z0 = M + bz * 128, y0 = M + by * TY;
+    const int z = z0 + 2 * lz, y = y0 + ly;
+    const bool act0 = z < g.nz - M && y < g.ny - M, act1 = z + 1 < g.nz - M && y < g.ny - M;
+    const int64_t col = (int64_t)min(y, g.ny - 1) * g.sy + pair_start(g, z);
+    const int xb = M + bx * xchunk, xe = min(xb + xchunk, g.nx - M);
+    const S* TXX = F.f[3][t1];
+    const S* TYY = F.f[4][t1];
+    const S* TZZ = F.f[5][t1];
+    const S* TXY = F.f[6][t1];
+    const S* TYZ = F.f[7][t1];
+    const S* TXZ = F.f[8][t1];
+    HaloSlots2<M, TY> hs;
+    hs.init(g, y0, z0);
+    // per point v: qXX[v][i] = Txx(x+i-M+1), qXY[v][i] = Txy(x+i-M), qXZ[v][i] = Txz(x+i-M)
+    R qXX[2][2 * M], qXY[2][2 * M], qXZ[2][2 * M];
+#pragma unroll
+    for (int i = 1; i < 2 * M; ++i) {
+        ldnt2(TXX + (int64_t)(xb - M + i) * g.sx + col, qXX[0][i], qXX[1][i]);
+        ld2(TXY + (int64_t)(xb - M - 1 + i) * g.sx + col, qXY[0][i], qXY[1][i]);
+        ld2(TXZ + (int64_t)(xb - M - 1 + i) * g.sx + col, qXZ[0][i], qXZ[1][i]);
+    }
+    R nXX[2], nXY[2], nXZ[2], nYY[2], nZZ[2], nYZ[2], nU[2], nV[2], nW[2];
+    Staged2<M, TY, true, false> hYY, hXY;
+    Staged2<M, TY, false, true> hZZ, hXZ;
+    Staged2<M, TY, true, true> hYZ;
+    auto fetch = [&](int x) {
+        const int64_t px = (int64_t)x * g.sx;
+        ldnt2(TXX + px + (int64_t)M * g.sx + col, nXX[0], nXX[1]);
+        ld2(TXY + px + (int64_t)(M - 1) * g.sx + col, nXY[0], nXY[1]);
+        ld2(TXZ + px + (int64_t)(M - 1) * g.sx + col, nXZ[0], nXZ[1]);
+        ld2(TYY + px + col, nYY[0], nYY[1]);
+        ld2(TZZ + px + col, nZZ[0], nZZ[1]);
+        ld2(TYZ + px + col, nYZ[0], nYZ[1]);
+        ldnt2(F.f[0][t0] + px + col, nU[0], nU[1]);
+        ldnt2(F.f[1][t0] + px + col, nV[0], nV[1]);
+        ldnt2(F.f[2][t0] + px + col, nW[0], nW[1]);
+        hYY.fetch(TYY, px, hs);
+        hXY.fetch(TXY, px, hs);
+        hZZ.fetch(TZZ, px, hs);
+        hXZ.fetch(TXZ, px, hs);
+        hYZ.fetch(TYZ, px, hs);
+    };
+    fetch(xb);
+    for (int x = xb; x < xe; ++x) {
+        const int b = (x - xb) & 1;
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+#pragma unroll
+            for (int i = 0; i < 2 * M - 1; ++i) {
+                qXX[v][i] = qXX[v][i + 1];
+                qXY[v][i] = qXY[v][i + 1];
+                qXZ[v][i] = qXZ[v][i + 1];
+            }
+            qXX[v][2 * M - 1] = nXX[v];
+            qXY[v][2 * M - 1] = nXY[v];
+            qXZ[v][2 * M - 1] = nXZ[v];
+        }
+        const R u0[2] = {nU[0], nU[1]}, v0[2] = {nV[0], nV[1]}, w0[2] = {nW[0], nW[1]};
+        R* LYY = sYY[b];
+        R* LZZ = sZZ[b];
+        R* LXY = sXY[b];
+        R* LYZ = sYZ[b];
+        R* LXZ = sXZ[b];
+        const int me0 = (ly + M) * P::W + 2 * lz + M;
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+            LYY[me0 + v] = nYY[v];
+            LZZ[me0 + v] = nZZ[v];
+            LYZ[me0 + v] = nYZ[v];
+            LXY[me0 + v] = qXY[v][M];
+            LXZ[me0 + v] = qXZ[v][M];
+        }
+        hYY.put(LYY, hs);
+        hXY.put(LXY, hs);
+        hZZ.put(LZZ, hs);
+        hXZ.put(LXZ, hs);
+        hYZ.put(LYZ, hs);
+        if (x + 1 < xe) fetch(x + 1);
+        __syncthreads();
+        if (act0) {
+            const int64_t q = (int64_t)x * g.sx + col;
+            R ou[2], ov[2], ow[2];
+            mpair_t b1, b2, b3;
+            if (HET) {
+                b1 = media2(md.m[OPESCI_M_BETA1] + q);
+                b2 = media2(md.m[OPESCI_M_BETA2] + q);
+                b3 = media2(md.m[OPESCI_M_BETA3] + q);
+            }
+#pragma unroll
+            for (int v = 0; v < 2; ++v) {
+                const int me = me0 + v;
+                R su = 0, sv = 0, sw = 0;
+#pragma unroll
+                for (int k = 0; k < M; ++k) su = fmac(c.cv[0][0][k], qXX[v][M + k] - qXX[v][M - 1 - k], su);
+#pragma unroll
+                for (int k = 0; k < M; ++k) su = fmac(c.cv[0][1][k], LXY[me + k * P::W] - LXY[me - (1 + k) * P::W], su);
+#pragma unroll
+                for (int k = 0; k < M; ++k) su = fmac(c.cv[0][2][k], LXZ[me + k] - LXZ[me - 1 - k], su);
+#pragma unroll
+                for (int k = 0; k < M; ++k) sv = fmac(c.cv[1][0][k], qXY[v][M + k] - qXY[v][M - 1 - k], sv);
+#pragma unroll
+                for (int k = 0; k < M; ++k) sv = fmac(c.cv[1][1][k], LYY[me + (1 + k) * P::W] - LYY[me - k * P::W], sv);
+#pragma unroll
+                for (int k = 0; k < M; ++k) sv = fmac(c.cv[1][2][k], LYZ[me + k] - LYZ[me - 1 - k], sv);
+#pragma unroll
+                for (int k = 0; k < M; ++k) sw = fmac(c.cv[2][0][k], qXZ[v][M + k] - qXZ[v][M - 1 - k], sw);
+#pragma unroll
+                for (int k = 0; k < M; ++k) sw = fmac(c.cv[2][1][k], LYZ[me + k * P::W] - LYZ[me - (1 + k) * P::W], sw);
+#pragma unroll
+                for (int k = 0; k < M; ++k) sw = fmac(c.cv[2][2][k], LZZ[me + 1 + k] - LZZ[me - k], sw);
+                if (HET) {
+                    ou[v] = fmac(b1[v], su, u0[v]);
+                    ov[v] = fmac(b2[v], sv, v0[v]);
+                    ow[v] = fmac(b3[v], sw, w0[v]);
+                } else {
+                    ou[v] = u0[v] + su;
+                    ov[v] = v0[v] + sv;
+                    ow[v] = w0[v] + sw;
+                }
+            }
+            store_points(F.f[0][t1] + q, ou[0], ou[1], act0, act1);
+            store_points(F.f[1][t1] + q, ov[0], ov[1], act0, act1);
+            store_points(F.f[2][t1] + q, ow[0], ow[1], act0, act1);
+        }
+    }
+}
+
 // ------------------------------------------------------------------ dispatch
 struct Tiling {
     int ty, tz, tyn, tx, xchunk, nb, grid;
 };
 
 template <int M>
-static Tiling tiling(const Geom& g, const InteriorLaunch& cfg, int ty) {
+static Tiling tiling(const Geom& g, const InteriorLaunch& cfg, int ty, int tzw = 64) {
     Tiling t;
     const int ni[3] = {g.nx - 2 * M, g.ny - 2 * M, g.nz - 2 * M};
     t.ty = ty;
-    t.tz = (ni[2] + 63) / 64;
+    t.tz = (ni[2] + tzw - 1) / tzw;
     t.tyn = (ni[1] + ty - 1) / ty;
     int xc = cfg.x_chunk;
     if (xc <= 0) {
@@ -519,6 +907,25 @@ static hipError_t run_interior(const Geom& g, void* fields, const void* media, c
         return hipGetLastError();
     }
     const int tile_y = cfg.tile_y ? cfg.tile_y : default_tile_y<R, S, HET>();
+    if constexpr (sizeof(S) == 2) {
+        // 16-bit storage: the pair sweeps (128-wide z tiles; 4 or 8 rows, as
+        // 1024-thread blocks would spill the doubled register queues)
+        const int ty = tile_y == 4 ? 4 : 8;
+        const Tiling t = tiling<M>(g, cfg, ty, 128);
+#define OPESCI_PAIR(TYV)                                                                                        \
+    if (ty == TYV) {                                                                                            \
+        if (VEL)                                                                                                \
+            hipLaunchKernelGGL((velocity_sweep2<S, M, TYV, HET>), dim3(t.grid), dim3(64 * TYV), 0, s, g, F, md, c, \
+                               t0, t1, t.tz, t.tyn, t.xchunk, t.nb);                                            \
+        else                                                                                                    \
+            hipLaunchKernelGGL((stress_sweep2<S, M, TYV, HET>), dim3(t.grid), dim3(64 * TYV), 0, s, g, F, md, c,   \
+                               t0, t1, t.tz, t.tyn, t.xchunk, t.nb);                                            \
+        return hipGetLastError();                                                                               \
+    }
+        OPESCI_PAIR(4)
+        OPESCI_PAIR(8)
+#undef OPESCI_PAIR
+    }
     if (tile_y == 16) {
         const Tiling t = tiling<M>(g, cfg, 16);
         if (VEL)

@@ -214,11 +214,14 @@ def test_c1_second_order_fp64_64cube():
     (6, True, 'native', False, 8), (8, False, 'native', False, 8), (8, True, 'native', False, 4),
     (4, False, 'fp16', False, 8), (4, False, 'bf16', False, 8), (4, False, 'native', True, 8),
     (8, True, 'native', True, 8), (4, False, 'native', False, 16), (8, False, 'native', True, 16),
-    (2, True, 'native', False, 16), (4, False, 'native', False, 0)])
+    (2, True, 'native', False, 16), (4, False, 'native', False, 0), (2, False, 'fp16', False, 8),
+    (6, False, 'bf16', False, 4), (8, False, 'fp16', True, 8), (4, False, 'bf16', True, 16),
+    (8, False, 'bf16', False, 4), (4, False, 'fp16', True, 0)])
 def test_sweep_equals_gather_bitwise(order, double, storage, het, tile_y):
-    """The LDS/register-queue sweep kernels and the plain gather kernels keep
-    the same accumulation order, so their results agree bit for bit (odd
-    sizes exercise partial tiles and x-chunks)."""
+    """The LDS/register-queue sweep kernels (two points per lane for 16-bit
+    storage) and the plain gather kernels keep the same accumulation order,
+    so their results agree bit for bit (the odd interior size 27 exercises
+    partial tiles, x-chunks and a half-filled last pair)."""
     if het:
         g, _ = hetero_model(27, order, 7, double=double, storage=storage, sigma=2.0)
     else:
