@@ -1,6 +1,7 @@
-"""Batched propagators: several models of different shapes, orders and media
-stepped together by one graph launch (autotuned launch shapes) give exactly
-the fields and L2 norms of running each model alone."""
+"""Batched propagators: several models of different shapes, orders, media and
+storage stepped together by one graph launch of batched kernels (autotuned
+launch shapes; one launch per kind of model and per boundary level index)
+give exactly the fields and L2 norms of running each model alone."""
 import numpy as np
 import pytest
 
@@ -11,8 +12,13 @@ from workloads import box_model
 
 pytestmark = pytest.mark.gpu
 
-SPECS = [((30, 24, 20), 2, False, False), ((22, 28, 26), 4, True, False), ((26, 20, 30), 8, False, False),
-         ((20, 22, 24), 4, False, True), ((24, 26, 22), 8, True, False)]
+# (sizes, order, heterogeneous, FP64, storage): kinds that share a batched
+# launch (the first two, and the two FP16 ones) and 16-bit storage
+SPECS = [((30, 24, 20), 2, False, False, 'native'), ((19, 27, 23), 2, False, False, 'native'),
+         ((22, 28, 26), 4, True, False, 'native'), ((26, 20, 30), 8, False, False, 'native'),
+         ((20, 22, 24), 4, False, True, 'native'), ((24, 26, 22), 8, True, False, 'native'),
+         ((23, 25, 27), 4, False, False, 'fp16'), ((27, 21, 25), 4, False, False, 'fp16'),
+         ((21, 23, 29), 2, True, False, 'bf16')]
 
 
 def single(plan, steps):
@@ -23,7 +29,8 @@ def single(plan, steps):
 
 
 def test_batch_equals_single_runs():
-    models = [box_model(sizes, order, 12, hetero=het, double=dbl, sigma=2.0) for sizes, order, het, dbl in SPECS]
+    models = [box_model(sizes, order, 12, hetero=het, double=dbl, sigma=2.0, storage=st)
+              for sizes, order, het, dbl, st in SPECS]
     plans = [g.lower() for g, _ in models]
     with Batch(plans) as b:
         assert len(b) == len(SPECS) and all(c is not None for c in b.configs)

@@ -37,7 +37,7 @@ def hetero_model(N, order, steps, double=False, storage='native', seed=2016, sig
     return g, dict(order=order, n=(n, n, n), ntsteps=steps, dt=dt, double=double, media=(rho, vp, vs))
 
 
-def box_model(sizes, order, steps, hetero=False, double=False, seed=2016, sigma=4.0):
+def box_model(sizes, order, steps, hetero=False, double=False, seed=2016, sigma=4.0, storage='native'):
     """Eigenwave (or the seeded smooth medium) on the unit cube with a
     different number of interior points per axis (dx_k = 1/(N_k - 1))."""
     n = tuple(int(v) - 1 for v in sizes)
@@ -47,11 +47,11 @@ def box_model(sizes, order, steps, hetero=False, double=False, seed=2016, sigma=
         rho, vp, vs = smooth_random_media(dims, seed=seed, sigma=sigma)
         dt = stable_dt(dmin, float(vp.max()), order=4)
         g = opesci.eigenwave3d((1.0, 1.0, 1.0), n, dt, dt * (steps + 0.5), accuracy_order=[2, order, order, order],
-                               double=double, media_arrays=(rho, vp, vs))
+                               double=double, storage=storage, media_arrays=(rho, vp, vs))
         return g, dict(order=order, n=n, ntsteps=steps, dt=dt, double=double, media=(rho, vp, vs))
     dt = 0.5 * dmin / np.sqrt(3.0 * 3.0)
     g = opesci.eigenwave3d((1.0, 1.0, 1.0), n, dt, dt * (steps + 0.5), accuracy_order=[2, order, order, order],
-                           rho=1.0, vp=np.sqrt(3.0), vs=1.0, double=double)
+                           rho=1.0, vp=np.sqrt(3.0), vs=1.0, double=double, storage=storage)
     g.set_variable('lambda', 1.0, 'float', True)
     g.set_variable('mu', 1.0, 'float', True)
     g._invalidate()
