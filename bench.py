@@ -47,7 +47,7 @@ def main():
     ap.add_argument('--order', type=int, default=4)
     ap.add_argument('--storage', default='native')
     ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--cpu-steps', type=int, default=40)
+    ap.add_argument('--cpu-steps', type=int, default=100)
     ap.add_argument('--variant', type=int, default=None, help='interior kernel variant (0 sweep, 1 gather)')
     ap.add_argument('--x-chunk', type=int, default=0)
     ap.add_argument('--tile-y', type=int, default=0)
@@ -95,6 +95,9 @@ def main():
         kt = ctx.kernel_times(10)   # per-launch averages, after the timed region
         bc_stats = ctx.stats()
     workload_key = 'stress_sweep %d^3 order %d %s%s' % (N, order, args.storage, ' hetero' if args.hetero else '')
+    # flops per point-step: ADD + MUL of the stress and velocity kernels in the
+    # reference's own operation count (get_*_kernel_ai)
+    flops_pt = sum(ai[2] + ai[3] for ai in (g.get_stress_kernel_ai(), g.get_velocity_kernel_ai()))
     ms_step = secs * 1e3 / K
     value = pts * K / secs
     media_streams = (5, 3) if args.hetero else (0, 0)   # lambda, mu, mu12/13/23 | beta1..3 (FP32)
@@ -125,6 +128,8 @@ def main():
             'step_hbm_roofline_frac': value * step_bytes / pts / HBM_PEAK,
             'wall_s': wall,
             'kernel_ms': {'stress': kt[0], 'velocity': kt[1], 'boundary': kt[2]},
+            'flops_per_point_step': flops_pt,
+            'mflops': flops_pt * value / 1e6,
             'boundary_launches_per_step': bc_stats[0],
             'deep_z_face_statements': bc_stats[1],
         },
@@ -184,7 +189,9 @@ def committed_traffic(key):
     return None
 
 
-def cpu_baseline(N, order, steps, hetero=False):
+def cpu_baseline(N, order, steps, hetero=False, runs=3):
+    """Median of `runs` timed runs of the oracle's time loop, built with the
+    reference GNU flags, and of its -march=native build beside it."""
     sys.path.insert(0, os.path.join(ROOT, 'oracle'))
     import oracle
     from workloads import eigen_model, hetero_model
@@ -192,12 +199,18 @@ def cpu_baseline(N, order, steps, hetero=False):
     cores = min(16, os.cpu_count() or 1)
     kw['nthreads'] = cores
     kw['want_fields'] = False
-    r = oracle.run(**kw)
     pts = float(N) ** 3 * steps
-    return {'value': pts / r['seconds'], 'unit': 'grid-pt updates/s', 'cores': cores, 'kind': 'port',
-            'sample': '%d steps of the same %d^3 order-%d FP32 %s, oracle/elastic_oracle.cpp built with the '
-                      'reference GNU flags (-O3 -fno-tree-vectorize -fopenmp), time loop only'
-                      % (steps, N, order, 'heterogeneous model' if hetero else 'eigenwave')}
+    rates = {}
+    for native in (False, True):
+        secs = sorted(oracle.run(native=native, **kw)['seconds'] for _ in range(runs))
+        rates[native] = pts / secs[len(secs) // 2]
+    return {'value': rates[False], 'unit': 'grid-pt updates/s', 'cores': cores, 'kind': 'port',
+            'sample': 'median of %d runs of %d steps of the same %d^3 order-%d FP32 %s, oracle/elastic_oracle.cpp '
+                      'built with the reference GNU flags (-O3 -fno-tree-vectorize -fopenmp), time loop only, '
+                      'OpenMP on %d threads' % (runs, steps, N, order,
+                                                'heterogeneous model' if hetero else 'eigenwave', cores),
+            'native_value': rates[True],
+            'native_sample': 'the same built with -O3 -march=native -fopenmp (the stronger CPU baseline)'}
 
 
 if __name__ == '__main__':
