@@ -370,4 +370,4 @@ class StaggeredGrid(RegularGrid):
     @property
     def initialise_bc(self):
         from .emit import bc_listing
-        return bc_listing(self, phase=0) + bc_listing(self, phase=1)
+        return bc_listing(self, phase=0) + '\n' + bc_listing(self, phase=1)
