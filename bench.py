@@ -48,7 +48,7 @@ def main():
     ap.add_argument('--storage', default='native')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--cpu-steps', type=int, default=100)
-    ap.add_argument('--variant', type=int, default=None, help='interior kernel variant (0 sweep, 1 gather)')
+    ap.add_argument('--variant', type=int, default=None, help='interior kernel variant (0 sweep, 1 gather, 2 one-point-per-lane sweep)')
     ap.add_argument('--x-chunk', type=int, default=0)
     ap.add_argument('--tile-y', type=int, default=0)
     ap.add_argument('--copy-probe', type=int, default=0, help='MiB copied device-to-device 3x first')

@@ -172,7 +172,9 @@ typedef struct OpesciModelDesc {
     /* launch tuning (0 = library default) */
     int32_t tile_y;
     int32_t x_chunk;
-    int32_t kernel_variant;     /* 0 default (LDS/register-queue), 1 plain gather */
+    int32_t kernel_variant;     /* 0 default (LDS/register-queue sweep, two z points per lane
+                                   for FP32 arithmetic), 1 plain gather, 2 sweep with one
+                                   point per lane (the FP64 default) */
     int32_t use_graph;          /* capture the time loop in a hipGraph */
     /* adjoint: derivative programs of the media-dependent boundary
      * coefficients (may be 0 / null when no gradient is asked for) */
@@ -245,10 +247,11 @@ int opesci_ctx_kernel_times(OpesciContext* ctx, int32_t nsteps, double out_ms[3]
 int opesci_ctx_stats(OpesciContext* ctx, int32_t out[4]);
 
 /* Launch configuration of the interior kernels (kernel_variant 0 = sweep,
- * 1 = gather; tile_y 0 (default: 16 for FP32 constant media, else 8) | 4 |
- * 8 | 16; x_chunk 0 = default), used by the
- * autotuner.  Results do not depend on it (same arithmetic in every
- * configuration).  No reference counterpart. */
+ * two z points per lane for FP32 arithmetic, tile_y 0 (default 8) | 4 | 8,
+ * 16 runs as 8; 1 = gather; 2 = sweep with one point per lane, tile_y 0
+ * (default: 16 for FP32 constant media, else 8) | 4 | 8 | 16; x_chunk 0 =
+ * default), used by the autotuner.  Results do not depend on it (same
+ * arithmetic in every configuration).  No reference counterpart. */
 int opesci_ctx_configure(OpesciContext* ctx, int32_t kernel_variant, int32_t tile_y, int32_t x_chunk);
 
 /* Batches: independent contexts (any shapes, orders, media) stepped together

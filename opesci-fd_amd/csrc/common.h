@@ -130,6 +130,25 @@ template <typename S>
 __device__ __forceinline__ void stnt2(S* p, float a, float b) {
     __builtin_nontemporal_store(Pair<S>::join(a, b), reinterpret_cast<uint32_t*>(p));
 }
+// FP32 storage: a lane's two points as one 8-byte access (z - m even is
+// 8-byte aligned: z = m starts a 128-byte line)
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void ld2(const float* p, float& a, float& b) {
+    const f32x2_t v = *reinterpret_cast<const f32x2_t*>(p);
+    a = v.x;
+    b = v.y;
+}
+__device__ __forceinline__ void ldnt2(const float* p, float& a, float& b) {
+    const f32x2_t v = __builtin_nontemporal_load(reinterpret_cast<const f32x2_t*>(p));
+    a = v.x;
+    b = v.y;
+}
+__device__ __forceinline__ void stnt2(float* p, float a, float b) {
+    f32x2_t v;
+    v.x = a;
+    v.y = b;
+    __builtin_nontemporal_store(v, reinterpret_cast<f32x2_t*>(p));
+}
 
 // ---------------------------------------------------------------- geometry
 struct Geom {

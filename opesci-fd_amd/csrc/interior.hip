@@ -15,8 +15,10 @@
 // the read path.  Arithmetic in the compute type R (float or double); fields
 // stored as S (float, double, _Float16, bf16).
 //
-// Two implementations:
-//  * sweep (default): a block owns a 64(z) x TY(y) column tile and sweeps x.
+// Three implementations (kernel_variant):
+//  * 0, pair sweep (default for FP32 arithmetic, any storage): the sweep below
+//    with two z points per lane and 128(z) x TY(y) tiles; FP64 uses 2.
+//  * 2, sweep: a block owns a 64(z) x TY(y) column tile and sweeps x.
 //    Fields differentiated along x live in per-thread register queues, the
 //    current x plane of fields differentiated along y or z is staged with its
 //    halo in LDS, so every field is read from HBM once per sweep.  Blocks are
@@ -24,8 +26,9 @@
 //    halo lines in that XCD's L2.  The cell's own previous value and every
 //    store are non-temporal (streamed once); only the stencil operands, whose
 //    halo lines neighbouring tiles re-read, are ordinary cached loads.
-//  * gather: one thread per (y,z) column, every stencil operand loaded from
+//  * 1, gather: one thread per (y,z) column, every stencil operand loaded from
 //    global memory (caches only); kept as the simple reference form.
+// All three keep one accumulation order, so their results agree bit for bit.
 #include "common.h"
 
 namespace opesci {
@@ -466,13 +469,16 @@ __global__ __launch_bounds__(64 * TY) void velocity_sweep(Geom g, FieldPtrs<S> F
     }
 }
 
-// ------------------------------------------------------------------ pair sweeps (16-bit storage)
-// The same sweeps for FP16/BF16 fields with two z points per lane: a tile is
-// 128 (z) x TY (y), lane lz owns z0 + 2 lz and z0 + 2 lz + 1, and every
-// own-column value and y-halo row is moved as one 32-bit pair (tiles start
-// at z = m, which the layout aligns to 128 bytes, so pairs are aligned; the
-// row padding holds the partner of the last point).  Per point the
-// arithmetic is the single-point sweeps' and the gather kernels'.
+// ------------------------------------------------------------------ pair sweeps (FP32 arithmetic)
+// The same sweeps with two z points per lane: a tile is 128 (z) x TY (y),
+// lane lz owns z0 + 2 lz and z0 + 2 lz + 1, and every own-column value and
+// y-halo row is moved as one pair (32 bits for FP16/BF16 storage, 64 bits
+// for FP32; tiles start at z = m, which the layout aligns to 128 bytes, so
+// pairs are aligned; the row padding holds the partner of the last point).
+// Twice the bytes per load instruction and half the blocks' z halos: at
+// 256^3 the FP32 read path runs 11 % faster than with one point per lane.
+// Per point the arithmetic is the single-point sweeps' and the gather
+// kernels'.
 template <int M, int TY>
 struct Plane2 {
     static constexpr int W = 128 + 2 * M;
@@ -882,9 +888,10 @@ static Tiling tiling(const Geom& g, const InteriorLaunch& cfg, int ty, int tzw =
     return t;
 }
 
-// Rows per tile when the caller leaves it open: 16 for FP32 on constant media
-// (fewer y-halo rows; measured 2-3 % faster at 256^3), else 8 (the media
-// streams and the 16-bit conversions favour the smaller tile).
+// Rows per tile of the single-point sweeps when the caller leaves it open: 16
+// for FP32 on constant media (fewer y-halo rows; measured 2-3 % faster at
+// 256^3), else 8 (the media streams and the 16-bit conversions favour the
+// smaller tile).
 template <typename R, typename S, bool HET>
 static int default_tile_y() {
     return (!HET && sizeof(R) == 4 && sizeof(S) == 4) ? 16 : 8;
@@ -906,12 +913,13 @@ static hipError_t run_interior(const Geom& g, void* fields, const void* media, c
                                t.tz, t.tyn, t.xchunk, t.nb);
         return hipGetLastError();
     }
-    const int tile_y = cfg.tile_y ? cfg.tile_y : default_tile_y<R, S, HET>();
-    if constexpr (sizeof(S) == 2) {
-        // 16-bit storage: the pair sweeps (128-wide z tiles; 4 or 8 rows, as
-        // 1024-thread blocks would spill the doubled register queues)
-        const int ty = tile_y == 4 ? 4 : 8;
-        const Tiling t = tiling<M>(g, cfg, ty, 128);
+    if constexpr (sizeof(R) == 4) {
+        if (cfg.variant == 0) {
+            // FP32 arithmetic: the pair sweeps (two z points per lane, 128-wide
+            // z tiles; 4 or 8 rows, default 8, as 1024-thread blocks would
+            // spill the doubled register queues)
+            const int ty = cfg.tile_y == 4 ? 4 : 8;
+            const Tiling t = tiling<M>(g, cfg, ty, 128);
 #define OPESCI_PAIR(TYV)                                                                                        \
     if (ty == TYV) {                                                                                            \
         if (VEL)                                                                                                \
@@ -922,10 +930,13 @@ static hipError_t run_interior(const Geom& g, void* fields, const void* media, c
                                t0, t1, t.tz, t.tyn, t.xchunk, t.nb);                                            \
         return hipGetLastError();                                                                               \
     }
-        OPESCI_PAIR(4)
-        OPESCI_PAIR(8)
+            OPESCI_PAIR(4)
+            OPESCI_PAIR(8)
 #undef OPESCI_PAIR
+        }
     }
+    // one point per lane: kernel_variant 2, and every FP64 sweep
+    const int tile_y = cfg.tile_y ? cfg.tile_y : default_tile_y<R, S, HET>();
     if (tile_y == 16) {
         const Tiling t = tiling<M>(g, cfg, 16);
         if (VEL)

@@ -144,6 +144,9 @@ int validate(const OpesciModelDesc* d) {
     if (d->storage != OPESCI_STORE_NATIVE && d->real_bytes != 4)
         return fail(OPESCI_EUNSUPPORTED, "16-bit storage needs FP32 compute");
     if (d->ntsteps < 0) return fail(OPESCI_EINVAL, "negative ntsteps");
+    if (d->kernel_variant < 0 || d->kernel_variant > 2 || d->x_chunk < 0 ||
+        (d->tile_y != 0 && d->tile_y != 4 && d->tile_y != 8 && d->tile_y != 16))
+        return fail(OPESCI_EINVAL, "kernel_variant 0|1|2, tile_y 0|4|8|16, x_chunk >= 0");
     if (d->media_arrays && (!d->rho || !d->vp || !d->vs)) return fail(OPESCI_EINVAL, "media arrays missing");
     if (d->nstmts < 0 || d->nterms < 0 || d->ncode < 0 || d->nsol < 0)
         return fail(OPESCI_EINVAL, "negative table size");
@@ -754,9 +757,9 @@ int opesci_ctx_stats(OpesciContext* ctx, int32_t out[4]) {
 
 int opesci_ctx_configure(OpesciContext* ctx, int32_t kernel_variant, int32_t tile_y, int32_t x_chunk) {
     if (!ctx) return fail(OPESCI_EINVAL, "null context");
-    if ((kernel_variant != 0 && kernel_variant != 1) || (tile_y != 0 && tile_y != 4 && tile_y != 8 && tile_y != 16) ||
+    if (kernel_variant < 0 || kernel_variant > 2 || (tile_y != 0 && tile_y != 4 && tile_y != 8 && tile_y != 16) ||
         x_chunk < 0)
-        return fail(OPESCI_EINVAL, "kernel_variant 0|1, tile_y 0|4|8|16, x_chunk >= 0");
+        return fail(OPESCI_EINVAL, "kernel_variant 0|1|2, tile_y 0|4|8|16, x_chunk >= 0");
     ctx->cfg = InteriorLaunch{kernel_variant, tile_y, x_chunk};
     ctx->desc.kernel_variant = kernel_variant;
     ctx->desc.tile_y = tile_y;

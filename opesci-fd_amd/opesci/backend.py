@@ -281,7 +281,9 @@ class Context(object):
         return tuple(out[:3])
 
     def configure(self, kernel_variant=0, tile_y=0, x_chunk=0):
-        """Interior launch configuration (results are the same for every one)."""
+        """Interior launch configuration: kernel_variant 0 sweep (two z points
+        per lane for FP32 arithmetic), 1 gather, 2 sweep with one point per
+        lane; results are the same for every one."""
         _check(self.lib.opesci_ctx_configure(self.h, kernel_variant, tile_y, x_chunk), 'opesci_ctx_configure')
 
     def gradient(self, observed, checkpoint_every=32):

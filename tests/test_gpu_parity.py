@@ -209,19 +209,23 @@ def test_c1_second_order_fp64_64cube():
         assert abs(l2[f] - ref['l2'][f]) <= 1e-10 * ref['l2'][f] + 1e-15, FIELDS[f]
 
 
-@pytest.mark.parametrize("order,double,storage,het,tile_y", [
-    (2, False, 'native', False, 8), (4, False, 'native', False, 8), (4, False, 'native', False, 4),
-    (6, True, 'native', False, 8), (8, False, 'native', False, 8), (8, True, 'native', False, 4),
-    (4, False, 'fp16', False, 8), (4, False, 'bf16', False, 8), (4, False, 'native', True, 8),
-    (8, True, 'native', True, 8), (4, False, 'native', False, 16), (8, False, 'native', True, 16),
-    (2, True, 'native', False, 16), (4, False, 'native', False, 0), (2, False, 'fp16', False, 8),
-    (6, False, 'bf16', False, 4), (8, False, 'fp16', True, 8), (4, False, 'bf16', True, 16),
-    (8, False, 'bf16', False, 4), (4, False, 'fp16', True, 0)])
-def test_sweep_equals_gather_bitwise(order, double, storage, het, tile_y):
-    """The LDS/register-queue sweep kernels (two points per lane for 16-bit
-    storage) and the plain gather kernels keep the same accumulation order,
-    so their results agree bit for bit (the odd interior size 27 exercises
-    partial tiles, x-chunks and a half-filled last pair)."""
+@pytest.mark.parametrize("order,double,storage,het,tile_y,variant", [
+    (2, False, 'native', False, 8, 0), (4, False, 'native', False, 8, 0), (4, False, 'native', False, 4, 0),
+    (6, True, 'native', False, 8, 0), (8, False, 'native', False, 8, 0), (8, True, 'native', False, 4, 0),
+    (4, False, 'fp16', False, 8, 0), (4, False, 'bf16', False, 8, 0), (4, False, 'native', True, 8, 0),
+    (8, True, 'native', True, 8, 0), (4, False, 'native', False, 16, 0), (8, False, 'native', True, 16, 0),
+    (2, True, 'native', False, 16, 0), (4, False, 'native', False, 0, 0), (2, False, 'fp16', False, 8, 0),
+    (6, False, 'bf16', False, 4, 0), (8, False, 'fp16', True, 8, 0), (4, False, 'bf16', True, 16, 0),
+    (8, False, 'bf16', False, 4, 0), (4, False, 'fp16', True, 0, 0), (6, False, 'native', True, 4, 0),
+    (4, False, 'native', False, 16, 2), (4, False, 'native', False, 8, 2), (8, False, 'native', True, 4, 2),
+    (2, False, 'native', True, 16, 2), (4, False, 'fp16', False, 8, 2), (6, False, 'bf16', True, 0, 2),
+    (4, True, 'native', False, 0, 2)])
+def test_sweep_equals_gather_bitwise(order, double, storage, het, tile_y, variant):
+    """The LDS/register-queue sweep kernels (two points per lane for FP32
+    arithmetic, variant 0; one point per lane, variant 2) and the plain
+    gather kernels keep the same accumulation order, so their results agree
+    bit for bit (the odd interior size 27 exercises partial tiles, x-chunks
+    and a half-filled last pair)."""
     if het:
         g, _ = hetero_model(27, order, 7, double=double, storage=storage, sigma=2.0)
     else:
@@ -229,7 +233,7 @@ def test_sweep_equals_gather_bitwise(order, double, storage, het, tile_y):
     plan = g.lower()
     plan.tile_y = tile_y
     plan.x_chunk = 5
-    plan.kernel_variant = 0
+    plan.kernel_variant = variant
     a, _ = gpu_run(plan)
     plan.kernel_variant = 1
     b, _ = gpu_run(plan)
