@@ -68,7 +68,7 @@ def gradient(ctx, model, obs, every):
     J = ElasticMisfit.apply(*t, ctx, obs, every)
     J.backward()
     secs = time.time() - t0
-    return float(J), {k: v.grad.numpy() for k, v in zip(PARAMS, t)}, secs
+    return float(J.detach()), {k: v.grad.numpy() for k, v in zip(PARAMS, t)}, secs
 
 
 def main(argv=None):
