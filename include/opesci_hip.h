@@ -247,8 +247,10 @@ int opesci_ctx_kernel_times(OpesciContext* ctx, int32_t nsteps, double out_ms[3]
 int opesci_ctx_stats(OpesciContext* ctx, int32_t out[4]);
 
 /* Launch configuration of the interior kernels (kernel_variant 0 = sweep,
- * two z points per lane for FP32 arithmetic, tile_y 0 (default 8) | 4 | 8,
- * 16 runs as 8; 1 = gather; 2 = sweep with one point per lane, tile_y 0
+ * two z points per lane for FP32 arithmetic, tile_y 0 | 4 | 8 | 16: stress
+ * tiles have 4 or 8 rows (default 8, 16 runs as 8), velocity tiles 16 at
+ * orders 2 and 4 when asked or by default with FP32 storage, else as the
+ * stress tiles; 1 = gather; 2 = sweep with one point per lane, tile_y 0
  * (default: 16 for FP32 constant media, else 8) | 4 | 8 | 16; x_chunk 0 =
  * default), used by the autotuner.  Results do not depend on it (same
  * arithmetic in every configuration).  No reference counterpart. */

@@ -916,9 +916,14 @@ static hipError_t run_interior(const Geom& g, void* fields, const void* media, c
     if constexpr (sizeof(R) == 4) {
         if (cfg.variant == 0) {
             // FP32 arithmetic: the pair sweeps (two z points per lane, 128-wide
-            // z tiles; 4 or 8 rows, default 8, as 1024-thread blocks would
-            // spill the doubled register queues)
-            const int ty = cfg.tile_y == 4 ? 4 : 8;
+            // z tiles).  Stress tiles have 4 or 8 rows (default 8), as
+            // 1024-thread blocks would spill the doubled register queues; the
+            // velocity kernel, with fewer of them, takes 16 rows at orders 2
+            // and 4 (the default for FP32 storage: fewer y-halo rows, 2-5 %
+            // faster at 256^3)
+            int ty = cfg.tile_y == 4 ? 4 : 8;
+            if constexpr (VEL && M <= 2)
+                if (cfg.tile_y == 16 || (cfg.tile_y == 0 && sizeof(S) == 4)) ty = 16;
             const Tiling t = tiling<M>(g, cfg, ty, 128);
 #define OPESCI_PAIR(TYV)                                                                                        \
     if (ty == TYV) {                                                                                            \
@@ -933,6 +938,13 @@ static hipError_t run_interior(const Geom& g, void* fields, const void* media, c
             OPESCI_PAIR(4)
             OPESCI_PAIR(8)
 #undef OPESCI_PAIR
+            if constexpr (VEL && M <= 2) {
+                if (ty == 16) {
+                    hipLaunchKernelGGL((velocity_sweep2<S, M, 16, HET>), dim3(t.grid), dim3(1024), 0, s, g, F, md, c,
+                                       t0, t1, t.tz, t.tyn, t.xchunk, t.nb);
+                    return hipGetLastError();
+                }
+            }
         }
     }
     // one point per lane: kernel_variant 2, and every FP64 sweep

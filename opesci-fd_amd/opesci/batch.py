@@ -19,9 +19,9 @@ from . import backend
 __all__ = ['Batch', 'autotune', 'CONFIGS']
 
 # (kernel_variant, tile_y, x_chunk): the default sweep (two z points per lane
-# for FP32 arithmetic) with 8 or 4 rows per tile, default or fixed x chunks,
-# and the one-point-per-lane sweep with 16 or 8 rows
-CONFIGS = ((0, 8, 0), (0, 8, 64), (0, 4, 0), (2, 16, 0), (2, 8, 0))
+# for FP32 arithmetic) with its default rows, 8 or 4 rows per tile, and the
+# one-point-per-lane sweep with 16 or 8 rows
+CONFIGS = ((0, 0, 0), (0, 8, 0), (0, 4, 0), (2, 16, 0), (2, 8, 0))
 
 
 def _key(plan):

@@ -219,7 +219,8 @@ def test_c1_second_order_fp64_64cube():
     (8, False, 'bf16', False, 4, 0), (4, False, 'fp16', True, 0, 0), (6, False, 'native', True, 4, 0),
     (4, False, 'native', False, 16, 2), (4, False, 'native', False, 8, 2), (8, False, 'native', True, 4, 2),
     (2, False, 'native', True, 16, 2), (4, False, 'fp16', False, 8, 2), (6, False, 'bf16', True, 0, 2),
-    (4, True, 'native', False, 0, 2)])
+    (4, True, 'native', False, 0, 2), (4, False, 'native', True, 0, 0), (2, False, 'native', True, 16, 0),
+    (4, False, 'fp16', False, 16, 0)])
 def test_sweep_equals_gather_bitwise(order, double, storage, het, tile_y, variant):
     """The LDS/register-queue sweep kernels (two points per lane for FP32
     arithmetic, variant 0; one point per lane, variant 2) and the plain
