@@ -15,7 +15,7 @@ import pytest
 import oracle
 from opesci.backend import Context
 from planvm import PlanVM
-from workloads import eigen_model, hetero_model
+from workloads import box_model, eigen_model, hetero_model
 
 pytestmark = pytest.mark.gpu
 
@@ -235,6 +235,25 @@ def test_sweep_equals_gather_bitwise(order, double, storage, het, tile_y, varian
     plan.tile_y = tile_y
     plan.x_chunk = 5
     plan.kernel_variant = variant
+    a, _ = gpu_run(plan)
+    plan.kernel_variant = 1
+    b, _ = gpu_run(plan)
+    for f in range(9):
+        assert np.array_equal(a[f], b[f]), FIELDS[f]
+
+
+@pytest.mark.parametrize("sizes,order,het,storage,tile_y", [
+    ((31, 19, 45), 4, False, 'native', 0), ((18, 40, 27), 2, True, 'native', 0), ((23, 21, 133), 8, True, 'native', 8),
+    ((40, 33, 29), 4, True, 'native', 16), ((25, 30, 131), 6, False, 'fp16', 4), ((33, 17, 38), 4, True, 'bf16', 0)])
+def test_pair_sweeps_equal_gather_on_boxes(sizes, order, het, storage, tile_y):
+    """The default pair sweeps on non-cubic boxes (odd and even z extents,
+    z tiles cut short and across two 128-point tiles, ragged y tiles of 8
+    and 16 rows) give the gather kernels' bits."""
+    g, _ = box_model(sizes, order, 6, hetero=het, storage=storage, sigma=2.0)
+    plan = g.lower()
+    plan.tile_y = tile_y
+    plan.x_chunk = 0
+    plan.kernel_variant = 0
     a, _ = gpu_run(plan)
     plan.kernel_variant = 1
     b, _ = gpu_run(plan)
