@@ -244,8 +244,10 @@ __device__ __forceinline__ bool interior_cell(const Geom& g, int& x, int& y, int
     return z < g.nz - M && y < g.ny - M;
 }
 
-// Q_i = beta_i bar V_i(t1); bar V(t0) += bar V(t1); bar V(t1) = 0; beta
-// gradients from the forward stresses at t1.
+// Q_i = beta_i bar V_i(t1); beta gradients from the forward stresses at t1.
+// The identity part of the update's transpose (bar V(t0) += bar V(t1),
+// bar V(t1) = 0) is a swap of the two levels' arrays on the host plus
+// adj_move_cells (see adj_swap_levels).
 template <typename R, int M>
 __global__ __launch_bounds__(256) void adj_vel_prep(Geom g, FieldPtrs<R> F, AdjPtrs<R> B, MediaPtrs<R> md,
                                                     GradPtrs<R> gm, Coefs<R> c, Scratch<R> S, int t0, int t1) {
@@ -260,8 +262,6 @@ __global__ __launch_bounds__(256) void adj_vel_prep(Geom g, FieldPtrs<R> F, AdjP
         // stencil reads stay cached)
         const R b = __builtin_nontemporal_load(B.f[i][t1] + a);
         __builtin_nontemporal_store(__builtin_nontemporal_load(md.m[OPESCI_M_BETA1 + i] + a) * b, S.a[i] + a);
-        __builtin_nontemporal_store(__builtin_nontemporal_load(B.f[i][t0] + a) + b, B.f[i][t0] + a);
-        __builtin_nontemporal_store((R)0, B.f[i][t1] + a);
         R s = 0;
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
@@ -275,8 +275,9 @@ __global__ __launch_bounds__(256) void adj_vel_prep(Geom g, FieldPtrs<R> F, AdjP
     }
 }
 
-// P_j, W_s from bar T(t1); bar T(t0) += bar T(t1); bar T(t1) = 0; lambda, mu,
-// mu12/mu23/mu13 gradients from the forward velocities at t0.
+// P_j, W_s from bar T(t1); lambda, mu, mu12/mu23/mu13 gradients from the
+// forward velocities at t0.  The identity part (bar T(t0) += bar T(t1),
+// bar T(t1) = 0) is done by adj_swap_levels.
 template <typename R, int M>
 __global__ __launch_bounds__(256) void adj_stress_prep(Geom g, FieldPtrs<R> F, AdjPtrs<R> B, MediaPtrs<R> md,
                                                        GradPtrs<R> gm, Coefs<R> c, Scratch<R> S, int t0, int t1) {
@@ -288,11 +289,7 @@ __global__ __launch_bounds__(256) void adj_stress_prep(Geom g, FieldPtrs<R> F, A
     // reads stay cached)
     R bt[6];
 #pragma unroll
-    for (int f = 0; f < 6; ++f) {
-        bt[f] = __builtin_nontemporal_load(B.f[3 + f][t1] + a);
-        __builtin_nontemporal_store(__builtin_nontemporal_load(B.f[3 + f][t0] + a) + bt[f], B.f[3 + f][t0] + a);
-        __builtin_nontemporal_store((R)0, B.f[3 + f][t1] + a);
-    }
+    for (int f = 0; f < 6; ++f) bt[f] = __builtin_nontemporal_load(B.f[3 + f][t1] + a);
     const R lam = __builtin_nontemporal_load(md.m[OPESCI_M_LAMBDA] + a);
     const R mu = __builtin_nontemporal_load(md.m[OPESCI_M_MU] + a);
     const R tr = bt[0] + bt[1] + bt[2];
@@ -581,13 +578,47 @@ static dim3 rim_grid(const Geom& g) {
     return dim3((unsigned)((slab + 255) / 256), 6);
 }
 
+// bar F(t0)[c] += bar F(t1)[c]; bar F(t1)[c] = 0 for a list of distinct
+// cells, each entry (offset << 4) | field.
+template <typename R>
+__global__ __launch_bounds__(256) void adj_move_cells(AdjPtrs<R> B, const int64_t* __restrict__ cells, int n, int t0,
+                                                      int t1) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int64_t e = cells[i];
+    const int f = (int)(e & 15);
+    const int64_t off = e >> 4;
+    R* a = B.f[f][t0] + off;
+    R* b = B.f[f][t1] + off;
+    *a += *b;
+    *b = (R)0;
+}
+
+// The identity part of an update's transpose, bar F(t0) += bar F(t1) and
+// bar F(t1) = 0 for the fields [f0, f1), without touching the arrays: at
+// this point bar F(t0) is zero everywhere except at the cells that
+// boundary statements of the step read at level 0 (their transposes ran
+// first) and at cells no statement or update ever writes (whose adjoint
+// values nothing reads).  So the two levels' arrays swap places in the
+// host-side pointer image, and only the listed cells move.  Every later
+// launch, of this step and of the steps before it, reads the swapped image.
+template <typename R>
+static hipError_t adj_swap_levels(AdjPtrs<R>& B, int f0, int f1, const AdjCells& cells, int t0, int t1,
+                                  hipStream_t s) {
+    for (int f = f0; f < f1; ++f) std::swap(B.f[f][t0], B.f[f][t1]);
+    if (cells.n > 0)
+        hipLaunchKernelGGL((adj_move_cells<R>), dim3((unsigned)((cells.n + 255) / 256)), dim3(256), 0, s, B,
+                           cells.d, cells.n, t0, t1);
+    return hipGetLastError();
+}
+
 template <typename R, int M>
 static hipError_t adj_stress_t(const Geom& g, void* fields, void* bar, const void* media, void* grad,
-                               const void* coefs, void* const* scratch, const InteriorLaunch& cfg, int t0, int t1,
-                               hipStream_t s) {
+                               const void* coefs, void* const* scratch, const AdjCells& cells,
+                               const InteriorLaunch& cfg, int t0, int t1, hipStream_t s) {
     const int64_t n = (int64_t)g.nx * g.ny * g.nz;
     const FieldPtrs<R>& F = *static_cast<const FieldPtrs<R>*>(fields);
-    const AdjPtrs<R>& B = *static_cast<const AdjPtrs<R>*>(bar);
+    AdjPtrs<R>& B = *static_cast<AdjPtrs<R>*>(bar);
     const MediaPtrs<R>& md = *static_cast<const MediaPtrs<R>*>(media);
     const GradPtrs<R>& gm = *static_cast<const GradPtrs<R>*>(grad);
     const Coefs<R>& c = *static_cast<const Coefs<R>*>(coefs);
@@ -599,8 +630,9 @@ static hipError_t adj_stress_t(const Geom& g, void* fields, void* bar, const voi
     Scratch<R> S;
     for (int i = 0; i < 6; ++i) S.a[i] = static_cast<R*>(scratch[i]);
     // the weighted stresses must be read from bar T(t1) before the rim and
-    // sweep kernels touch bar V(t0); the prep kernel zeroes bar T(t1)
+    // sweep kernels touch bar V(t0)
     hipLaunchKernelGGL((adj_stress_prep<R, M>), interior_grid(g), dim3(256), 0, s, g, F, B, md, gm, c, S, t0, t1);
+    if (hipError_t e = adj_swap_levels(B, 3, NF, cells, t0, t1, s)) return e;
     // velocity-shaped sweep: level 0 = level 1 = bar V(t0), stresses = P, W
     FieldPtrs<R> A;
     for (int i = 0; i < 3; ++i) A.f[i][0] = A.f[i][1] = B.f[i][t0];
@@ -623,10 +655,11 @@ static hipError_t adj_stress_t(const Geom& g, void* fields, void* bar, const voi
 
 template <typename R, int M>
 static hipError_t adj_vel_t(const Geom& g, void* fields, void* bar, const void* media, void* grad, const void* coefs,
-                            void* const* scratch, const InteriorLaunch& cfg, int t0, int t1, hipStream_t s) {
+                            void* const* scratch, const AdjCells& cells, const InteriorLaunch& cfg, int t0, int t1,
+                            hipStream_t s) {
     const int64_t n = (int64_t)g.nx * g.ny * g.nz;
     const FieldPtrs<R>& F = *static_cast<const FieldPtrs<R>*>(fields);
-    const AdjPtrs<R>& B = *static_cast<const AdjPtrs<R>*>(bar);
+    AdjPtrs<R>& B = *static_cast<AdjPtrs<R>*>(bar);
     const MediaPtrs<R>& md = *static_cast<const MediaPtrs<R>*>(media);
     const GradPtrs<R>& gm = *static_cast<const GradPtrs<R>*>(grad);
     const Coefs<R>& c = *static_cast<const Coefs<R>*>(coefs);
@@ -638,6 +671,7 @@ static hipError_t adj_vel_t(const Geom& g, void* fields, void* bar, const void* 
     Scratch<R> S;
     for (int i = 0; i < 6; ++i) S.a[i] = static_cast<R*>(scratch[i]);
     hipLaunchKernelGGL((adj_vel_prep<R, M>), interior_grid(g), dim3(256), 0, s, g, F, B, md, gm, c, S, t0, t1);
+    if (hipError_t e = adj_swap_levels(B, 0, 3, cells, t0, t1, s)) return e;
     // stress-shaped sweep: velocities = Q, level 0 = level 1 = bar T(t1)
     FieldPtrs<R> A;
     for (int i = 0; i < 3; ++i) A.f[i][0] = A.f[i][1] = S.a[i];
@@ -660,16 +694,16 @@ static hipError_t adj_vel_t(const Geom& g, void* fields, void* bar, const void* 
 
 #define OPESCI_ADJ_M(FN, R)                                                                     \
     switch (g.m) {                                                                              \
-        case 1: return FN<R, 1>(g, fields, bar, media, grad, coefs, scratch, cfg, t0, t1, s);    \
-        case 2: return FN<R, 2>(g, fields, bar, media, grad, coefs, scratch, cfg, t0, t1, s);    \
-        case 3: return FN<R, 3>(g, fields, bar, media, grad, coefs, scratch, cfg, t0, t1, s);    \
-        case 4: return FN<R, 4>(g, fields, bar, media, grad, coefs, scratch, cfg, t0, t1, s);    \
+        case 1: return FN<R, 1>(g, fields, bar, media, grad, coefs, scratch, cells, cfg, t0, t1, s); \
+        case 2: return FN<R, 2>(g, fields, bar, media, grad, coefs, scratch, cells, cfg, t0, t1, s); \
+        case 3: return FN<R, 3>(g, fields, bar, media, grad, coefs, scratch, cells, cfg, t0, t1, s); \
+        case 4: return FN<R, 4>(g, fields, bar, media, grad, coefs, scratch, cells, cfg, t0, t1, s); \
         default: return hipErrorInvalidValue;                                                   \
     }
 
 hipError_t launch_adj_stress(const Geom& g, int real_bytes, void* fields, void* bar, const void* media, void* grad,
-                             const void* coefs, void* const* scratch, const InteriorLaunch& cfg, int t0, int t1,
-                             hipStream_t s) {
+                             const void* coefs, void* const* scratch, const AdjCells& cells, const InteriorLaunch& cfg,
+                             int t0, int t1, hipStream_t s) {
     if (real_bytes == 8) {
         OPESCI_ADJ_M(adj_stress_t, double)
     }
@@ -677,8 +711,8 @@ hipError_t launch_adj_stress(const Geom& g, int real_bytes, void* fields, void* 
 }
 
 hipError_t launch_adj_velocity(const Geom& g, int real_bytes, void* fields, void* bar, const void* media, void* grad,
-                               const void* coefs, void* const* scratch, const InteriorLaunch& cfg, int t0, int t1,
-                               hipStream_t s) {
+                               const void* coefs, void* const* scratch, const AdjCells& cells,
+                               const InteriorLaunch& cfg, int t0, int t1, hipStream_t s) {
     if (real_bytes == 8) {
         OPESCI_ADJ_M(adj_vel_t, double)
     }

@@ -243,13 +243,21 @@ hipError_t launch_bc_coef(const Geom& g, int real_bytes, const void* media, cons
                           const OpesciExprOp* d_code, void* pool, int first, int count, int maxpts, hipStream_t s);
 // adjoint (adjoint.hip); native storage, compute type given by real_bytes
 // scratch: six field-sized arrays, zero outside the interior (null selects
-// the simple gather kernels, as does cfg.variant == 1)
+// the simple gather kernels, as does cfg.variant == 1).  bar is the host-side
+// image of the adjoint fields' level pointers: with the scratch path the
+// launch swaps the two levels of the updated fields in it (the identity part
+// of the transpose), moving only `cells` (distinct (offset << 4) | field
+// entries: the cells the step's boundary statements read at level 0).
+struct AdjCells {
+    const int64_t* d = nullptr;
+    int n = 0;
+};
 hipError_t launch_adj_stress(const Geom& g, int real_bytes, void* fields, void* bar, const void* media, void* grad,
-                             const void* coefs, void* const* scratch, const InteriorLaunch& cfg, int t0, int t1,
-                             hipStream_t s);
+                             const void* coefs, void* const* scratch, const AdjCells& cells, const InteriorLaunch& cfg,
+                             int t0, int t1, hipStream_t s);
 hipError_t launch_adj_velocity(const Geom& g, int real_bytes, void* fields, void* bar, const void* media, void* grad,
-                               const void* coefs, void* const* scratch, const InteriorLaunch& cfg, int t0, int t1,
-                               hipStream_t s);
+                               const void* coefs, void* const* scratch, const AdjCells& cells,
+                               const InteriorLaunch& cfg, int t0, int t1, hipStream_t s);
 hipError_t launch_bc_logged(const Geom& g, int real_bytes, void* fields, const BcDesc* desc, const void* pool,
                             const BcLevel& lv, void* log, int t0, int t1, int undo, hipStream_t s);
 hipError_t launch_adj_bc(const Geom& g, int real_bytes, void* fields, void* bar, const BcDesc* desc, const void* pool,

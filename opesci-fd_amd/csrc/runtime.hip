@@ -47,6 +47,7 @@ struct OpesciContext {
     std::vector<OpesciBcStmt> stmts;  // sorted by (phase, sched, axis, ranges)
     std::vector<BcLevel> levels[5];    // phases 0..3 + 4 (deep z-face rows of the stress phase, one launch)
     BcDesc* d_desc = nullptr;
+    std::vector<BcDesc> h_desc;       // host copy of the descriptors (adjoint cell lists)
     void* d_pool = nullptr;           // precomputed media-dependent boundary coefficients (compute type)
     int64_t pool_elems = 0;
     OpesciBcDTerm* d_dterms = nullptr;  // derivative programs of those coefficients (adjoint)
@@ -58,6 +59,9 @@ struct OpesciContext {
     void* sbase[6] = {};              // adjoint scratch (weighted adjoint fields)
     void* sptr[6] = {};
     void* d_gpool = nullptr;
+    int64_t* d_tcells = nullptr;      // adjoint: cells read at level 0 by boundary statements (stresses)
+    int64_t* d_vcells = nullptr;      // ... (velocities, velocity phase)
+    int n_tcells = 0, n_vcells = 0;
     void* d_log = nullptr;            // undo log of one step's boundary levels
     std::vector<int64_t> log_off[5];  // per phase and level
     char* d_ckpt = nullptr;           // checkpoint states, 18 field levels each
@@ -319,6 +323,7 @@ int build_program(OpesciContext* c, const OpesciBcTerm* terms, const OpesciBcDTe
         i = j;
     }
     if (int e = upload(&c->d_desc, descs.data(), descs.size(), c->stream)) return e;
+    c->h_desc = std::move(descs);
     return 0;
 }
 
@@ -382,7 +387,8 @@ void destroy(OpesciContext* c) {
         if (c->mbase[i]) (void)hipFree(c->mbase[i]);
     void* ptrs[] = {c->d_desc,  c->d_code,    c->d_sol,  c->d_partial, c->d_stage, c->d_rec,
                     c->d_trace, c->d_wavelet, c->d_step, c->d_pool,    c->d_dterms, c->d_gpool,
-                    c->d_log,   c->d_ckpt,    c->d_seg,  c->d_obs,     c->d_srclog};
+                    c->d_log,   c->d_ckpt,    c->d_seg,  c->d_obs,     c->d_srclog, c->d_tcells,
+                    c->d_vcells};
     for (int f = 0; f < NF; ++f)
         for (int l = 0; l < 2; ++l)
             if (c->abase[f][l]) (void)hipFree(c->abase[f][l]);
@@ -796,6 +802,41 @@ static int adj_alloc(OpesciContext* c, int ckpt, size_t seg_bytes, int log_steps
             const size_t pb = ((size_t)c->pool_elems * rb + 15) & ~size_t(15);
             if (int e = alloc_zero(c, &c->d_gpool, pb)) return e;
         }
+        // Cells the step's boundary statements read at level 0, where the
+        // level swap of the interior transposes must move values: stresses
+        // from every stepping phase, velocities from the velocity phase (the
+        // stress phase's transposes run after the velocity levels swapped).
+        {
+            std::vector<int64_t> tc, vc;
+            const int64_t so[3] = {g.sx, g.sy, 1};
+            for (int ph = 2; ph < 5; ++ph)
+                for (const BcLevel& lv : c->levels[ph])
+                    for (int di = lv.first; di < lv.first + lv.count; ++di) {
+                        const BcDesc& d = c->h_desc[di];
+                        const int o0 = d.axis == 0 ? 1 : 0, o1 = d.axis == 2 ? 1 : 2;
+                        for (int k = 0; k < d.nterms; ++k) {
+                            if (d.t[k].fl & 1) continue;  // level 1
+                            const int f = d.t[k].fl >> 1;
+                            if (f < 3 && ph != 3) continue;
+                            std::vector<int64_t>& out = f >= 3 ? tc : vc;
+                            for (int p = 0; p < d.npts; ++p) {
+                                const int64_t cell = (int64_t)(d.lo0 + p / d.n1) * so[o0] +
+                                                     (int64_t)(d.lo1 + p % d.n1) * so[o1] + d.disp + d.t[k].disp;
+                                out.push_back(cell * 16 + f);
+                            }
+                        }
+                    }
+            for (std::vector<int64_t>* v : {&tc, &vc}) {
+                std::sort(v->begin(), v->end());
+                v->erase(std::unique(v->begin(), v->end()), v->end());
+            }
+            if (tc.size() > (size_t)INT32_MAX || vc.size() > (size_t)INT32_MAX)
+                return fail(OPESCI_EUNSUPPORTED, "too many boundary cells for the adjoint");
+            if (int e = upload(&c->d_tcells, tc.data(), tc.size(), c->stream)) return e;
+            if (int e = upload(&c->d_vcells, vc.data(), vc.size(), c->stream)) return e;
+            c->n_tcells = (int)tc.size();
+            c->n_vcells = (int)vc.size();
+        }
         int64_t off = 0;
         for (int ph = 2; ph < 5; ++ph) {
             c->log_off[ph].clear();
@@ -885,11 +926,11 @@ static int step_adjoint(OpesciContext* c, int n, void* fp, char* log) {
                                         c->d_obs + (size_t)n * c->nrec * 3, c->stream));
     if (int e = levels(3)) return e;
     OPESCI_HIP_TRY(launch_adj_velocity(c->g, c->real_bytes, fp, c->aptr, c->mptr, c->gptr, c->coefs(), c->sptr,
-                                       c->cfg, t0, t1, c->stream));
+                                       AdjCells{c->d_vcells, c->n_vcells}, c->cfg, t0, t1, c->stream));
     if (int e = levels(2)) return e;
     if (int e = levels(4)) return e;
     OPESCI_HIP_TRY(launch_adj_stress(c->g, c->real_bytes, fp, c->aptr, c->mptr, c->gptr, c->coefs(), c->sptr,
-                                     c->cfg, t0, t1, c->stream));
+                                     AdjCells{c->d_tcells, c->n_tcells}, c->cfg, t0, t1, c->stream));
     return 0;
 }
 
