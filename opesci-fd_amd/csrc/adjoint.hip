@@ -578,8 +578,11 @@ static dim3 rim_grid(const Geom& g) {
     return dim3((unsigned)((slab + 255) / 256), 6);
 }
 
-// bar F(t0)[c] += bar F(t1)[c]; bar F(t1)[c] = 0 for a list of distinct
-// cells, each entry (offset << 4) | field.
+// After the two levels' arrays swapped places: for a list of distinct cells,
+// each entry (offset << 5) | exchange << 4 | field, either
+// bar F(t0)[c] += bar F(t1)[c], bar F(t1)[c] = 0 (interior cells read at
+// level 0) or an exchange of the two values (non-interior cells, which keep
+// their values in their physical arrays).
 template <typename R>
 __global__ __launch_bounds__(256) void adj_move_cells(AdjPtrs<R> B, const int64_t* __restrict__ cells, int n, int t0,
                                                       int t1) {
@@ -587,21 +590,31 @@ __global__ __launch_bounds__(256) void adj_move_cells(AdjPtrs<R> B, const int64_
     if (i >= n) return;
     const int64_t e = cells[i];
     const int f = (int)(e & 15);
-    const int64_t off = e >> 4;
+    const int64_t off = e >> 5;
     R* a = B.f[f][t0] + off;
     R* b = B.f[f][t1] + off;
-    *a += *b;
-    *b = (R)0;
+    const R va = *a, vb = *b;
+    if (e & 16) {
+        *a = vb;
+        *b = va;
+    } else {
+        *a = va + vb;
+        *b = (R)0;
+    }
 }
 
-// The identity part of an update's transpose, bar F(t0) += bar F(t1) and
-// bar F(t1) = 0 for the fields [f0, f1), without touching the arrays: at
-// this point bar F(t0) is zero everywhere except at the cells that
-// boundary statements of the step read at level 0 (their transposes ran
-// first) and at cells no statement or update ever writes (whose adjoint
-// values nothing reads).  So the two levels' arrays swap places in the
-// host-side pointer image, and only the listed cells move.  Every later
-// launch, of this step and of the steps before it, reads the swapped image.
+// The identity part of an update's transpose, bar F(t0) += bar F(t1) over
+// the interior and bar F(t1) = 0 there, for the fields [f0, f1), without
+// touching the arrays: at this point bar F(t0) is zero in the interior
+// except at the cells that boundary statements of the step read at level
+// 0 (their transposes ran first).  So the two levels' arrays swap places in
+// the host-side pointer image; those cells move, and the non-interior
+// cells that boundary statements read take back their physical arrays'
+// values (the remaining non-interior cells are never read as adjoint
+// values: a written cell's is consumed and zeroed by its statement's
+// transpose before this point, an unwritten cell's is never used).  Every
+// later launch, of this step and of the steps before it, reads the swapped
+// image.
 template <typename R>
 static hipError_t adj_swap_levels(AdjPtrs<R>& B, int f0, int f1, const AdjCells& cells, int t0, int t1,
                                   hipStream_t s) {

@@ -246,8 +246,9 @@ hipError_t launch_bc_coef(const Geom& g, int real_bytes, const void* media, cons
 // the simple gather kernels, as does cfg.variant == 1).  bar is the host-side
 // image of the adjoint fields' level pointers: with the scratch path the
 // launch swaps the two levels of the updated fields in it (the identity part
-// of the transpose), moving only `cells` (distinct (offset << 4) | field
-// entries: the cells the step's boundary statements read at level 0).
+// of the transpose) and fixes up only `cells` (distinct entries
+// (offset << 5) | exchange << 4 | field, built from the cells the step's
+// boundary statements read).
 struct AdjCells {
     const int64_t* d = nullptr;
     int n = 0;

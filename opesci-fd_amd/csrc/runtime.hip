@@ -59,8 +59,8 @@ struct OpesciContext {
     void* sbase[6] = {};              // adjoint scratch (weighted adjoint fields)
     void* sptr[6] = {};
     void* d_gpool = nullptr;
-    int64_t* d_tcells = nullptr;      // adjoint: cells read at level 0 by boundary statements (stresses)
-    int64_t* d_vcells = nullptr;      // ... (velocities, velocity phase)
+    int64_t* d_tcells = nullptr;      // adjoint level-swap cells of the stresses
+    int64_t* d_vcells = nullptr;      // ... and of the velocities
     int n_tcells = 0, n_vcells = 0;
     void* d_log = nullptr;            // undo log of one step's boundary levels
     std::vector<int64_t> log_off[5];  // per phase and level
@@ -802,27 +802,47 @@ static int adj_alloc(OpesciContext* c, int ckpt, size_t seg_bytes, int log_steps
             const size_t pb = ((size_t)c->pool_elems * rb + 15) & ~size_t(15);
             if (int e = alloc_zero(c, &c->d_gpool, pb)) return e;
         }
-        // Cells the step's boundary statements read at level 0, where the
-        // level swap of the interior transposes must move values: stresses
-        // from every stepping phase, velocities from the velocity phase (the
-        // stress phase's transposes run after the velocity levels swapped).
+        // Cells where the level swap of an interior transpose must act
+        // (entries (offset << 5) | exchange << 4 | field):
+        //  * interior cells that the step's boundary statements read at
+        //    level 0 hold adjoint values of level t0 at the swap: they move
+        //    into the swapped-in level t0 (add, then zero);
+        //  * non-interior cells that boundary statements read at either
+        //    level keep their values in their physical arrays, as the point
+        //    update leaves every non-interior cell (exchange).  A read of a
+        //    level-t1 cell that a later statement of the step writes sees
+        //    the value of two steps before, which lives in the same array.
+        // Stresses: statements of every stepping phase (their transposes all
+        // run before the stress swap).  Velocities: the velocity phase; the
+        // stress phase may read velocities at level t0 only (its transposes
+        // run after the velocity swap, into the swapped-in level t0).
         {
             std::vector<int64_t> tc, vc;
             const int64_t so[3] = {g.sx, g.sy, 1};
+            const int m = g.m;
             for (int ph = 2; ph < 5; ++ph)
                 for (const BcLevel& lv : c->levels[ph])
                     for (int di = lv.first; di < lv.first + lv.count; ++di) {
                         const BcDesc& d = c->h_desc[di];
                         const int o0 = d.axis == 0 ? 1 : 0, o1 = d.axis == 2 ? 1 : 2;
                         for (int k = 0; k < d.nterms; ++k) {
-                            if (d.t[k].fl & 1) continue;  // level 1
-                            const int f = d.t[k].fl >> 1;
-                            if (f < 3 && ph != 3) continue;
+                            const int f = d.t[k].fl >> 1, lvl = d.t[k].fl & 1;
+                            if (f < 3 && ph != 3) {
+                                if (lvl == 1)
+                                    return fail(OPESCI_EUNSUPPORTED,
+                                                "gradient: a stress boundary statement reads velocities at t1");
+                                continue;
+                            }
                             std::vector<int64_t>& out = f >= 3 ? tc : vc;
                             for (int p = 0; p < d.npts; ++p) {
                                 const int64_t cell = (int64_t)(d.lo0 + p / d.n1) * so[o0] +
                                                      (int64_t)(d.lo1 + p % d.n1) * so[o1] + d.disp + d.t[k].disp;
-                                out.push_back(cell * 16 + f);
+                                const int x = (int)(cell / g.sx), y = (int)((cell % g.sx) / g.sy),
+                                          z = (int)(cell % g.sy);
+                                const bool inner = x >= m && x < g.nx - m && y >= m && y < g.ny - m && z >= m &&
+                                                   z < g.nz - m;
+                                if (inner && lvl == 1) continue;   // moved by the swap itself
+                                out.push_back(cell * 32 + (inner ? 0 : 16) + f);
                             }
                         }
                     }

@@ -188,12 +188,16 @@ def test_gradient_matches_fp64_cpu_adjoint(order):
         assert err <= 1e-9 * scale, (k, err, scale)
 
 
-@pytest.mark.parametrize("order", [2, 4, 8])
-def test_fast_interior_transposes_equal_simple_kernels(order):
-    """The sweep-based interior transposes and the slot-based segment
-    scheme (default) against the per-cell gather kernels and full-state
-    segments (launch variant 1), FP64, on a box of uneven sides: every
-    gradient entry to 1e-12 of the array's largest."""
+@pytest.mark.parametrize("order,initial,every", [(2, 'zero', 4), (4, 'zero', 4), (8, 'zero', 4),
+                                                  (4, 'eigenmode', 5), (2, 'eigenmode', 4)])
+def test_fast_interior_transposes_equal_simple_kernels(order, initial, every):
+    """The sweep-based interior transposes, the level swaps and the
+    slot-based segment scheme (default) against the per-cell gather kernels,
+    explicit level moves and full-state segments (launch variant 1), FP64,
+    on a box of uneven sides, from rest and from the eigenmode (whose
+    initial boundary values sit in cells the steps never write), with even
+    and odd checkpoint intervals: every gradient entry to 1e-12 of the
+    array's largest."""
     import opesci
     from opesci.media import smooth_random_media
     from opesci.sources import ricker
@@ -203,7 +207,7 @@ def test_fast_interior_transposes_equal_simple_kernels(order):
     rho, vp, vs = smooth_random_media(dims, seed=9, sigma=2.0)
     dt = 0.4 * 0.495 / max(n) / float(vp.max())
     g = opesci.eigenwave3d((1.0, 1.0, 1.0), n, dt, dt * (steps + 0.5), accuracy_order=[2, order, order, order],
-                           double=True, media_arrays=(rho, vp, vs), initial='zero', o_converge=False)
+                           double=True, media_arrays=(rho, vp, vs), initial=initial, o_converge=False)
     m = order // 2
     g.set_source(ricker(20.0, dt, steps, t0=6 * dt) * 100.0, index=(m + 4, m + 6, m + 8))
     g.set_receivers(index=[(m + i, m + j, m + 1) for i in range(0, sizes[0], 3) for j in range(0, sizes[1], 3)])
@@ -214,7 +218,7 @@ def test_fast_interior_transposes_equal_simple_kernels(order):
         out = []
         for variant in (0, 1):
             ctx.configure(variant)
-            J = ctx.gradient(obs, checkpoint_every=4)
+            J = ctx.gradient(obs, checkpoint_every=every)
             out.append((J, {k: ctx.download_gradient(k) for k in EFFECTIVE}))
     assert out[0][0] == pytest.approx(out[1][0], rel=1e-13)
     for k in EFFECTIVE:
