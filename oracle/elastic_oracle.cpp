@@ -6,8 +6,9 @@
  * (opesci/staggeredgrid.py, opesci/regulargrid.py, opesci/fields.py,
  * opesci/templates/staggered3d_tmpl.py, driven by tests/eigenwave3d.py).
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
- * load this library, as the checker or as the timed CPU baseline.  The
- * product (opesci-fd_amd/) never links, imports or calls it.
+ * load this library, as the reference the tests compare with or as the
+ * timed CPU baseline.  The product (opesci-fd_amd/) never links, imports or
+ * calls it.
  *
  * Two conventions are restated:
  *   current  -- the generator as it stands: constants folded into float

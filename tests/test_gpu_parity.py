@@ -2,8 +2,8 @@
 (same arithmetic contract) and the oracle (independent restatement of the
 reference), plus the reference's own known answer.
 
-Tolerances (stated here, per north star: reference L2-vs-analytic within 1e-5
-relative):
+Tolerances (the L2 error against the analytic solution must match the
+reference's to within 1e-5 relative):
   * fields vs plan interpreter: FP64 1e-12, FP32 2e-6 of the field maximum
   * L2 vs oracle: FP64 1e-10 relative, FP32 1e-5 relative
   * test3d known answer: 1e-5 relative against the oracle with the same
