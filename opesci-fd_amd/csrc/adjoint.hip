@@ -815,6 +815,34 @@ hipError_t launch_src_cells(const Geom& g, int real_bytes, void* fields, int t, 
     return hipGetLastError();
 }
 
+// D.f[field][td][offset] = S.f[field][ts][offset] for a list of distinct
+// cells, each entry (offset << 4) | field.
+template <typename R>
+__global__ __launch_bounds__(256) void copy_cells_kernel(FieldPtrs<R> D, int td, FieldPtrs<R> S, int ts,
+                                                         const int64_t* __restrict__ cells, int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int64_t e = cells[i];
+    const int f = (int)(e & 15);
+    const int64_t off = e >> 4;
+    D.f[f][td][off] = S.f[f][ts][off];
+}
+
+hipError_t launch_copy_cells(int real_bytes, void* dst, int td, void* src, int ts, const int64_t* cells, int n,
+                             hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const dim3 grid((unsigned)((n + 255) / 256));
+    if (real_bytes == 8)
+        hipLaunchKernelGGL((copy_cells_kernel<double>), grid, dim3(256), 0, s,
+                           *static_cast<const FieldPtrs<double>*>(dst), td,
+                           *static_cast<const FieldPtrs<double>*>(src), ts, cells, n);
+    else
+        hipLaunchKernelGGL((copy_cells_kernel<float>), grid, dim3(256), 0, s,
+                           *static_cast<const FieldPtrs<float>*>(dst), td, *static_cast<const FieldPtrs<float>*>(src),
+                           ts, cells, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_set_step(int* step, int value, hipStream_t s) {
     hipLaunchKernelGGL(set_step_kernel, dim3(1), dim3(1), 0, s, step, value);
     return hipGetLastError();

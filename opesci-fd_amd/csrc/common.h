@@ -269,6 +269,10 @@ hipError_t launch_adj_coef_media(const Geom& g, int real_bytes, const void* medi
 hipError_t launch_adj_receivers(const Geom& g, int real_bytes, void* fields, void* bar, int t1, int nrec,
                                 const int32_t* rec, const float* obs, hipStream_t s);
 hipError_t launch_set_step(int* step, int value, hipStream_t s);
+// dst level td := src level ts at a list of distinct cells ((offset << 4) |
+// field), both FieldPtrs images of the compute type
+hipError_t launch_copy_cells(int real_bytes, void* dst, int td, void* src, int ts, const int64_t* cells, int n,
+                             hipStream_t s);
 // mode 0: buf[0..2] = Txx, Tyy, Tzz at (x, y, z) of level t; mode 1: the reverse
 hipError_t launch_src_cells(const Geom& g, int real_bytes, void* fields, int t, int x, int y, int z, void* buf,
                             int mode, hipStream_t s);

@@ -61,7 +61,8 @@ struct OpesciContext {
     void* d_gpool = nullptr;
     int64_t* d_tcells = nullptr;      // adjoint level-swap cells of the stresses
     int64_t* d_vcells = nullptr;      // ... and of the velocities
-    int n_tcells = 0, n_vcells = 0;
+    int64_t* d_scells = nullptr;      // cells boundary statements read at level t1 outside the interior
+    int n_tcells = 0, n_vcells = 0, n_scells = 0;
     void* d_log = nullptr;            // undo log of one step's boundary levels
     std::vector<int64_t> log_off[5];  // per phase and level
     char* d_ckpt = nullptr;           // checkpoint states, 18 field levels each
@@ -388,7 +389,7 @@ void destroy(OpesciContext* c) {
     void* ptrs[] = {c->d_desc,  c->d_code,    c->d_sol,  c->d_partial, c->d_stage, c->d_rec,
                     c->d_trace, c->d_wavelet, c->d_step, c->d_pool,    c->d_dterms, c->d_gpool,
                     c->d_log,   c->d_ckpt,    c->d_seg,  c->d_obs,     c->d_srclog, c->d_tcells,
-                    c->d_vcells};
+                    c->d_vcells, c->d_scells};
     for (int f = 0; f < NF; ++f)
         for (int l = 0; l < 2; ++l)
             if (c->abase[f][l]) (void)hipFree(c->abase[f][l]);
@@ -817,7 +818,7 @@ static int adj_alloc(OpesciContext* c, int ckpt, size_t seg_bytes, int log_steps
         // stress phase may read velocities at level t0 only (its transposes
         // run after the velocity swap, into the swapped-in level t0).
         {
-            std::vector<int64_t> tc, vc;
+            std::vector<int64_t> tc, vc, sc;
             const int64_t so[3] = {g.sx, g.sy, 1};
             const int m = g.m;
             for (int ph = 2; ph < 5; ++ph)
@@ -827,13 +828,12 @@ static int adj_alloc(OpesciContext* c, int ckpt, size_t seg_bytes, int log_steps
                         const int o0 = d.axis == 0 ? 1 : 0, o1 = d.axis == 2 ? 1 : 2;
                         for (int k = 0; k < d.nterms; ++k) {
                             const int f = d.t[k].fl >> 1, lvl = d.t[k].fl & 1;
-                            if (f < 3 && ph != 3) {
-                                if (lvl == 1)
-                                    return fail(OPESCI_EUNSUPPORTED,
-                                                "gradient: a stress boundary statement reads velocities at t1");
-                                continue;
-                            }
-                            std::vector<int64_t>& out = f >= 3 ? tc : vc;
+                            if (f < 3 && ph != 3 && lvl == 1)
+                                return fail(OPESCI_EUNSUPPORTED,
+                                            "gradient: a stress boundary statement reads velocities at t1");
+                            // velocities read by the stress phase (level t0) need
+                            // no swap fix-up: their transposes run after the swap
+                            std::vector<int64_t>* out = f >= 3 ? &tc : (ph == 3 ? &vc : nullptr);
                             for (int p = 0; p < d.npts; ++p) {
                                 const int64_t cell = (int64_t)(d.lo0 + p / d.n1) * so[o0] +
                                                      (int64_t)(d.lo1 + p % d.n1) * so[o1] + d.disp + d.t[k].disp;
@@ -841,21 +841,24 @@ static int adj_alloc(OpesciContext* c, int ckpt, size_t seg_bytes, int log_steps
                                           z = (int)(cell % g.sy);
                                 const bool inner = x >= m && x < g.nx - m && y >= m && y < g.ny - m && z >= m &&
                                                    z < g.nz - m;
-                                if (inner && lvl == 1) continue;   // moved by the swap itself
-                                out.push_back(cell * 32 + (inner ? 0 : 16) + f);
+                                if (!inner && lvl == 1) sc.push_back(cell * 16 + f);
+                                if (!out || (inner && lvl == 1)) continue;   // (inner: moved by the swap)
+                                out->push_back(cell * 32 + (inner ? 0 : 16) + f);
                             }
                         }
                     }
-            for (std::vector<int64_t>* v : {&tc, &vc}) {
+            for (std::vector<int64_t>* v : {&tc, &vc, &sc}) {
                 std::sort(v->begin(), v->end());
                 v->erase(std::unique(v->begin(), v->end()), v->end());
             }
-            if (tc.size() > (size_t)INT32_MAX || vc.size() > (size_t)INT32_MAX)
+            if (tc.size() > (size_t)INT32_MAX || vc.size() > (size_t)INT32_MAX || sc.size() > (size_t)INT32_MAX)
                 return fail(OPESCI_EUNSUPPORTED, "too many boundary cells for the adjoint");
             if (int e = upload(&c->d_tcells, tc.data(), tc.size(), c->stream)) return e;
             if (int e = upload(&c->d_vcells, vc.data(), vc.size(), c->stream)) return e;
+            if (int e = upload(&c->d_scells, sc.data(), sc.size(), c->stream)) return e;
             c->n_tcells = (int)tc.size();
             c->n_vcells = (int)vc.size();
+            c->n_scells = (int)sc.size();
         }
         int64_t off = 0;
         for (int ph = 2; ph < 5; ++ph) {
@@ -906,22 +909,23 @@ static int state_copy(OpesciContext* c, char* slot, bool save) {
     return 0;
 }
 
-// one forward step up to its velocity boundary levels, with every boundary
-// level's overwritten values logged into `log` (one step's log)
-static int step_logged(OpesciContext* c, int t0, char* log) {
+// one forward step up to its velocity boundary levels on the fields of the
+// FieldPtrs image `fp`, with every boundary level's overwritten values
+// logged into `log` (one step's log)
+static int step_logged(OpesciContext* c, void* fp, int t0, char* log) {
     const int t1 = 1 - t0;
     auto levels = [&](int ph) -> int {
         for (size_t L = 0; L < c->levels[ph].size(); ++L)
-            OPESCI_HIP_TRY(launch_bc_logged(c->g, c->real_bytes, c->fptr, c->d_desc, c->d_pool, c->levels[ph][L],
+            OPESCI_HIP_TRY(launch_bc_logged(c->g, c->real_bytes, fp, c->d_desc, c->d_pool, c->levels[ph][L],
                                             log + c->log_off[ph][L] * c->real_bytes, t0, t1, 0, c->stream));
         return 0;
     };
-    OPESCI_HIP_TRY(launch_stress(c->g, c->real_bytes, c->storage, c->het, c->fptr, c->mptr, c->coefs(), t0, t1,
-                                 c->cfg, c->stream));
+    OPESCI_HIP_TRY(launch_stress(c->g, c->real_bytes, c->storage, c->het, fp, c->mptr, c->coefs(), t0, t1, c->cfg,
+                                 c->stream));
     if (int e = levels(4)) return e;
     if (int e = levels(2)) return e;
-    OPESCI_HIP_TRY(launch_velocity(c->g, c->real_bytes, c->storage, c->het, c->fptr, c->mptr, c->coefs(), t0, t1,
-                                   c->cfg, c->stream));
+    OPESCI_HIP_TRY(launch_velocity(c->g, c->real_bytes, c->storage, c->het, fp, c->mptr, c->coefs(), t0, t1, c->cfg,
+                                   c->stream));
     // no receivers / source here: the adjoint reads the state the velocity
     // boundary left (the source, added after it, is a constant forcing with
     // an identity adjoint, and must not show in the stresses it reads)
@@ -954,14 +958,6 @@ static int step_adjoint(OpesciContext* c, int n, void* fp, char* log) {
     return 0;
 }
 
-// copy level `lvl` of the nine fields to a slot of nine arrays
-static int level_save(OpesciContext* c, int lvl, char* slot) {
-    for (int f = 0; f < NF; ++f)
-        OPESCI_HIP_TRY(hipMemcpyAsync(slot + (size_t)f * c->field_bytes, c->base[f][lvl], c->field_bytes,
-                                      hipMemcpyDeviceToDevice, c->stream));
-    return 0;
-}
-
 // every stepping boundary statement writes level t1 (so a step leaves level t0 as it found it)
 static bool steps_keep_t0(const OpesciContext* c) {
     for (const OpesciBcStmt& st : c->stmts)
@@ -988,9 +984,9 @@ int opesci_ctx_gradient(OpesciContext* c, const float* observed, int64_t count, 
     const int nck = (N + every - 1) / every;
     const size_t state = (size_t)2 * NF * c->field_bytes;
     // Segment scheme.  keep_t0: the segment's forward pass logs every step and
-    // keeps its end state's level t1 (nine arrays); the reverse step runs on
-    // pointers into those slots (level t0 = the previous step's slot with the
-    // source cells set to their injected values, or the checkpoint).
+    // writes its level t1 (nine arrays) into a slot of its own; the reverse
+    // step runs on pointers into those slots (level t0 = the previous step's
+    // slot, or the checkpoint).
     // Otherwise (and with the simple launch variant 1, which selects the
     // reference forms throughout): full states before each step, and the step
     // recomputed with its log before its adjoint.
@@ -1027,56 +1023,83 @@ int opesci_ctx_gradient(OpesciContext* c, const float* observed, int64_t count, 
     const size_t rb = c->real_bytes, zo = (size_t)c->g.zoff * rb;
     const int* si = c->src_idx;
     char* srclog = static_cast<char*>(c->d_srclog);
+    // keep_t0: slot k of a segment holds level t1 of step k0 + k, written in
+    // place by the logged forward (the step reads level t0 from the slot
+    // before it, or from the checkpoint).  Cells no step writes hold a
+    // constant per level, so a slot is filled once with the whole level of
+    // its parity (slot_par; once per gradient when `every` is even).  Cells
+    // a boundary statement reads at level t1 before a later statement of the
+    // step writes them hold the value of two steps before, copied in from
+    // the slot two back (or the checkpoint) ahead of the step.
+    std::vector<int> slot_par(keep_t0 ? every : 0, -1);
+    const size_t fb = c->field_bytes;
     for (int seg = nck - 1; seg >= 0; --seg) {
         const int k0 = seg * every, k1 = std::min(N, k0 + every);
         char* ck = c->d_ckpt + (size_t)seg * state;
-        if (int e = state_copy(c, ck, false)) return e;
         OPESCI_HIP_TRY(launch_set_step(c->d_step, k0, c->stream));
         if (!keep_t0) {
+            if (int e = state_copy(c, ck, false)) return e;
             for (int n = k0; n < k1; ++n) {
                 if (int e = state_copy(c, c->d_seg + (size_t)(n - k0) * state, true)) return e;
                 if (int e = step_once(c, n % 2)) return e;
             }
             for (int n = k1 - 1; n >= k0; --n) {
                 if (int e = state_copy(c, c->d_seg + (size_t)(n - k0) * state, false)) return e;
-                if (int e = step_logged(c, n % 2, static_cast<char*>(c->d_log))) return e;
+                if (int e = step_logged(c, c->fptr, n % 2, static_cast<char*>(c->d_log))) return e;
                 if (int e = step_adjoint(c, n, c->fptr, static_cast<char*>(c->d_log))) return e;
             }
             continue;
         }
+        // level pointers of step n: t1 in slot n - k0, t0 in the slot before
+        // (or the checkpoint)
+        auto slot_image = [&](int n, void* fp[NF][2]) {
+            const int k = n - k0, t0 = n % 2, t1 = 1 - t0;
+            for (int f = 0; f < NF; ++f) {
+                fp[f][t1] = c->d_seg + (size_t)k * slot + (size_t)f * fb + zo;
+                fp[f][t0] = k > 0 ? c->d_seg + (size_t)(k - 1) * slot + (size_t)f * fb + zo
+                                  : ck + (size_t)(2 * f + t0) * fb + zo;
+            }
+        };
         for (int n = k0; n < k1; ++n) {
-            const int t1 = 1 - n % 2;
-            char* log = static_cast<char*>(c->d_log) + (size_t)(n - k0) * c->log_step * rb;
-            char* sl = srclog + (size_t)6 * (n - k0) * rb;
-            if (int e = step_logged(c, n % 2, log)) return e;
-            if (int e = level_save(c, t1, c->d_seg + (size_t)(n - k0) * slot)) return e;
+            const int k = n - k0, t0 = n % 2, t1 = 1 - t0;
+            char* sk = c->d_seg + (size_t)k * slot;
+            if (slot_par[k] != t1) {
+                for (int f = 0; f < NF; ++f)
+                    OPESCI_HIP_TRY(hipMemcpyAsync(sk + (size_t)f * fb, ck + (size_t)(2 * f + t1) * fb, fb,
+                                                  hipMemcpyDeviceToDevice, c->stream));
+                slot_par[k] = t1;
+            }
+            void* fp[NF][2];
+            slot_image(n, fp);
+            void* fs[NF][2];
+            for (int f = 0; f < NF; ++f)
+                fs[f][0] = fs[f][1] = k >= 2 ? c->d_seg + (size_t)(k - 2) * slot + (size_t)f * fb + zo
+                                             : ck + (size_t)(2 * f + t1) * fb + zo;
+            OPESCI_HIP_TRY(launch_copy_cells(c->real_bytes, fp, t1, fs, t1, c->d_scells, c->n_scells, c->stream));
+            char* log = static_cast<char*>(c->d_log) + (size_t)k * c->log_step * rb;
+            char* sl = srclog + (size_t)6 * k * rb;
+            if (int e = step_logged(c, fp, t0, log)) return e;
             if (c->src_ntr)
-                OPESCI_HIP_TRY(launch_src_cells(c->g, c->real_bytes, c->fptr, t1, si[0], si[1], si[2], sl, 0,
-                                                c->stream));
+                OPESCI_HIP_TRY(launch_src_cells(c->g, c->real_bytes, fp, t1, si[0], si[1], si[2], sl, 0, c->stream));
             // source and step counter only (the traces are already recorded)
-            OPESCI_HIP_TRY(launch_record_source(c->g, c->real_bytes, c->storage, c->fptr, t1, 0, c->d_rec,
-                                                c->d_trace, c->src_ntr, c->src_idx, c->d_wavelet, c->d_step,
-                                                c->desc.ntsteps, c->stream));
+            OPESCI_HIP_TRY(launch_record_source(c->g, c->real_bytes, c->storage, fp, t1, 0, c->d_rec, c->d_trace,
+                                                c->src_ntr, c->src_idx, c->d_wavelet, c->d_step, c->desc.ntsteps,
+                                                c->stream));
             if (c->src_ntr)
-                OPESCI_HIP_TRY(launch_src_cells(c->g, c->real_bytes, c->fptr, t1, si[0], si[1], si[2], sl + 3 * rb, 0,
+                OPESCI_HIP_TRY(launch_src_cells(c->g, c->real_bytes, fp, t1, si[0], si[1], si[2], sl + 3 * rb, 0,
                                                 c->stream));
         }
         for (int n = k1 - 1; n >= k0; --n) {
-            const int t0 = n % 2, t1 = 1 - t0;
+            const int t1 = 1 - n % 2;
             void* fp[NF][2];
-            for (int f = 0; f < NF; ++f) {
-                fp[f][t1] = c->d_seg + (size_t)(n - k0) * slot + (size_t)f * c->field_bytes + zo;
-                fp[f][t0] = n > k0 ? c->d_seg + (size_t)(n - 1 - k0) * slot + (size_t)f * c->field_bytes + zo
-                                   : ck + (size_t)(2 * f + t0) * c->field_bytes + zo;
-            }
-            char* prev = n > k0 ? srclog + (size_t)6 * (n - 1 - k0) * rb : nullptr;
-            if (n > k0 && c->src_ntr)
-                OPESCI_HIP_TRY(launch_src_cells(c->g, c->real_bytes, fp, t0, si[0], si[1], si[2], prev + 3 * rb, 1,
-                                                c->stream));
+            slot_image(n, fp);
+            // the slots hold the injected source (what the next step read);
+            // the adjoint of step n reads the state before the injection
+            if (c->src_ntr)
+                OPESCI_HIP_TRY(launch_src_cells(c->g, c->real_bytes, fp, t1, si[0], si[1], si[2],
+                                                srclog + (size_t)6 * (n - k0) * rb, 1, c->stream));
             char* log = static_cast<char*>(c->d_log) + (size_t)(n - k0) * c->log_step * rb;
             if (int e = step_adjoint(c, n, fp, log)) return e;
-            if (n > k0 && c->src_ntr)
-                OPESCI_HIP_TRY(launch_src_cells(c->g, c->real_bytes, fp, t0, si[0], si[1], si[2], prev, 1, c->stream));
         }
     }
     // media-dependent boundary coefficients -> media cells
