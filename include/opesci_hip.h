@@ -282,6 +282,11 @@ int opesci_ctx_download_gradient(OpesciContext* ctx, int32_t media, void* host);
  * which effective media computed elsewhere (e.g. by autograd) drive the
  * propagator.  No reference counterpart. */
 int opesci_ctx_upload_media(OpesciContext* ctx, int32_t media, const void* host);
+/* One effective-media array as the device holds it (dense [dim1][dim2][dim3],
+ * compute type): the read path's beta1..3 / lambda / mu / mu12.. built by
+ * the media kernels from rho, Vp, Vs, the arrays the reference computes in
+ * StaggeredGrid.read_data (opesci/staggeredgrid.py:572-597). */
+int opesci_ctx_download_media(OpesciContext* ctx, int32_t media, void* host);
 
 typedef struct OpesciBatch OpesciBatch;
 int opesci_batch_create(OpesciContext** ctxs, int32_t n, OpesciBatch** out);

@@ -588,6 +588,12 @@ int ctx_upload(OpesciContext* c, int f, int level, const void* host) {
     const size_t dense = (size_t)c->g.nx * c->g.ny * c->g.nz;
     OPESCI_HIP_TRY(hipMemcpyAsync(c->d_stage, host, dense * c->real_bytes, hipMemcpyHostToDevice, c->stream));
     OPESCI_HIP_TRY(launch_pack(c->g, c->real_bytes, c->storage, c->d_stage, c->fptr[f][level], c->stream));
+    if (!c->initialised) {
+        // a state built by uploads starts at step 0: the source and receiver
+        // step counter restarts with it (a gradient leaves it elsewhere)
+        if (c->d_step) OPESCI_HIP_TRY(launch_zero(c->d_step, 256, c->stream));
+        c->steps = 0;
+    }
     OPESCI_HIP_TRY(hipStreamSynchronize(c->stream));
     c->initialised = true;
     return 0;
@@ -1117,6 +1123,16 @@ int opesci_ctx_download_gradient(OpesciContext* c, int32_t media, void* host) {
     if (!c->gbase[media]) return fail(OPESCI_ESTATE, "no gradient: call opesci_ctx_gradient first");
     const size_t dense = (size_t)c->g.nx * c->g.ny * c->g.nz;
     OPESCI_HIP_TRY(launch_unpack(c->g, c->real_bytes, OPESCI_STORE_NATIVE, c->gptr[media], c->d_stage, c->stream));
+    OPESCI_HIP_TRY(hipMemcpyAsync(host, c->d_stage, dense * c->real_bytes, hipMemcpyDeviceToHost, c->stream));
+    OPESCI_HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int opesci_ctx_download_media(OpesciContext* c, int32_t media, void* host) {
+    if (!c || !host || media < 0 || media >= OPESCI_M_COUNT) return fail(OPESCI_EINVAL, "bad arguments");
+    if (!c->het || !c->mptr[media]) return fail(OPESCI_EUNSUPPORTED, "media arrays exist on the read path only");
+    const size_t dense = (size_t)c->g.nx * c->g.ny * c->g.nz;
+    OPESCI_HIP_TRY(launch_unpack(c->g, c->real_bytes, OPESCI_STORE_NATIVE, c->mptr[media], c->d_stage, c->stream));
     OPESCI_HIP_TRY(hipMemcpyAsync(host, c->d_stage, dense * c->real_bytes, hipMemcpyDeviceToHost, c->stream));
     OPESCI_HIP_TRY(hipStreamSynchronize(c->stream));
     return 0;

@@ -92,6 +92,7 @@ def load(path=None):
         lib.opesci_ctx_gradient.argtypes = [_vp, _vp, ctypes.c_int64, _i32, ctypes.POINTER(_f64)]
         lib.opesci_ctx_download_gradient.argtypes = [_vp, _i32, _vp]
         lib.opesci_ctx_upload_media.argtypes = [_vp, _i32, _vp]
+        lib.opesci_ctx_download_media.argtypes = [_vp, _i32, _vp]
         lib.opesci_model_load.argtypes = [ctypes.POINTER(OpesciModelDesc)]
         lib.opesci_execute.argtypes = [_vp, _vp]
         lib.opesci_convergence.argtypes = [_vp, _vp]
@@ -301,6 +302,13 @@ class Context(object):
         mid = MEDIA_IDS[media] if isinstance(media, str) else int(media)
         a = np.zeros(self.plan.dims, dtype=self.dtype)
         _check(self.lib.opesci_ctx_download_gradient(self.h, mid, a.ctypes.data), 'opesci_ctx_download_gradient')
+        return a
+
+    def download_media(self, media):
+        """One media array as the device holds it (read path), dense (dim1, dim2, dim3)."""
+        mid = MEDIA_IDS[media] if isinstance(media, str) else int(media)
+        a = np.zeros(self.plan.dims, dtype=self.dtype)
+        _check(self.lib.opesci_ctx_download_media(self.h, mid, a.ctypes.data), 'opesci_ctx_download_media')
         return a
 
     def upload_media(self, media, arr):

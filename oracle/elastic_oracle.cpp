@@ -68,6 +68,8 @@ int oracle_run(const OracleParams* p, void* fields_out[9], double l2_f64[9],
 int oracle_fd_weights(int half_order, double* w);
 int oracle_resample(const float* src, int snt, float dt, double sdt, float* out, int cap);
 int oracle_norms(const OracleParams* p, void* const fields_in[9], double l2_f64[9], double l2_ref[9]);
+int oracle_run_from(const OracleParams* p, void* const state_in[9], void* fields_out[9]);
+int oracle_media(const OracleParams* p, double* out[9]);
 }
 
 namespace {
@@ -107,6 +109,7 @@ struct Sim {
     size_t vol;
     bool levander;
     bool no_tnn = false;
+    bool given_state = false;  /* level 0 supplied by the caller: no initialise / initial BCs */
     R dx[3];               /* const real_t dx1..dx3 */
     R dt;
     double dxd[3], dtd;
@@ -884,9 +887,11 @@ struct Sim {
         int t0 = 0, t1 = 0;
 #pragma omp parallel
         {
-            initialise();
-            stress_bc(true, 0, 0);
-            velocity_bc(0);
+            if (!given_state) {
+                initialise();
+                stress_bc(true, 0, 0);
+                velocity_bc(0);
+            }
 #pragma omp barrier
 #pragma omp single
             t_start = omp_get_wtime();
@@ -994,6 +999,49 @@ extern "C" int oracle_norms(const OracleParams* p, void* const fields_in[9], dou
     if (p->media_arrays && (!p->rho_arr || !p->vp_arr || !p->vs_arr)) return 4;
     if (p->real_bytes == 8) return norms_typed<double>(p, fields_in, l2_f64, l2_ref);
     return norms_typed<float>(p, fields_in, l2_f64, l2_ref);
+}
+
+/* Time steps from a caller-supplied state: dense [dim1][dim2][dim3] real_t
+ * arrays become level 0 as they are (every cell, ghosts included), then
+ * p->ntsteps steps run as in oracle_run and the final level is copied out.
+ * Used to probe single statements (a linear velocity state makes every
+ * difference stencil exact). */
+template <typename R>
+int run_from_typed(const OracleParams* p, void* const in[9], void* out[9]) {
+    Sim<R> s(*p);
+    s.given_state = true;
+    for (int f = 0; f < NF; ++f) std::memcpy(s.F[f].data(), in[f], s.vol * sizeof(R));
+    s.run_loop();
+    s.copy_out(out);
+    return 0;
+}
+
+extern "C" int oracle_run_from(const OracleParams* p, void* const state_in[9], void* fields_out[9]) {
+    if (!p || !state_in || !fields_out) return 1;
+    if (p->order < 2 || p->order > 8 || (p->order % 2)) return 2;
+    if (p->snapshot && p->order != 4) return 3;
+    if (p->media_arrays && (!p->rho_arr || !p->vp_arr || !p->vs_arr)) return 4;
+    if (p->nthreads > 0) omp_set_num_threads(p->nthreads);
+    if (p->real_bytes == 8) return run_from_typed<double>(p, state_in, fields_out);
+    return run_from_typed<float>(p, state_in, fields_out);
+}
+
+/* The read path's media arrays as build_media leaves them (double copies of
+ * real_t values): beta1, beta2, beta3, lambda, mu, mu12, mu13, mu23, beta. */
+template <typename R>
+int media_typed(const OracleParams* p, double* out[9]) {
+    Sim<R> s(*p);
+    const std::vector<R>* src[9] = {&s.beta1, &s.beta2, &s.beta3, &s.lam_a, &s.mu_a,
+                                    &s.mu12, &s.mu13, &s.mu23, &s.beta_a};
+    for (int k = 0; k < 9; ++k)
+        for (size_t q = 0; q < s.vol; ++q) out[k][q] = (double)(*src[k])[q];
+    return 0;
+}
+
+extern "C" int oracle_media(const OracleParams* p, double* out[9]) {
+    if (!p || !out || !p->media_arrays || !p->rho_arr || !p->vp_arr || !p->vs_arr) return 1;
+    if (p->real_bytes == 8) return media_typed<double>(p, out);
+    return media_typed<float>(p, out);
 }
 
 extern "C" int oracle_fd_weights(int half_order, double* w) {

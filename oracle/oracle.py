@@ -71,6 +71,10 @@ def load(native=False):
         lib.oracle_norms.restype = ctypes.c_int
         lib.oracle_norms.argtypes = [ctypes.POINTER(OracleParams), ctypes.c_void_p * 9,
                                      ctypes.c_double * 9, ctypes.c_double * 9]
+        lib.oracle_run_from.restype = ctypes.c_int
+        lib.oracle_run_from.argtypes = [ctypes.POINTER(OracleParams), ctypes.c_void_p * 9, ctypes.c_void_p * 9]
+        lib.oracle_media.restype = ctypes.c_int
+        lib.oracle_media.argtypes = [ctypes.POINTER(OracleParams), ctypes.c_void_p * 9]
         lib.oracle_fd_weights.restype = ctypes.c_int
         lib.oracle_fd_weights.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         _LIBS[name] = lib
@@ -210,6 +214,57 @@ def run(order=4, n=(16, 16, 16), ntsteps=10, dt=None, L=(1.0, 1.0, 1.0), double=
         raise RuntimeError("oracle_run failed with code %d" % rc)
     return dict(fields=outs, l2=list(l2), l2_ref=list(l2r), seconds=tm[0], seconds_total=tm[1],
                 dims=dims, rec=rec_out)
+
+
+def run_from(state, order=4, n=(16, 16, 16), ntsteps=1, dt=None, L=(1.0, 1.0, 1.0), double=True,
+             rho=1.0, lam=None, mu=None, vp=1.0, vs=0.5, media=None, bc_algo=-1, nthreads=0):
+    """Steps from a given state: ``state`` = 9 arrays (dim1, dim2, dim3) that
+    become level 0 as they are (ghost cells included, no analytic initial
+    state and no initial boundary pass); returns the 9 arrays of the level
+    after ``ntsteps`` steps."""
+    n = tuple(int(v) for v in n)
+    M = order // 2
+    dims = tuple(v + 1 + 2 * M for v in n)
+    if dt is None:
+        raise ValueError("dt required")
+    p = _params(order, n, ntsteps, dt, L, double, rho, lam, mu, vp, vs, False, nthreads, bc_algo, 1)
+    keep = []
+    if media is not None:
+        arrs = [np.ascontiguousarray(a, dtype=np.float32).reshape(-1) for a in media]
+        keep += arrs
+        p.media_arrays = 1
+        p.rho_arr, p.vp_arr, p.vs_arr = [a.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) for a in arrs]
+    dtype = np.float64 if double else np.float32
+    ins = [np.ascontiguousarray(a, dtype=dtype) for a in state]
+    for a in ins:
+        assert a.shape == dims, (a.shape, dims)
+    outs = [np.zeros(dims, dtype=dtype) for _ in range(9)]
+    rc = load().oracle_run_from(ctypes.byref(p), (ctypes.c_void_p * 9)(*[a.ctypes.data for a in ins]),
+                                (ctypes.c_void_p * 9)(*[o.ctypes.data for o in outs]))
+    if rc != 0:
+        raise RuntimeError("oracle_run_from failed with code %d" % rc)
+    return outs
+
+
+MEDIA_NAMES = ("beta1", "beta2", "beta3", "lambda", "mu", "mu12", "mu13", "mu23", "beta")
+
+
+def media(rho, vp, vs, order=4, double=True):
+    """The read path's effective media as the oracle builds them from float32
+    rho/Vp/Vs arrays of the full grid shape: dict name -> float64 array."""
+    rho, vp, vs = (np.ascontiguousarray(a, dtype=np.float32) for a in (rho, vp, vs))
+    M = order // 2
+    dims = rho.shape
+    n = tuple(v - 1 - 2 * M for v in dims)
+    p = _params(order, n, 0, 1.0, (1.0, 1.0, 1.0), double, 1.0, 1.0, 1.0, 1.0, 0.5, False, 0, -1, 1)
+    p.media_arrays = 1
+    flat = [a.reshape(-1) for a in (rho, vp, vs)]
+    p.rho_arr, p.vp_arr, p.vs_arr = [a.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) for a in flat]
+    outs = [np.zeros(dims, dtype=np.float64) for _ in range(9)]
+    rc = load().oracle_media(ctypes.byref(p), (ctypes.c_void_p * 9)(*[o.ctypes.data for o in outs]))
+    if rc != 0:
+        raise RuntimeError("oracle_media failed with code %d" % rc)
+    return dict(zip(MEDIA_NAMES, outs))
 
 
 def snapshot_test3d(nthreads=0):
