@@ -201,15 +201,15 @@ typedef struct OpesciContext OpesciContext;
 /* --- reference-named entry points ---------------------------------------
  * opesci_model_load binds a model to the process-default context, the way
  * loading a generated .so did (opesci/grid.py:35-42).  Then:
- * opesci_execute      <- generated TU, opesci/templates/regular3d_tmpl.py:102-117
+ * opesci_execute      <- generated TU, opesci/templates/regular3d_tmpl.py:44-59
  *   allocates the fields on the device, applies the initial condition and the
  *   initial boundary conditions, runs ntsteps on the GPU, and hands back host
  *   arrays real_t[2][dim1][dim2][dim3] (both time levels) through grid, which
  *   the caller releases with opesci_free.
- * opesci_convergence  <- regular3d_tmpl.py:164-170, staggeredgrid.py:892-945
+ * opesci_convergence  <- regular3d_tmpl.py:106-112, staggeredgrid.py:892-945
  *   conv points to 9 real_t: sqrt(sum (F - F_exact)^2 dx dy dz) per field,
  *   accumulated in double on the device.
- * opesci_free         <- regular3d_tmpl.py:172-176 */
+ * opesci_free         <- regular3d_tmpl.py:114-118 */
 int opesci_model_load(const OpesciModelDesc* desc);
 int opesci_execute(OpesciGrid* grid, OpesciProfiling* profiling);
 int opesci_convergence(OpesciGrid* grid, void* conv);

@@ -57,7 +57,8 @@ EXPORTS = ('opesci_model_load', 'opesci_execute', 'opesci_convergence', 'opesci_
            'opesci_ctx_l2', 'opesci_ctx_download', 'opesci_ctx_upload', 'opesci_ctx_receivers', 'opesci_ctx_receivers_f64',
            'opesci_ctx_steps', 'opesci_ctx_kernel_times', 'opesci_ctx_stats', 'opesci_ctx_configure',
            'opesci_batch_create', 'opesci_batch_run', 'opesci_batch_destroy', 'opesci_ctx_gradient',
-           'opesci_ctx_download_gradient', 'opesci_ctx_upload_media', 'opesci_last_error',
+           'opesci_ctx_download_gradient', 'opesci_ctx_upload_media', 'opesci_ctx_download_media',
+           'opesci_last_error',
            'opesci_abi_version', 'opesci_device_info')
 
 

@@ -7,10 +7,11 @@ self-contained generated propagator benchmarks/test3d.cpp, built by
 oracle/Makefile into oracle/_ref/test3d):
   * known-answer L2 vector: stdout of oracle/_ref/test3d
   * tests/eigenwave3d.ipynb saved outputs: 4th-order staggered weights (cell 11),
-    kernel op counts (cells 18-20), CFL limit (cell 3)
+    kernel op counts (cells 18-20; 8th-order counts, see below), CFL limit (cell 3)
   * tests/eigenwave3d-higher_order.ipynb: 8th-order velocity coefficients
-    (cell 31), 8th-order AI (cells 27-28), Robertsson ghost planes (cells 34-35)
-  * tests/eigenwave3d-read.ipynb: 2nd-order AI (cells 14-15)
+    (cell 31), 4th-order op counts (cells 27-28), Robertsson ghost planes
+    (cells 34-35)
+  * tests/eigenwave3d-read.ipynb: 4th-order read-path op counts (cells 14-15)
 """
 import json
 import os
@@ -52,7 +53,11 @@ def main():
     t11 = cell_text(nb, 11)
     fx["fd_weights_4th"] = {"source": "tests/eigenwave3d.ipynb cell 11",
                             "text": t11.strip().splitlines()[-1].strip()}
-    fx["ai_4th"] = {"source": "tests/eigenwave3d.ipynb cells 18-20",
+    # eigenwave3d.ipynb's source cell 7 sets order [2,4,4,4], but its saved
+    # outputs of cells 18-19 are the counts of an 8th-order kernel (3 axes x
+    # 8 stencil points per update: 72 / 120); the outputs predate the source.
+    # The key names the notebook, not an order.
+    fx["ai_nb_eigenwave3d"] = {"source": "tests/eigenwave3d.ipynb cells 18-20",
                     "velocity": json.loads(cell_text(nb, 18).replace("(", "[").replace(")", "]")),
                     "stress": json.loads(cell_text(nb, 19).replace("(", "[").replace(")", "]")),
                     "overall": json.loads(cell_text(nb, 20).replace("(", "[").replace(")", "]"))}
@@ -65,7 +70,8 @@ def main():
         "source": "tests/eigenwave3d-higher_order.ipynb cell 31 (dt/dx = 0.2, beta = 1)",
         "terms": [[float(c.replace(" ", "")), f, a, b, c2] for c, f, a, b, c2 in terms],
     }
-    fx["ai_8th"] = {"source": "tests/eigenwave3d-higher_order.ipynb cells 27-28",
+    # set_accuracy([1,2,2,2]) of the older API: 4th order (36 / 60)
+    fx["ai_nb_higher_order"] = {"source": "tests/eigenwave3d-higher_order.ipynb cells 27-28",
                     "velocity": json.loads(cell_text(nbh, 27).replace("(", "[").replace(")", "]")),
                     "stress": json.loads(cell_text(nbh, 28).replace("(", "[").replace(")", "]"))}
     fx["robertsson_8th_U_bc"] = {"source": "tests/eigenwave3d-higher_order.ipynb cell 34",
@@ -73,7 +79,8 @@ def main():
     fx["robertsson_8th_Txx_bc"] = {"source": "tests/eigenwave3d-higher_order.ipynb cell 35",
                                    "text": cell_text(nbh, 35)}
     nbr = json.load(open(os.path.join(REF, "tests", "eigenwave3d-read.ipynb")))
-    fx["ai_2nd"] = {"source": "tests/eigenwave3d-read.ipynb cells 14-15",
+    # set_accuracy([1,2,2,2]) on the read path: 4th order
+    fx["ai_nb_read"] = {"source": "tests/eigenwave3d-read.ipynb cells 14-15",
                     "velocity": json.loads(cell_text(nbr, 14).replace("(", "[").replace(")", "]")),
                     "stress": json.loads(cell_text(nbr, 15).replace("(", "[").replace(")", "]"))}
     with open(os.path.join(HERE, "reference_fixtures.json"), "w") as f:

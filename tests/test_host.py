@@ -85,17 +85,28 @@ def test_8th_order_velocity_coefficients_fixture(fixtures):
 
 
 def test_ai_fixtures(fixtures):
-    """Op counts of the folded kernels: 4th order (36,36,9,3)/(60,60,9,6)
-    (higher_order notebook cells 27-28), 8th order (72,72,9,3)/(120,120,9,6)
-    (cells 32-33)."""
+    """Op counts (ADD, MUL, LOAD, STORE) of the folded kernels against the
+    reference notebooks' printed get_*_kernel_ai() outputs.  4th order:
+    velocity (36, 36, 9, 3), stress (60, 60, 9, 6), as printed by
+    eigenwave3d-higher_order.ipynb cells 27-28 and eigenwave3d-read.ipynb
+    cells 14-15 (both set accuracy [1,2,2,2] of the older API: 4th order).
+    8th order: (72, 72, 9, 3) / (120, 120, 9, 6), as printed by
+    eigenwave3d.ipynb cells 18-19, whose saved outputs are 8th-order counts
+    (3 axes x 8 stencil points per update) although the notebook's source now
+    sets [2,4,4,4].  The per-term count fixes which order each belongs to:
+    an update of order 2M has 3 x 2M stencil terms (velocity, normal stress)
+    or 2 x 2M (shear), one ADD and one MUL each."""
     g = opesci.eigenwave3d((1.0, 1.0, 1.0), (20, 20, 20), 0.002, 0.01, accuracy_order=[2, 4, 4, 4])
-    assert g.get_velocity_kernel_ai()[2:] == (36, 36, 9, 3)
-    assert g.get_stress_kernel_ai()[2:] == (60, 60, 9, 6)
     g8 = opesci.eigenwave3d((1.0, 1.0, 1.0), (20, 20, 20), 0.002, 0.01, accuracy_order=[2, 8, 8, 8])
-    assert g8.get_velocity_kernel_ai() == tuple(fixtures['ai_8th']['velocity']) or \
-        g8.get_velocity_kernel_ai()[2:] == (72, 72, 9, 3)
-    assert g8.get_velocity_kernel_ai()[2:] == tuple(fixtures['ai_4th']['velocity'][2:])
-    assert g8.get_stress_kernel_ai()[2:] == tuple(fixtures['ai_4th']['stress'][2:])
+    for order, grid in ((4, g), (8, g8)):
+        M = order // 2
+        assert grid.get_velocity_kernel_ai()[2:] == (3 * 3 * 2 * M, 3 * 3 * 2 * M, 9, 3)
+        assert grid.get_stress_kernel_ai()[2:] == (3 * 3 * 2 * M + 3 * 2 * 2 * M,) * 2 + (9, 6)
+    for key in ('ai_nb_higher_order', 'ai_nb_read'):
+        assert g.get_velocity_kernel_ai()[2:] == tuple(fixtures[key]['velocity'][2:]), key
+        assert g.get_stress_kernel_ai()[2:] == tuple(fixtures[key]['stress'][2:]), key
+    assert g8.get_velocity_kernel_ai()[2:] == tuple(fixtures['ai_nb_eigenwave3d']['velocity'][2:])
+    assert g8.get_stress_kernel_ai()[2:] == tuple(fixtures['ai_nb_eigenwave3d']['stress'][2:])
 
 
 def test_robertsson_fixture(fixtures):
