@@ -225,6 +225,17 @@ int opesci_ctx_init(OpesciContext* ctx);
 int opesci_ctx_run(OpesciContext* ctx, int32_t nsteps, double* seconds);
 /* per-field L2 vs the analytic solution at the current time (double) */
 int opesci_ctx_l2(OpesciContext* ctx, double out[OPESCI_NFIELDS]);
+/* The same L2 norms without any allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  The per-block
+ * partial sums live in a fixed library-owned device array of 9 x 512
+ * doubles (36 KiB per device, held until the process ends; the partial
+ * sums are grid-stride, so any grid size fits); `d_out` is device memory
+ * for 9 doubles; the work is enqueued on `stream` (a hipStream_t; null =
+ * the context's stream) with no host synchronisation.  The context's device
+ * must be current.  Calls that overlap on one device race on the array.
+ * Same block count and summation order as opesci_ctx_l2. */
+int opesci_ctx_l2_noalloc(OpesciContext* ctx, double* d_out, void* stream);
 /* copy one field, one time level (0/1, or -1 = latest) to/from a dense host array real_t[dim1][dim2][dim3] */
 int opesci_ctx_download(OpesciContext* ctx, int32_t field, int32_t level, void* host);
 int opesci_ctx_upload(OpesciContext* ctx, int32_t field, int32_t level, const void* host);

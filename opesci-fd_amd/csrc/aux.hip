@@ -2,6 +2,8 @@
 // layout conversion and point source / receivers.
 #include "bcdev.h"
 
+#include <mutex>
+
 namespace opesci {
 
 // real_t x = dx*(i - m [+ 0.5]) of the reference loops, rounded once to R
@@ -332,6 +334,69 @@ hipError_t launch_l2(const Geom& g, int real_bytes, int storage, void* fields, c
         out_host[f] = sqrt(a * vol);
     }
     return hipSuccess;
+}
+
+// ------------------------------------------------------------------ L2 without allocation
+// The per-block partial sums of opesci_ctx_l2_noalloc live in one
+// library-owned array per device: NF x L2_BLOCKS doubles (36 KiB), held
+// until the process ends and shared by every call on that device (calls
+// that overlap on one device race on it).  Every block of l2_kernel writes
+// its partial, so nothing needs clearing between calls.  In host code the
+// array's name stands for a host-side shadow, never for device memory: the
+// device address is looked up once per device with hipGetSymbolAddress and
+// only that address is handed to kernels.
+constexpr int L2_BLOCKS = 512;
+constexpr int L2_MAX_DEVICES = 64;
+static __device__ __align__(256) double l2_scratch[NF * L2_BLOCKS];
+
+// out[f] = sqrt(vol * sum_b partial[f][b]), summed in block order like the
+// host reduction of launch_l2
+__global__ void l2_finish_kernel(const double* __restrict__ partial, int nblk, double vol, double* __restrict__ out) {
+    const int f = threadIdx.x;
+    if (f >= NF) return;
+    double a = 0.0;
+    for (int b = 0; b < nblk; ++b) a += partial[(size_t)f * nblk + b];
+    out[f] = sqrt(a * vol);
+}
+
+static hipError_t l2_scratch_of_current_device(double** out) {
+    static std::mutex mu;
+    static double* addr[L2_MAX_DEVICES] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= L2_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(l2_scratch));
+        if (e != hipSuccess) return e;
+        addr[dev] = static_cast<double*>(p);
+    }
+    *out = addr[dev];
+    return hipSuccess;
+}
+
+int l2_noalloc_partials() { return L2_BLOCKS; }
+
+hipError_t launch_l2_noalloc(const Geom& g, int real_bytes, int storage, void* fields, const void* media,
+                             const OpesciExprOp* d_sol, const int* sol_off, const int* sol_len,
+                             const double* spacing, int level, const double t_field[NF], double* d_out,
+                             hipStream_t s) {
+    double* partial = nullptr;
+    hipError_t e = l2_scratch_of_current_device(&partial);
+    if (e != hipSuccess) return e;
+    SolTable tab = make_tab(sol_off, sol_len, spacing);
+    for (int f = 0; f < NF; ++f) tab.t[f] = t_field[f];
+    e = OPESCI_TYPED(real_bytes, storage,
+                     (l2_t<float, float>(g, fields, media, d_sol, tab, level, partial, L2_BLOCKS, s)),
+                     (l2_t<double, double>(g, fields, media, d_sol, tab, level, partial, L2_BLOCKS, s)),
+                     (l2_t<float, _Float16>(g, fields, media, d_sol, tab, level, partial, L2_BLOCKS, s)),
+                     (l2_t<float, bf16_t>(g, fields, media, d_sol, tab, level, partial, L2_BLOCKS, s)));
+    if (e != hipSuccess) return e;
+    const double vol = spacing[0] * spacing[1] * spacing[2];
+    hipLaunchKernelGGL(l2_finish_kernel, dim3(1), dim3(64), 0, s, (const double*)partial, L2_BLOCKS, vol, d_out);
+    return hipGetLastError();
 }
 
 hipError_t launch_media(const Geom& g, int real_bytes, const float* d_rho, const float* d_vp, const float* d_vs,

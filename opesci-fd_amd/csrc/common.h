@@ -284,6 +284,15 @@ hipError_t launch_l2(const Geom& g, int real_bytes, int storage, void* fields, c
                      const OpesciExprOp* d_sol, const int* sol_off, const int* sol_len, const double* spacing,
                      int level, const double t_field[NF], double* d_partial, int partial_cap, double* out_host,
                      hipStream_t s);
+// the same L2 with its partial sums in a fixed library-owned device array
+// and the norms written to d_out (9 doubles, device memory), stream-ordered,
+// no allocation and no host synchronisation; the block count equals
+// launch_l2's, so the two agree
+int l2_noalloc_partials();
+hipError_t launch_l2_noalloc(const Geom& g, int real_bytes, int storage, void* fields, const void* media,
+                             const OpesciExprOp* d_sol, const int* sol_off, const int* sol_len,
+                             const double* spacing, int level, const double t_field[NF], double* d_out,
+                             hipStream_t s);
 hipError_t launch_media(const Geom& g, int real_bytes, const float* d_rho, const float* d_vp, const float* d_vs,
                         void* const* d_media, hipStream_t s);
 hipError_t launch_pack(const Geom& g, int real_bytes, int storage, const void* dense, void* field, hipStream_t s);

@@ -38,6 +38,7 @@ struct OpesciContext {
     int real_bytes = 4, storage = 0, elem = 4;
     bool het = false;
     hipStream_t stream = nullptr;
+    int device = 0;                 // the device current at creation
     void* base[NF][2] = {};
     void* fptr[NF][2] = {};         // FieldPtrs<S> image
     void* mbase[OPESCI_M_COUNT] = {};
@@ -416,6 +417,7 @@ int create(const OpesciModelDesc* d, OpesciContext** out) {
     c->cfg.variant = d->kernel_variant;
     c->cfg.tile_y = d->tile_y;
     c->cfg.x_chunk = d->x_chunk;
+    OPESCI_HIP_TRY(hipGetDevice(&c->device));
     OPESCI_HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     Geom& g = c->g;
     g.nx = d->dims[0];
@@ -661,6 +663,20 @@ int opesci_ctx_run(OpesciContext* ctx, int32_t nsteps, double* seconds) {
 int opesci_ctx_l2(OpesciContext* ctx, double out[OPESCI_NFIELDS]) {
     if (!ctx || !out) return fail(OPESCI_EINVAL, "null argument");
     return ctx_l2(ctx, out);
+}
+
+int opesci_ctx_l2_noalloc(OpesciContext* ctx, double* d_out, void* stream) {
+    if (!ctx || !d_out) return fail(OPESCI_EINVAL, "null argument");
+    if (!ctx->initialised) return fail(OPESCI_ESTATE, "no state: call opesci_ctx_init first");
+    int dev = -1;
+    OPESCI_HIP_TRY(hipGetDevice(&dev));
+    if (dev != ctx->device) return fail(OPESCI_ESTATE, "the context's device must be the current device");
+    double t[NF];
+    for (int f = 0; f < NF; ++f) t[f] = ctx->desc.dt * ctx->steps + (time_stag(f) ? ctx->desc.dt / 2.0 : 0.0);
+    const hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    OPESCI_HIP_TRY(launch_l2_noalloc(ctx->g, ctx->real_bytes, ctx->storage, ctx->fptr, ctx->mptr, ctx->d_sol,
+                                     ctx->sol_off, ctx->sol_len, ctx->desc.spacing, ctx->steps % 2, t, d_out, s));
+    return 0;
 }
 
 int opesci_ctx_download(OpesciContext* ctx, int32_t field, int32_t level, void* host) {

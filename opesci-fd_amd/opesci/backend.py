@@ -58,7 +58,7 @@ EXPORTS = ('opesci_model_load', 'opesci_execute', 'opesci_convergence', 'opesci_
            'opesci_ctx_steps', 'opesci_ctx_kernel_times', 'opesci_ctx_stats', 'opesci_ctx_configure',
            'opesci_batch_create', 'opesci_batch_run', 'opesci_batch_destroy', 'opesci_ctx_gradient',
            'opesci_ctx_download_gradient', 'opesci_ctx_upload_media', 'opesci_ctx_download_media',
-           'opesci_last_error',
+           'opesci_ctx_l2_noalloc', 'opesci_last_error',
            'opesci_abi_version', 'opesci_device_info')
 
 
@@ -79,6 +79,7 @@ def load(path=None):
         lib.opesci_ctx_init.argtypes = [_vp]
         lib.opesci_ctx_run.argtypes = [_vp, _i32, ctypes.POINTER(_f64)]
         lib.opesci_ctx_l2.argtypes = [_vp, ctypes.POINTER(_f64)]
+        lib.opesci_ctx_l2_noalloc.argtypes = [_vp, _vp, _vp]
         lib.opesci_ctx_download.argtypes = [_vp, _i32, _i32, _vp]
         lib.opesci_ctx_upload.argtypes = [_vp, _i32, _i32, _vp]
         lib.opesci_ctx_receivers.argtypes = [_vp, _vp, ctypes.c_int64]
