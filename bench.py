@@ -54,8 +54,9 @@ def main():
     ap.add_argument('--copy-probe', type=int, default=0, help='MiB copied device-to-device 3x first')
     ap.add_argument('--hetero', action='store_true',
                     help='seeded smooth heterogeneous medium through the read path (8 more media streams)')
-    ap.add_argument('--split-z', action='store_true',
-                    help='run the deep rows of the stress z-face loops as one launch ahead of the others')
+    ap.add_argument('--no-split-z', action='store_true',
+                    help='keep the deep rows of the stress z-face loops in the boundary launches '
+                         '(default: split off and applied by the stress sweep)')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py runs on exactly one GPU: this backend has no multi-GPU path '
@@ -74,8 +75,8 @@ def main():
         g, kw = hetero_model(N, order, K + W, storage=args.storage)
     else:
         g, kw = eigen_model(N, order, K + W, storage=args.storage)
-    if args.split_z:
-        g.fuse_boundary = True
+    if args.no_split_z:
+        g.fuse_boundary = False
     plan = g.lower()
     if args.variant is not None:
         plan.kernel_variant = args.variant
@@ -93,7 +94,7 @@ def main():
         wall = time.perf_counter() - t0
         l2 = ctx.l2()
         kt = ctx.kernel_times(10)   # per-launch averages, after the timed region
-        bc_stats = ctx.stats()
+        bc_stats = ctx.stats(full=True)
     workload_key = 'stress_sweep %d^3 order %d %s%s' % (N, order, args.storage, ' hetero' if args.hetero else '')
     # flops per point-step: ADD + MUL of the stress and velocity kernels in the
     # reference's own operation count (get_*_kernel_ai)
@@ -132,6 +133,7 @@ def main():
             'mflops': flops_pt * value / 1e6,
             'boundary_launches_per_step': bc_stats[0],
             'deep_z_face_statements': bc_stats[1],
+            'deep_z_faces_in_stress_sweep': bool(bc_stats[3]),
         },
         'roofline': {
             'bound': 'hbm',

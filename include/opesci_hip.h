@@ -253,7 +253,8 @@ int opesci_ctx_steps(OpesciContext* ctx, int32_t* steps);
 int opesci_ctx_kernel_times(OpesciContext* ctx, int32_t nsteps, double out_ms[3]);
 /* Boundary execution plan of a context: out[0] = boundary launches per time
  * step, out[1] = phase-4 statements (deep z-face rows of the stress phase),
- * out[2] = launches they take (0 or 1), out[3] = 0.  No reference
+ * out[2] = launches they take (0 or 1), out[3] = 1 when the stress sweep
+ * applies them in place (FP32 pair sweeps), else 0.  No reference
  * counterpart. */
 int opesci_ctx_stats(OpesciContext* ctx, int32_t out[4]);
 

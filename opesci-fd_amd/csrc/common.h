@@ -58,10 +58,20 @@ struct Conv<double, double> {
     __device__ __forceinline__ static double from(double v) { return v; }
     __device__ __forceinline__ static double to(double v) { return v; }
 };
+// FP32 -> FP16, round to nearest even, from a value already rounded to FP32.
+// The empty register constraint keeps the compiler from fusing a producing
+// fma with the conversion into one mixed-precision fma (v_fma_mix*), which
+// rounds once instead of twice and so differs in the last bit now and then
+// -- and does so in some kernels and not in others, while every kernel that
+// stores a value must round it the same way.
+__device__ __forceinline__ _Float16 to_f16(float v) {
+    asm("" : "+v"(v));
+    return (_Float16)v;
+}
 template <>
 struct Conv<_Float16, float> {
     __device__ __forceinline__ static float from(_Float16 v) { return (float)v; }
-    __device__ __forceinline__ static _Float16 to(float v) { return (_Float16)v; }  // round to nearest even
+    __device__ __forceinline__ static _Float16 to(float v) { return to_f16(v); }
     __device__ __forceinline__ static float ld(const _Float16* p) { return from(*p); }
     __device__ __forceinline__ static void st(_Float16* p, float v) { *p = to(v); }
     __device__ __forceinline__ static float ldnt(const _Float16* p) { return from(__builtin_nontemporal_load(p)); }
@@ -103,8 +113,8 @@ struct Pair<_Float16> {
         b = (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16));
     }
     __device__ __forceinline__ static uint32_t join(float a, float b) {
-        return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)a) |
-               ((uint32_t)__builtin_bit_cast(uint16_t, (_Float16)b) << 16);
+        return (uint32_t)__builtin_bit_cast(uint16_t, to_f16(a)) |
+               ((uint32_t)__builtin_bit_cast(uint16_t, to_f16(b)) << 16);
     }
 };
 template <>
@@ -199,8 +209,40 @@ struct InteriorLaunch {
 };
 
 
+// Free-surface statements of the stress phase on the z faces, applied by the
+// FP32 pair stress sweep in place, for the deep rows x, y in [F, dim - F)
+// (where they commute with every other statement of the phase, so they may
+// follow the interior update directly).  Per face (b = m low, dim - m - 1
+// high) the statements have three fixed forms, which the host checks
+// against the lowered boundary program before enabling them:
+//   zero      Tzz(b) = 0
+//   image     Tzz(b -+ k) = c Tzz(b +- k), k = 1..M-1;
+//             Tyz, Txz(b - 1 - k) = c T(b + k) (low),  T(b + k) = c T(b - 1 - k) (high), k = 0..M-1
+//   recompute Txx, Tyy(b) = sum of 9 terms in the canonical order
+//             U(x-2..x+1), V(y-2..y+1), T(t0) (Levander, half order 2),
+//             literal coefficients (constant media).
+// Values, term order and fma accumulation equal bc_point's, so the fields
+// are those of the boundary launches bit for bit.
+// coefficient block in device memory, staged into LDS once per block
+struct ZCoef {
+    float rc[2][2][9];     // recompute literals [low, high][Txx, Tyy][term]
+    float zc[2][3][MAXM];  // image coefficients [low, high][Tzz, Tyz, Txz][k]
+};
+constexpr int ZCOEF_WORDS = (int)(sizeof(ZCoef) / 4);
+static_assert(sizeof(ZCoef) % 4 == 0, "ZCoef is copied as 32-bit words");
+// scalar part, passed by value
+struct ZFaceArgs {
+    int F = 0;        // 0: off
+    int rec = 0;      // recompute statements present
+    int n1 = 0, npts = 0;
+    const ZCoef* coef = nullptr;
+};
+
 hipError_t launch_stress(const Geom& g, int real_bytes, int storage, bool het, void* fields, const void* media,
-                         const void* coefs, int t0, int t1, const InteriorLaunch& cfg, hipStream_t s);
+                         const void* coefs, int t0, int t1, const InteriorLaunch& cfg, hipStream_t s,
+                         const ZFaceArgs* zf = nullptr, const void* pool = nullptr);
+// whether launch_stress applies ZFaceArgs for this configuration
+bool stress_applies_zfaces(int real_bytes, bool het, const InteriorLaunch& cfg);
 hipError_t launch_velocity(const Geom& g, int real_bytes, int storage, bool het, void* fields, const void* media,
                            const void* coefs, int t0, int t1, const InteriorLaunch& cfg, hipStream_t s);
 

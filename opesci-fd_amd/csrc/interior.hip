@@ -571,10 +571,18 @@ __device__ __forceinline__ void store_points(S* p, float a, float b, bool act0, 
         Conv<S, float>::stnt(p, a);
 }
 
-template <typename S, int M, int TY, bool HET>
+// v as lane `lane` (wave-uniform) holds it, rounded to the storage type as a
+// boundary statement reading it back from memory sees it
+template <typename S>
+__device__ __forceinline__ float lane_read(float v, int lane) {
+    const float r = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+    return Conv<S, float>::from(Conv<S, float>::to(r));
+}
+
+template <typename S, int M, int TY, bool HET, bool ZF>
 __global__ __launch_bounds__(64 * TY) void stress_sweep2(Geom g, FieldPtrs<S> F, MediaPtrs<float> md,
                                                          Coefs<float> c, int t0, int t1, int tz, int ty, int xchunk,
-                                                         int nblocks) {
+                                                         int nblocks, ZFaceArgs zf) {
     using R = float;
     using P = Plane2<M, TY>;
     __shared__ R sU[2][P::H * P::W], sV[2][P::H * P::W], sW[2][P::H * P::W];
@@ -592,6 +600,18 @@ __global__ __launch_bounds__(64 * TY) void stress_sweep2(Geom g, FieldPtrs<S> F,
     const S* W0 = F.f[2][t0];
     HaloSlots2<M, TY> hs;
     hs.init(g, y0, z0);
+    static_assert(!(ZF && HET), "in-sweep z faces are built for constant media");
+    // z-face statements: coefficient block staged in LDS once per block (kept
+    // out of the kernel arguments, whose scalar registers it would exhaust)
+    __shared__ ZCoef zl;
+    const int zrh = g.nz - M - 1 - z0;                 // high face relative to the tile
+    const bool zlo = bz == 0, zhi = zrh >= 0 && zrh < 128;
+    const int zlane = zlo && (threadIdx.x & 63) == 0 ? 0 : (zhi && (int)(threadIdx.x & 63) == (zrh >> 1) ? 1 : -1);
+    if constexpr (ZF) {
+        for (int w = threadIdx.x; w < ZCOEF_WORDS; w += blockDim.x)
+            reinterpret_cast<int*>(&zl)[w] = reinterpret_cast<const int*>(zf.coef)[w];
+        __syncthreads();
+    }
     // per point v: qU[v][i] = U(x+i-M), qV[v][i] = V(x+i-M+1), qW[v][i] = W(x+i-M+1)
     R qU[2][2 * M], qV[2][2 * M], qW[2][2 * M];
 #pragma unroll
@@ -713,6 +733,96 @@ __global__ __launch_bounds__(64 * TY) void stress_sweep2(Geom g, FieldPtrs<S> F,
                     out[3][v] = T0[3][v] + sxy;
                     out[4][v] = T0[4][v] + syz;
                     out[5][v] = T0[5][v] + sxz;
+                }
+            }
+            if constexpr (ZF) {
+                // free-surface statements of the z faces on the deep rows (the
+                // wave is one row: x and y are wave-uniform)
+                if (x >= zf.F && x < g.nx - zf.F && y >= zf.F && y < g.ny - zf.F) {
+                    const int64_t row = (int64_t)x * g.sx + (int64_t)y * g.sy;
+                    const int blo = M, bhi = g.nz - M - 1, rh = zrh;
+                    const int eh = rh & 1;
+                    // image sources (interior values, before any replacement):
+                    // [Tzz, Tyz, Txz][k] of the low and the high face
+                    float slo[3][M], shi[3][M];
+#pragma unroll
+                    for (int k = 0; k < M; ++k) {
+                        if (zlo) {
+                            slo[0][k] = k >= 1 ? lane_read<S>(out[2][k & 1], k >> 1) : 0.0f;
+                            slo[1][k] = lane_read<S>(out[4][k & 1], k >> 1);
+                            slo[2][k] = lane_read<S>(out[5][k & 1], k >> 1);
+                        }
+                        if (zhi) {
+                            const int r0 = rh - k, r1 = rh - 1 - k;
+                            shi[0][k] = k >= 1 ? lane_read<S>((r0 & 1) ? out[2][1] : out[2][0], r0 >> 1) : 0.0f;
+                            shi[1][k] = lane_read<S>((r1 & 1) ? out[4][1] : out[4][0], r1 >> 1);
+                            shi[2][k] = lane_read<S>((r1 & 1) ? out[5][1] : out[5][0], r1 >> 1);
+                        }
+                    }
+                    // Levander recompute of Txx (i = 0) or Tyy (i = 1) at the
+                    // surface plane, point element e of this (writer) lane
+                    auto recompute = [&](int face, int i, int e) -> float {
+                        float acc = 0.0f;
+                        if constexpr (M == 2) {
+#pragma unroll
+                            for (int k = 0; k < 9; ++k) {
+                                float v;
+                                if (k < 4)
+                                    v = e ? qU[1][k] : qU[0][k];
+                                else if (k < 8)
+                                    v = LV[me0 + e + (k - 6) * P::W];
+                                else
+                                    v = e ? T0[i][1] : T0[i][0];
+                                acc = fmac(zl.rc[face][i][k], v, acc);
+                            }
+                        }
+                        return acc;
+                    };
+                    // ghost cells: ordinary stores, so the few cells of each
+                    // face line merge in L2 before they reach HBM
+                    using CS = Conv<S, float>;
+                    if (zlane == 0) {
+                        if (zf.rec) {
+                            out[0][0] = recompute(0, 0, 0);
+                            out[1][0] = recompute(0, 1, 0);
+                        }
+                        out[2][0] = 0.0f;
+#pragma unroll
+                        for (int k = 0; k < M; ++k) {
+                            if (k >= 1) CS::st(F.f[5][t1] + row + blo - k, fmac(zl.zc[0][0][k], slo[0][k], 0.0f));
+                            CS::st(F.f[7][t1] + row + blo - 1 - k, fmac(zl.zc[0][1][k], slo[1][k], 0.0f));
+                            CS::st(F.f[8][t1] + row + blo - 1 - k, fmac(zl.zc[0][2][k], slo[2][k], 0.0f));
+                        }
+                    }
+                    if (zlane == 1) {
+                        const float rxx = zf.rec ? recompute(1, 0, eh) : 0.0f;
+                        const float ryy = zf.rec ? recompute(1, 1, eh) : 0.0f;
+                        const float tyz = fmac(zl.zc[1][1][0], shi[1][0], 0.0f);
+                        const float txz = fmac(zl.zc[1][2][0], shi[2][0], 0.0f);
+                        if (eh) {
+                            if (zf.rec) {
+                                out[0][1] = rxx;
+                                out[1][1] = ryy;
+                            }
+                            out[2][1] = 0.0f;
+                            out[4][1] = tyz;
+                            out[5][1] = txz;
+                        } else {
+                            if (zf.rec) {
+                                out[0][0] = rxx;
+                                out[1][0] = ryy;
+                            }
+                            out[2][0] = 0.0f;
+                            out[4][0] = tyz;
+                            out[5][0] = txz;
+                        }
+#pragma unroll
+                        for (int k = 1; k < M; ++k) {
+                            CS::st(F.f[5][t1] + row + bhi + k, fmac(zl.zc[1][0][k], shi[0][k], 0.0f));
+                            CS::st(F.f[7][t1] + row + bhi + k, fmac(zl.zc[1][1][k], shi[1][k], 0.0f));
+                            CS::st(F.f[8][t1] + row + bhi + k, fmac(zl.zc[1][2][k], shi[2][k], 0.0f));
+                        }
+                    }
                 }
             }
 #pragma unroll
@@ -899,7 +1009,7 @@ static int default_tile_y() {
 
 template <typename R, typename S, int M, bool HET, bool VEL>
 static hipError_t run_interior(const Geom& g, void* fields, const void* media, const void* coefs, int t0, int t1,
-                               const InteriorLaunch& cfg, hipStream_t s) {
+                               const InteriorLaunch& cfg, hipStream_t s, const ZFaceArgs* zf, const void* pool) {
     const FieldPtrs<S>& F = *static_cast<const FieldPtrs<S>*>(fields);
     const MediaPtrs<R>& md = *static_cast<const MediaPtrs<R>*>(media);
     const Coefs<R>& c = *static_cast<const Coefs<R>*>(coefs);
@@ -930,9 +1040,12 @@ static hipError_t run_interior(const Geom& g, void* fields, const void* media, c
         if (VEL)                                                                                                \
             hipLaunchKernelGGL((velocity_sweep2<S, M, TYV, HET>), dim3(t.grid), dim3(64 * TYV), 0, s, g, F, md, c, \
                                t0, t1, t.tz, t.tyn, t.xchunk, t.nb);                                            \
+        else if (zf && zf->F > 0)                                                                               \
+            hipLaunchKernelGGL((stress_sweep2<S, M, TYV, false, true>), dim3(t.grid), dim3(64 * TYV), 0, s, g, F,   \
+                               md, c, t0, t1, t.tz, t.tyn, t.xchunk, t.nb, *zf);                              \
         else                                                                                                    \
-            hipLaunchKernelGGL((stress_sweep2<S, M, TYV, HET>), dim3(t.grid), dim3(64 * TYV), 0, s, g, F, md, c,   \
-                               t0, t1, t.tz, t.tyn, t.xchunk, t.nb);                                            \
+            hipLaunchKernelGGL((stress_sweep2<S, M, TYV, HET, false>), dim3(t.grid), dim3(64 * TYV), 0, s, g, F, md, \
+                               c, t0, t1, t.tz, t.tyn, t.xchunk, t.nb, ZFaceArgs());                           \
         return hipGetLastError();                                                                               \
     }
             OPESCI_PAIR(4)
@@ -981,11 +1094,12 @@ static hipError_t run_interior(const Geom& g, void* fields, const void* media, c
 
 template <typename R, typename S, bool VEL>
 static hipError_t dispatch_m(const Geom& g, bool het, void* fields, const void* media, const void* coefs, int t0,
-                             int t1, const InteriorLaunch& cfg, hipStream_t s) {
-#define OPESCI_CASE(MM)                                                                        \
-    case MM:                                                                                   \
-        return het ? run_interior<R, S, MM, true, VEL>(g, fields, media, coefs, t0, t1, cfg, s) \
-                   : run_interior<R, S, MM, false, VEL>(g, fields, media, coefs, t0, t1, cfg, s);
+                             int t1, const InteriorLaunch& cfg, hipStream_t s, const ZFaceArgs* zf,
+                             const void* pool) {
+#define OPESCI_CASE(MM)                                                                                   \
+    case MM:                                                                                              \
+        return het ? run_interior<R, S, MM, true, VEL>(g, fields, media, coefs, t0, t1, cfg, s, zf, pool) \
+                   : run_interior<R, S, MM, false, VEL>(g, fields, media, coefs, t0, t1, cfg, s, zf, pool);
     switch (g.m) {
         OPESCI_CASE(1)
         OPESCI_CASE(2)
@@ -999,21 +1113,35 @@ static hipError_t dispatch_m(const Geom& g, bool het, void* fields, const void* 
 
 template <bool VEL>
 static hipError_t dispatch(const Geom& g, int real_bytes, int storage, bool het, void* fields, const void* media,
-                           const void* coefs, int t0, int t1, const InteriorLaunch& cfg, hipStream_t s) {
-    if (real_bytes == 8) return dispatch_m<double, double, VEL>(g, het, fields, media, coefs, t0, t1, cfg, s);
-    if (storage == OPESCI_STORE_F16) return dispatch_m<float, _Float16, VEL>(g, het, fields, media, coefs, t0, t1, cfg, s);
-    if (storage == OPESCI_STORE_BF16) return dispatch_m<float, bf16_t, VEL>(g, het, fields, media, coefs, t0, t1, cfg, s);
-    return dispatch_m<float, float, VEL>(g, het, fields, media, coefs, t0, t1, cfg, s);
+                           const void* coefs, int t0, int t1, const InteriorLaunch& cfg, hipStream_t s,
+                           const ZFaceArgs* zf, const void* pool) {
+    if (real_bytes == 8)
+        return dispatch_m<double, double, VEL>(g, het, fields, media, coefs, t0, t1, cfg, s, nullptr, nullptr);
+    if (storage == OPESCI_STORE_F16)
+        return dispatch_m<float, _Float16, VEL>(g, het, fields, media, coefs, t0, t1, cfg, s, zf, pool);
+    if (storage == OPESCI_STORE_BF16)
+        return dispatch_m<float, bf16_t, VEL>(g, het, fields, media, coefs, t0, t1, cfg, s, zf, pool);
+    return dispatch_m<float, float, VEL>(g, het, fields, media, coefs, t0, t1, cfg, s, zf, pool);
+}
+
+bool stress_applies_zfaces(int real_bytes, bool het, const InteriorLaunch& cfg) {
+    // the pair sweeps (FP32 arithmetic, variant 0) with their 4- or 8-row
+    // stress tiles, constant media (on the read path the surface recompute
+    // takes per-point coefficients, whose prefetch costs the sweep more
+    // occupancy than the separate launch costs time)
+    return real_bytes == 4 && !het && cfg.variant == 0;
 }
 
 hipError_t launch_stress(const Geom& g, int real_bytes, int storage, bool het, void* fields, const void* media,
-                         const void* coefs, int t0, int t1, const InteriorLaunch& cfg, hipStream_t s) {
-    return dispatch<false>(g, real_bytes, storage, het, fields, media, coefs, t0, t1, cfg, s);
+                         const void* coefs, int t0, int t1, const InteriorLaunch& cfg, hipStream_t s,
+                         const ZFaceArgs* zf, const void* pool) {
+    if (zf && zf->F > 0 && !stress_applies_zfaces(real_bytes, het, cfg)) return hipErrorInvalidValue;
+    return dispatch<false>(g, real_bytes, storage, het, fields, media, coefs, t0, t1, cfg, s, zf, pool);
 }
 
 hipError_t launch_velocity(const Geom& g, int real_bytes, int storage, bool het, void* fields, const void* media,
                            const void* coefs, int t0, int t1, const InteriorLaunch& cfg, hipStream_t s) {
-    return dispatch<true>(g, real_bytes, storage, het, fields, media, coefs, t0, t1, cfg, s);
+    return dispatch<true>(g, real_bytes, storage, het, fields, media, coefs, t0, t1, cfg, s, nullptr, nullptr);
 }
 
 }  // namespace opesci

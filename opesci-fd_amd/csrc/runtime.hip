@@ -92,6 +92,8 @@ struct OpesciContext {
     bool initialised = false;
     size_t field_bytes = 0;
     InteriorLaunch cfg{0, 0, 0};
+    ZFaceArgs zf;                   // deep z-face statements the pair stress sweep applies (F = 0: none)
+    ZCoef* d_zcoef = nullptr;       // their coefficient block (device)
 
     const void* coefs() const { return real_bytes == 8 ? (const void*)&cd : (const void*)&cf; }
 };
@@ -335,6 +337,95 @@ int alloc_zero(OpesciContext* c, void** p, size_t bytes) {
     return 0;
 }
 
+// The deep z-face statements (phase 4, one launch level) when they have the
+// fixed forms the pair stress sweep applies in place: zero and images of
+// Tzz, images of Tyz/Txz, and the Levander recompute of Txx/Tyy with its
+// terms in the canonical order.  Anything else leaves zf.F = 0 and the
+// statements run as their own launch.
+int build_zfaces(OpesciContext* c) {
+    c->zf = ZFaceArgs();
+    const Geom& g = c->g;
+    const int M = g.m;
+    if (c->levels[4].size() != 1) return 0;
+    const int blo = M, bhi = g.nz - M - 1;
+    // image sources within the wave that holds the face: 2M interior points
+    // on either side, and the high face at least M points into its tile
+    if (bhi - blo < 2 * M || (bhi - M) % 128 < M) return 0;
+    ZFaceArgs z;
+    ZCoef zc;
+    memset(&zc, 0, sizeof(zc));
+    bool zero[2] = {false, false}, img[2][3][MAXM] = {}, rec[2][2] = {};
+    int F = -1;
+    const BcLevel& lv = c->levels[4][0];
+    for (int di = lv.first; di < lv.first + lv.count; ++di) {
+        const BcDesc& d = c->h_desc[di];
+        const int f = d.fl >> 1;
+        if (d.axis != 2 || (d.fl & 1) != 1) return 0;
+        const int f0 = d.lo0;
+        if (d.lo1 != f0 || f0 < M || d.n1 != g.ny - 2 * f0 || d.npts != (g.nx - 2 * f0) * d.n1) return 0;
+        if (F >= 0 && F != f0) return 0;
+        F = f0;
+        const int plane = d.disp;
+        const int face = plane <= blo ? 0 : (plane >= bhi ? 1 : -1);
+        if (face < 0) return 0;
+        const int b = face ? bhi : blo;
+        if (f == 5) {
+            if (plane == b) {
+                if (d.nterms != 0) return 0;
+                zero[face] = true;
+                continue;
+            }
+            const int k = face ? plane - b : b - plane;
+            const int src = face ? b - k : b + k;
+            if (k < 1 || k >= M || d.nterms != 1) return 0;
+            if (d.t[0].fl != 5 * 2 + 1 || d.t[0].cslot >= 0 || d.t[0].disp != src - plane) return 0;
+            zc.zc[face][0][k] = d.t[0].coef;
+            img[face][0][k] = true;
+        } else if (f == 7 || f == 8) {
+            const int fi = f == 7 ? 1 : 2;
+            const int k = face ? plane - b : b - 1 - plane;
+            const int src = face ? b - 1 - k : b + k;
+            if (k < 0 || k >= M || d.nterms != 1) return 0;
+            if (d.t[0].fl != f * 2 + 1 || d.t[0].cslot >= 0 || d.t[0].disp != src - plane) return 0;
+            zc.zc[face][fi][k] = d.t[0].coef;
+            img[face][fi][k] = true;
+        } else if (f == 3 || f == 4) {
+            if (M != 2 || plane != b || d.nterms != 9) return 0;
+            for (int k = 0; k < 9; ++k) {
+                const BcTermDev& t = d.t[k];
+                const int want_fl = k < 4 ? 0 : (k < 8 ? 2 : f * 2);
+                const int64_t want_disp = k < 4 ? (k - 2) * g.sx : (k < 8 ? (k - 6) * g.sy : 0);
+                // literal coefficients only (constant media)
+                if (t.fl != want_fl || t.disp != want_disp || t.cslot >= 0) return 0;
+                zc.rc[face][f - 3][k] = t.coef;
+            }
+            rec[face][f - 3] = true;
+        } else {
+            return 0;
+        }
+    }
+    for (int face = 0; face < 2; ++face) {
+        if (!zero[face]) return 0;
+        for (int k = 0; k < M; ++k)
+            if ((k >= 1 && !img[face][0][k]) || !img[face][1][k] || !img[face][2][k]) return 0;
+    }
+    const int nrec = rec[0][0] + rec[0][1] + rec[1][0] + rec[1][1];
+    if (nrec != 0 && nrec != 4) return 0;
+    z.rec = nrec == 4;
+    z.F = F;
+    z.n1 = g.ny - 2 * F;
+    z.npts = (g.nx - 2 * F) * z.n1;
+    if (int e = upload(&c->d_zcoef, &zc, 1, c->stream)) return e;
+    z.coef = c->d_zcoef;
+    c->zf = z;
+    return 0;
+}
+
+// whether this step's stress launch applies the deep z-face statements
+bool zfaces_in_sweep(const OpesciContext* c) {
+    return c->zf.F > 0 && stress_applies_zfaces(c->real_bytes, c->het, c->cfg);
+}
+
 int run_bc_phase(OpesciContext* c, int phase, int t0, int t1) {
     for (const BcLevel& lv : c->levels[phase])
         OPESCI_HIP_TRY(launch_bc(c->g, c->real_bytes, c->storage, c->fptr, c->d_desc, c->d_pool, lv, t0, t1, c->stream));
@@ -343,9 +434,11 @@ int run_bc_phase(OpesciContext* c, int phase, int t0, int t1) {
 
 int step_once(OpesciContext* c, int t0) {
     const int t1 = 1 - t0;
+    const bool zin = zfaces_in_sweep(c);
     OPESCI_HIP_TRY(launch_stress(c->g, c->real_bytes, c->storage, c->het, c->fptr, c->mptr, c->coefs(), t0, t1,
-                                 c->cfg, c->stream));
-    if (int e = run_bc_phase(c, 4, t0, t1)) return e;
+                                 c->cfg, c->stream, zin ? &c->zf : nullptr, c->d_pool));
+    if (!zin)
+        if (int e = run_bc_phase(c, 4, t0, t1)) return e;
     if (int e = run_bc_phase(c, 2, t0, t1)) return e;
     OPESCI_HIP_TRY(launch_velocity(c->g, c->real_bytes, c->storage, c->het, c->fptr, c->mptr, c->coefs(), t0, t1,
                                    c->cfg, c->stream));
@@ -390,7 +483,7 @@ void destroy(OpesciContext* c) {
     void* ptrs[] = {c->d_desc,  c->d_code,    c->d_sol,  c->d_partial, c->d_stage, c->d_rec,
                     c->d_trace, c->d_wavelet, c->d_step, c->d_pool,    c->d_dterms, c->d_gpool,
                     c->d_log,   c->d_ckpt,    c->d_seg,  c->d_obs,     c->d_srclog, c->d_tcells,
-                    c->d_vcells, c->d_scells};
+                    c->d_vcells, c->d_scells, c->d_zcoef};
     for (int f = 0; f < NF; ++f)
         for (int l = 0; l < 2; ++l)
             if (c->abase[f][l]) (void)hipFree(c->abase[f][l]);
@@ -466,6 +559,7 @@ int create(const OpesciModelDesc* d, OpesciContext** out) {
             }
     c->stmts.assign(d->stmts, d->stmts + d->nstmts);
     if (int e = build_program(c.get(), d->terms, d->dterms, d->ndterms, d->nterms)) return e;
+    if (int e = build_zfaces(c.get())) return e;
     if (int e = upload(&c->d_dterms, d->dterms, (size_t)d->ndterms, c->stream)) return e;
     if (int e = upload(&c->d_code, d->bc_code, (size_t)d->ncode, c->stream)) return e;
     // time-invariant media-dependent coefficients, once (the media are ready)
@@ -738,10 +832,12 @@ int opesci_ctx_kernel_times(OpesciContext* ctx, int32_t nsteps, double out_ms[3]
         const int t0 = ctx->steps % 2, t1 = 1 - t0;
         hipEvent_t* e = &ev[(size_t)s * 5];
         OPESCI_HIP_TRY(hipEventRecord(e[0], ctx->stream));
+        const bool zin = zfaces_in_sweep(ctx);
         OPESCI_HIP_TRY(launch_stress(ctx->g, ctx->real_bytes, ctx->storage, ctx->het, ctx->fptr, ctx->mptr,
-                                     ctx->coefs(), t0, t1, ctx->cfg, ctx->stream));
+                                     ctx->coefs(), t0, t1, ctx->cfg, ctx->stream, zin ? &ctx->zf : nullptr,
+                                     ctx->d_pool));
         OPESCI_HIP_TRY(hipEventRecord(e[1], ctx->stream));
-        rc = run_bc_phase(ctx, 4, t0, t1);
+        if (!zin) rc = run_bc_phase(ctx, 4, t0, t1);
         if (!rc) rc = run_bc_phase(ctx, 2, t0, t1);
         OPESCI_HIP_TRY(hipEventRecord(e[2], ctx->stream));
         OPESCI_HIP_TRY(launch_velocity(ctx->g, ctx->real_bytes, ctx->storage, ctx->het, ctx->fptr, ctx->mptr,
@@ -777,10 +873,11 @@ int opesci_ctx_stats(OpesciContext* ctx, int32_t out[4]) {
     if (!ctx || !out) return fail(OPESCI_EINVAL, "null argument");
     int n4 = 0;
     for (const BcLevel& lv : ctx->levels[4]) n4 += lv.count;
-    out[0] = (int)(ctx->levels[2].size() + ctx->levels[3].size() + ctx->levels[4].size());
+    const bool zin = zfaces_in_sweep(ctx);
+    out[0] = (int)(ctx->levels[2].size() + ctx->levels[3].size() + (zin ? 0 : ctx->levels[4].size()));
     out[1] = n4;
-    out[2] = (int)ctx->levels[4].size();
-    out[3] = 0;
+    out[2] = zin ? 0 : (int)ctx->levels[4].size();
+    out[3] = zin ? 1 : 0;
     return 0;
 }
 

@@ -276,12 +276,13 @@ class Context(object):
         _check(self.lib.opesci_ctx_kernel_times(self.h, nsteps, out), 'opesci_ctx_kernel_times')
         return list(out)
 
-    def stats(self):
+    def stats(self, full=False):
         """(boundary launches per step, deep z-face statements of the stress
-        phase, launches they take)."""
+        phase, launches they take[, 1 if the stress sweep applies them in
+        place])."""
         out = (_i32 * 4)()
         _check(self.lib.opesci_ctx_stats(self.h, out), 'opesci_ctx_stats')
-        return tuple(out[:3])
+        return tuple(out[:4] if full else out[:3])
 
     def configure(self, kernel_variant=0, tile_y=0, x_chunk=0):
         """Interior launch configuration: kernel_variant 0 sweep (two z points

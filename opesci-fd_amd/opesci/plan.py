@@ -289,7 +289,10 @@ class _BcBuilder(object):
                         groups[key] = []
                         order.append(key)
                     groups[key].append((num, other))
-            for key in order:
+            # one canonical term order (field, level, plane, offsets): the
+            # boundary launches and the sweep kernels that apply a statement
+            # in place sum its terms in the same order
+            for key in sorted(order):
                 src, lvl, p, offs = key
                 parts = groups[key]
                 if all(not o for _, o in parts):
@@ -523,7 +526,9 @@ def _build_bc(grid, slots):
     b.op_of_stmt = [b.op_of_stmt[i] for i in order]
     b.seq = list(range(len(b.stmts)))
     b.fused_frame = 0
-    if getattr(grid, 'fuse_boundary', False):
+    # split by default where the FP32 pair stress sweep applies the deep rows
+    # in place (constant media); elsewhere they would only add a launch
+    if getattr(grid, 'fuse_boundary', not grid.read and not grid.double):
         dims = [d.value for d in grid.dim]
         b.fused_frame = b.fuse_z(2, 4, dims, grid.margin.value)
     b.schedule()
