@@ -288,6 +288,16 @@ int opesci_ctx_configure(OpesciContext* ctx, int32_t kernel_variant, int32_t til
  * reference counterpart. */
 int opesci_ctx_gradient(OpesciContext* ctx, const float* observed, int64_t count, int32_t checkpoint_every,
                         double* misfit);
+/* opesci_ctx_gradient in two halves, for callers that want the misfit
+ * first and the gradient only on demand (an autograd backward): the forward
+ * pass with checkpoints and J, then the reverse sweep.  Any init, run,
+ * upload or media upload in between drops the forward pass (the reverse
+ * sweep then fails with OPESCI_ESTATE).  `observed` and the host/device
+ * buffers of the media and gradient copies may be host or device memory.
+ * No reference counterpart. */
+int opesci_ctx_misfit_forward(OpesciContext* ctx, const float* observed, int64_t count, int32_t checkpoint_every,
+                              double* misfit);
+int opesci_ctx_gradient_reverse(OpesciContext* ctx);
 int opesci_ctx_download_gradient(OpesciContext* ctx, int32_t media, void* host);
 /* Replace one media array (dense [dim1][dim2][dim3], compute type) and
  * recompute the media-dependent boundary coefficients: the hook through

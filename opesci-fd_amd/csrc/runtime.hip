@@ -93,6 +93,8 @@ struct OpesciContext {
     size_t field_bytes = 0;
     InteriorLaunch cfg{0, 0, 0};
     ZFaceArgs zf;                   // deep z-face statements the pair stress sweep applies (F = 0: none)
+    int grad_every = 0;             // checkpoint interval of the last forward pass (0: none pending)
+    bool grad_keep_t0 = false;
     ZCoef* d_zcoef = nullptr;       // their coefficient block (device)
 
     const void* coefs() const { return real_bytes == 8 ? (const void*)&cd : (const void*)&cf; }
@@ -607,6 +609,7 @@ int create(const OpesciModelDesc* d, OpesciContext** out) {
 }
 
 int ctx_init(OpesciContext* c) {
+    c->grad_every = 0;
     for (int f = 0; f < NF; ++f)
         for (int l = 0; l < 2; ++l) OPESCI_HIP_TRY(launch_zero(c->base[f][l], c->field_bytes, c->stream));
     OPESCI_HIP_TRY(launch_initial(c->g, c->real_bytes, c->storage, c->fptr, c->mptr, c->d_sol, c->sol_off,
@@ -622,6 +625,7 @@ int ctx_init(OpesciContext* c) {
 
 int ctx_run(OpesciContext* c, int n, double* seconds) {
     if (!c->initialised) return fail(OPESCI_ESTATE, "opesci_ctx_init must run before the time loop");
+    c->grad_every = 0;
     if (n < 0) return fail(OPESCI_EINVAL, "negative step count");
     if (c->desc.use_graph && n >= 2) {
         if (int e = build_graph(c)) return e;
@@ -681,6 +685,7 @@ int ctx_download(OpesciContext* c, int f, int level, void* host) {
 
 int ctx_upload(OpesciContext* c, int f, int level, const void* host) {
     if (f < 0 || f >= NF || level < 0 || level > 1 || !host) return fail(OPESCI_EINVAL, "bad upload arguments");
+    c->grad_every = 0;
     const size_t dense = (size_t)c->g.nx * c->g.ny * c->g.nz;
     OPESCI_HIP_TRY(hipMemcpyAsync(c->d_stage, host, dense * c->real_bytes, hipMemcpyHostToDevice, c->stream));
     OPESCI_HIP_TRY(launch_pack(c->g, c->real_bytes, c->storage, c->d_stage, c->fptr[f][level], c->stream));
@@ -1084,22 +1089,31 @@ static bool steps_keep_t0(const OpesciContext* c) {
     return true;
 }
 
-int opesci_ctx_gradient(OpesciContext* c, const float* observed, int64_t count, int32_t every, double* misfit) {
+// Forward pass of the gradient: checkpoints every `every` steps and the
+// misfit J of the recorded U, V, W traces.  Leaves the final state in the
+// fields (a valid stepped state) and the checkpoints for
+// opesci_ctx_gradient_reverse.
+int opesci_ctx_misfit_forward(OpesciContext* c, const float* observed, int64_t count, int32_t every,
+                              double* misfit) {
     if (!c || !observed || every <= 0) return fail(OPESCI_EINVAL, "bad arguments");
     if (!c->het || c->storage != OPESCI_STORE_NATIVE)
         return fail(OPESCI_EUNSUPPORTED, "gradients need the read path (media arrays) and native storage");
     if (c->nrec <= 0) return fail(OPESCI_EINVAL, "gradients need receivers");
     const int N = c->desc.ntsteps;
     if (count != (int64_t)N * c->nrec * 3) return fail(OPESCI_EINVAL, "observed must hold ntsteps * nrec * 3 values");
-    // the fields end up holding a recomputed state (or, after a failure, a
-    // partial one): on every return, stepping needs opesci_ctx_init again
-    struct NeedsInit {
+    c->grad_every = 0;
+    // a failure part way leaves a partial state: stepping needs
+    // opesci_ctx_init again
+    struct InitOnFailure {
         OpesciContext* c;
-        ~NeedsInit() {
-            c->initialised = false;
-            c->steps = 0;
+        bool ok = false;
+        ~InitOnFailure() {
+            if (!ok) {
+                c->initialised = false;
+                c->steps = 0;
+            }
         }
-    } needs_init{c};
+    } guard{c};
     const int nck = (N + every - 1) / every;
     const size_t state = (size_t)2 * NF * c->field_bytes;
     // Segment scheme.  keep_t0: the segment's forward pass logs every step and
@@ -1113,7 +1127,10 @@ int opesci_ctx_gradient(OpesciContext* c, const float* observed, int64_t count, 
     const size_t slot = keep_t0 ? (size_t)NF * c->field_bytes : state;
     if (int e = adj_alloc(c, std::max(nck, 1), slot * every, keep_t0 ? every : 1)) return e;
     if (!c->d_obs) OPESCI_HIP_TRY(hipMalloc((void**)&c->d_obs, (size_t)count * sizeof(float)));
-    OPESCI_HIP_TRY(hipMemcpyAsync(c->d_obs, observed, (size_t)count * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    OPESCI_HIP_TRY(hipMemcpyAsync(c->d_obs, observed, (size_t)count * sizeof(float), hipMemcpyDefault, c->stream));
+    std::vector<float> obs_host((size_t)count);
+    OPESCI_HIP_TRY(hipMemcpyAsync(obs_host.data(), c->d_obs, (size_t)count * sizeof(float), hipMemcpyDeviceToHost,
+                                  c->stream));
     // forward with checkpoints
     if (int e = ctx_init(c)) return e;
     for (int n = 0; n < N; ++n) {
@@ -1128,10 +1145,37 @@ int opesci_ctx_gradient(OpesciContext* c, const float* observed, int64_t count, 
     double J = 0;
     for (int64_t i = 0; i < (int64_t)N * c->nrec; ++i)
         for (int k = 0; k < 3; ++k) {
-            const double r = tr[(size_t)i * 4 + k] - (double)observed[(size_t)i * 3 + k];
+            const double r = tr[(size_t)i * 4 + k] - (double)obs_host[(size_t)i * 3 + k];
             J += 0.5 * r * r;
         }
     if (misfit) *misfit = J;
+    c->grad_every = every;
+    c->grad_keep_t0 = keep_t0;
+    guard.ok = true;
+    return 0;
+}
+
+// Reverse sweep after opesci_ctx_misfit_forward: media gradients into the
+// context's gradient arrays.  Leaves a recomputed early state in the fields.
+int opesci_ctx_gradient_reverse(OpesciContext* c) {
+    if (!c) return fail(OPESCI_EINVAL, "null context");
+    if (c->grad_every <= 0) return fail(OPESCI_ESTATE, "no forward pass: call opesci_ctx_misfit_forward first");
+    // the fields end up holding a recomputed state (or, after a failure, a
+    // partial one): on every return, stepping needs opesci_ctx_init again,
+    // and a new forward pass precedes the next reverse sweep
+    struct NeedsInit {
+        OpesciContext* c;
+        ~NeedsInit() {
+            c->initialised = false;
+            c->steps = 0;
+            c->grad_every = 0;
+        }
+    } needs_init{c};
+    const int N = c->desc.ntsteps, every = c->grad_every;
+    const int nck = (N + every - 1) / every;
+    const size_t state = (size_t)2 * NF * c->field_bytes;
+    const bool keep_t0 = c->grad_keep_t0;
+    const size_t slot = keep_t0 ? (size_t)NF * c->field_bytes : state;
     // reverse sweep, segment by segment
     for (int f = 0; f < NF; ++f)
         for (int l = 0; l < 2; ++l) OPESCI_HIP_TRY(launch_zero(c->abase[f][l], c->field_bytes, c->stream));
@@ -1231,12 +1275,17 @@ int opesci_ctx_gradient(OpesciContext* c, const float* observed, int64_t count, 
     return 0;
 }
 
+int opesci_ctx_gradient(OpesciContext* c, const float* observed, int64_t count, int32_t every, double* misfit) {
+    if (int e = opesci_ctx_misfit_forward(c, observed, count, every, misfit)) return e;
+    return opesci_ctx_gradient_reverse(c);
+}
+
 int opesci_ctx_download_gradient(OpesciContext* c, int32_t media, void* host) {
     if (!c || !host || media < 0 || media >= OPESCI_M_COUNT) return fail(OPESCI_EINVAL, "bad arguments");
     if (!c->gbase[media]) return fail(OPESCI_ESTATE, "no gradient: call opesci_ctx_gradient first");
     const size_t dense = (size_t)c->g.nx * c->g.ny * c->g.nz;
     OPESCI_HIP_TRY(launch_unpack(c->g, c->real_bytes, OPESCI_STORE_NATIVE, c->gptr[media], c->d_stage, c->stream));
-    OPESCI_HIP_TRY(hipMemcpyAsync(host, c->d_stage, dense * c->real_bytes, hipMemcpyDeviceToHost, c->stream));
+    OPESCI_HIP_TRY(hipMemcpyAsync(host, c->d_stage, dense * c->real_bytes, hipMemcpyDefault, c->stream));
     OPESCI_HIP_TRY(hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -1246,7 +1295,7 @@ int opesci_ctx_download_media(OpesciContext* c, int32_t media, void* host) {
     if (!c->het || !c->mptr[media]) return fail(OPESCI_EUNSUPPORTED, "media arrays exist on the read path only");
     const size_t dense = (size_t)c->g.nx * c->g.ny * c->g.nz;
     OPESCI_HIP_TRY(launch_unpack(c->g, c->real_bytes, OPESCI_STORE_NATIVE, c->mptr[media], c->d_stage, c->stream));
-    OPESCI_HIP_TRY(hipMemcpyAsync(host, c->d_stage, dense * c->real_bytes, hipMemcpyDeviceToHost, c->stream));
+    OPESCI_HIP_TRY(hipMemcpyAsync(host, c->d_stage, dense * c->real_bytes, hipMemcpyDefault, c->stream));
     OPESCI_HIP_TRY(hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -1254,8 +1303,9 @@ int opesci_ctx_download_media(OpesciContext* c, int32_t media, void* host) {
 int opesci_ctx_upload_media(OpesciContext* c, int32_t media, const void* host) {
     if (!c || !host || media < 0 || media >= OPESCI_M_COUNT) return fail(OPESCI_EINVAL, "bad arguments");
     if (!c->het || !c->mptr[media]) return fail(OPESCI_EUNSUPPORTED, "media arrays exist on the read path only");
+    c->grad_every = 0;
     const size_t dense = (size_t)c->g.nx * c->g.ny * c->g.nz;
-    OPESCI_HIP_TRY(hipMemcpyAsync(c->d_stage, host, dense * c->real_bytes, hipMemcpyHostToDevice, c->stream));
+    OPESCI_HIP_TRY(hipMemcpyAsync(c->d_stage, host, dense * c->real_bytes, hipMemcpyDefault, c->stream));
     OPESCI_HIP_TRY(launch_pack_media(c->g, c->real_bytes, c->d_stage, c->mptr[media], c->stream));
     if (c->d_pool)
         for (int ph = 0; ph < 5; ++ph)

@@ -57,7 +57,7 @@ EXPORTS = ('opesci_model_load', 'opesci_execute', 'opesci_convergence', 'opesci_
            'opesci_ctx_l2', 'opesci_ctx_download', 'opesci_ctx_upload', 'opesci_ctx_receivers', 'opesci_ctx_receivers_f64',
            'opesci_ctx_steps', 'opesci_ctx_kernel_times', 'opesci_ctx_stats', 'opesci_ctx_configure',
            'opesci_batch_create', 'opesci_batch_run', 'opesci_batch_destroy', 'opesci_ctx_gradient',
-           'opesci_ctx_download_gradient', 'opesci_ctx_upload_media', 'opesci_ctx_download_media',
+           'opesci_ctx_misfit_forward', 'opesci_ctx_gradient_reverse', 'opesci_ctx_download_gradient', 'opesci_ctx_upload_media', 'opesci_ctx_download_media',
            'opesci_ctx_l2_noalloc', 'opesci_last_error',
            'opesci_abi_version', 'opesci_device_info')
 
@@ -92,6 +92,8 @@ def load(path=None):
         lib.opesci_batch_run.argtypes = [_vp, _i32, ctypes.POINTER(_f64)]
         lib.opesci_batch_destroy.argtypes = [_vp]
         lib.opesci_ctx_gradient.argtypes = [_vp, _vp, ctypes.c_int64, _i32, ctypes.POINTER(_f64)]
+        lib.opesci_ctx_misfit_forward.argtypes = [_vp, _vp, ctypes.c_int64, _i32, ctypes.POINTER(_f64)]
+        lib.opesci_ctx_gradient_reverse.argtypes = [_vp]
         lib.opesci_ctx_download_gradient.argtypes = [_vp, _i32, _vp]
         lib.opesci_ctx_upload_media.argtypes = [_vp, _i32, _vp]
         lib.opesci_ctx_download_media.argtypes = [_vp, _i32, _vp]
@@ -104,6 +106,10 @@ def load(path=None):
             raise OSError('libopesci_hip ABI %d, expected %d' % (lib.opesci_abi_version(), ABI_VERSION))
         _LIB[path] = lib
     return _LIB[path]
+
+
+def _is_device_tensor(a):
+    return hasattr(a, 'data_ptr') and getattr(a, 'is_cuda', False)
 
 
 def _check(rc, what):
@@ -290,22 +296,63 @@ class Context(object):
         lane; results are the same for every one."""
         _check(self.lib.opesci_ctx_configure(self.h, kernel_variant, tile_y, x_chunk), 'opesci_ctx_configure')
 
+    def _observed(self, observed):
+        """(pointer, size, keep-alive) of the observed traces: a float32
+        device tensor as it is (torch's stream synchronised first), anything
+        else as a contiguous float32 host array."""
+        if _is_device_tensor(observed):
+            import torch
+            t = observed.detach().to(torch.float32).contiguous()
+            torch.cuda.current_stream(t.device).synchronize()
+            return t.data_ptr(), t.numel(), t
+        obs = np.ascontiguousarray(observed, dtype=np.float32)
+        return obs.ctypes.data, obs.size, obs
+
     def gradient(self, observed, checkpoint_every=32):
         """Misfit J = 1/2 sum (U,V,W receiver traces - observed)^2 and, left on
         the device, dJ/d(media arrays) by the adjoint (see download_gradient).
-        observed: (ntsteps, nrec, 3) float32."""
-        obs = np.ascontiguousarray(observed, dtype=np.float32)
+        observed: (ntsteps, nrec, 3) float32, host array or device tensor."""
+        ptr, n, keep = self._observed(observed)
         j = _f64()
-        _check(self.lib.opesci_ctx_gradient(self.h, obs.ctypes.data, obs.size, int(checkpoint_every), ctypes.byref(j)),
+        _check(self.lib.opesci_ctx_gradient(self.h, ptr, n, int(checkpoint_every), ctypes.byref(j)),
                'opesci_ctx_gradient')
+        del keep
         return j.value
 
-    def download_gradient(self, media):
-        """dJ/d(media array) of the last gradient() call, dense (dim1, dim2, dim3)."""
+    def misfit_forward(self, observed, checkpoint_every=32):
+        """First half of gradient(): the forward pass with checkpoints and
+        the misfit J; gradient_reverse() then runs the adjoint."""
+        ptr, n, keep = self._observed(observed)
+        j = _f64()
+        _check(self.lib.opesci_ctx_misfit_forward(self.h, ptr, n, int(checkpoint_every), ctypes.byref(j)),
+               'opesci_ctx_misfit_forward')
+        del keep
+        return j.value
+
+    def gradient_reverse(self):
+        """Second half of gradient(): the reverse sweep after misfit_forward()."""
+        _check(self.lib.opesci_ctx_gradient_reverse(self.h), 'opesci_ctx_gradient_reverse')
+
+    def download_gradient(self, media, out=None):
+        """dJ/d(media array) of the last reverse sweep, dense (dim1, dim2, dim3);
+        into `out` (a contiguous device tensor of the compute type) if given."""
         mid = MEDIA_IDS[media] if isinstance(media, str) else int(media)
+        if out is not None:
+            import torch
+            self._check_tensor(out)
+            torch.cuda.current_stream(out.device).synchronize()
+            _check(self.lib.opesci_ctx_download_gradient(self.h, mid, out.data_ptr()), 'opesci_ctx_download_gradient')
+            return out
         a = np.zeros(self.plan.dims, dtype=self.dtype)
         _check(self.lib.opesci_ctx_download_gradient(self.h, mid, a.ctypes.data), 'opesci_ctx_download_gradient')
         return a
+
+    def _check_tensor(self, t):
+        import torch
+        want = torch.float64 if self.dtype == np.float64 else torch.float32
+        if not _is_device_tensor(t) or t.dtype != want or not t.is_contiguous() \
+                or tuple(t.shape) != tuple(self.plan.dims):
+            raise ValueError('expected a contiguous %s device tensor of shape %s' % (want, tuple(self.plan.dims)))
 
     def download_media(self, media):
         """One media array as the device holds it (read path), dense (dim1, dim2, dim3)."""
@@ -318,6 +365,13 @@ class Context(object):
         """Replace one media array (dense, compute type) and refresh the
         media-dependent boundary coefficients."""
         mid = MEDIA_IDS[media] if isinstance(media, str) else int(media)
+        if _is_device_tensor(arr):
+            import torch
+            t = arr.detach().to(torch.float64 if self.dtype == np.float64 else torch.float32).contiguous()
+            self._check_tensor(t)
+            torch.cuda.current_stream(t.device).synchronize()
+            _check(self.lib.opesci_ctx_upload_media(self.h, mid, t.data_ptr()), 'opesci_ctx_upload_media')
+            return
         a = np.ascontiguousarray(arr, dtype=self.dtype)
         if a.shape != tuple(self.plan.dims):
             raise ValueError('media array shape %s, grid %s' % (a.shape, self.plan.dims))

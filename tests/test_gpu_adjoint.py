@@ -14,7 +14,11 @@ by 10 % in a Gaussian blob.  Checks:
   * torch autograd through ElasticMisfit: dJ/d(rho, lambda, mu) at 16
     seeded points, relative step 1e-3 (tolerance 1e-4 of the local value
     plus 1e-6 of the largest gradient entry), three of them again with
-    1e-5 steps at 1e-5 tolerance.
+    1e-5 steps at 1e-5 tolerance;
+  * FP32 contexts: the pair-sweep transposes against the gather kernels,
+    FP32 against FP64, and the spread of two FP32 runs;
+  * the gradient in two halves (forward, then reverse) and ElasticMisfit on
+    device tensors against the one-call gradient.
 """
 import numpy as np
 import pytest
@@ -161,6 +165,50 @@ def test_autograd_rho_lambda_mu_finite_differences():
             assert abs(an - fd) <= 1e-5 * abs(fd) + 1e-8 * gmax, (k, idx, an, fd)
 
 
+def test_split_gradient_and_device_tensors_match_gradient():
+    """misfit_forward + gradient_reverse equal gradient(): J bit for bit, the
+    gradients to 1e-12 relative (the boundary transposes accumulate with
+    atomics, so two reverse sweeps may add in a different order), and
+    ElasticMisfit on device tensors (media uploaded and gradients downloaded
+    device to device, observed traces on the device) equals it on host
+    tensors to 1e-12 of each array's largest entry (torch's device and host
+    kernels for the media averaging may contract or order FP64 operations
+    differently)."""
+    from opesci.autograd import ElasticMisfit, effective_media
+    g, rho, lam, mu = build(12, 4, 20)
+    with Context(g.lower()) as ctx:
+        obs = observed_from(ctx, rho, lam, mu)
+        eff = {k: v.numpy().copy() for k, v in effective_media(torch.from_numpy(rho), torch.from_numpy(lam),
+                                                               torch.from_numpy(mu)).items()}
+        for k, v in eff.items():
+            ctx.upload_media(k, v)
+        J0 = ctx.gradient(obs, checkpoint_every=6)
+        g0 = {k: ctx.download_gradient(k) for k in EFFECTIVE}
+        J1 = ctx.misfit_forward(obs, checkpoint_every=6)
+        ctx.gradient_reverse()
+        g1 = {k: ctx.download_gradient(k) for k in EFFECTIVE}
+        assert J1 == J0
+        for k in EFFECTIVE:
+            assert np.allclose(g1[k], g0[k], rtol=1e-12, atol=1e-14 * np.abs(g0[k]).max()), k
+        grads = {}
+        for dev in ('cpu', 'cuda'):
+            ts = [torch.from_numpy(a).to(dev).requires_grad_(True) for a in (rho, lam, mu)]
+            o = torch.from_numpy(np.ascontiguousarray(obs)).to(dev)
+            J = ElasticMisfit.apply(*ts, ctx, o, 6)
+            assert J.device.type == dev
+            assert float(J.detach().cpu()) == pytest.approx(J0, rel=1e-12)
+            J.backward()
+            grads[dev] = [t.grad.detach().cpu().numpy() for t in ts]
+        for a, b in zip(grads['cpu'], grads['cuda']):
+            assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max()
+        # device-side download of the raw gradient equals the host one
+        out = torch.empty(tuple(g.lower().dims), dtype=torch.float64, device='cuda')
+        ctx.misfit_forward(obs, checkpoint_every=6)
+        ctx.gradient_reverse()
+        ctx.download_gradient('mu', out)
+        assert np.allclose(out.cpu().numpy(), g0['mu'], rtol=1e-12, atol=1e-14 * np.abs(g0['mu']).max())
+
+
 @pytest.mark.parametrize("order", [4, 8])
 def test_gradient_matches_fp64_cpu_adjoint(order):
     """Every cell of the eight effective-media gradients against the FP64 CPU
@@ -248,8 +296,68 @@ def test_gradient_errors_and_state():
         ctx.init()
         ctx.run()
         assert np.array_equal(ctx.receivers()[:, :, :3], obs)   # the model is unchanged
+        # the two halves: the reverse sweep needs its own forward pass
+        with pytest.raises(RuntimeError, match='forward'):
+            ctx.gradient_reverse()
+        J = ctx.misfit_forward(obs, checkpoint_every=4)
+        ctx.init()                                 # drops the forward pass
+        with pytest.raises(RuntimeError, match='forward'):
+            ctx.gradient_reverse()
+        assert ctx.misfit_forward(obs, checkpoint_every=4) == J
+        ctx.upload_media('lambda', lam)            # so does new media
+        with pytest.raises(RuntimeError, match='forward'):
+            ctx.gradient_reverse()
+        assert ctx.misfit_forward(obs, checkpoint_every=4) == J
+        ctx.gradient_reverse()
+        with pytest.raises(RuntimeError, match='forward'):
+            ctx.gradient_reverse()                 # once per forward pass
     plain, _ = eigen_model(10, 4, 4, double=True)    # constant media: no media arrays
     with Context(plain.lower()) as ctx:
         ctx.init()
         with pytest.raises(RuntimeError, match='read path'):
             ctx.gradient(np.zeros((4, 0, 3), np.float32))
+
+
+@pytest.mark.parametrize("order", [4, 8])
+def test_fp32_gradient_against_fp64_and_simple_kernels(order):
+    """FP32 contexts, bounds stated before the run:
+      * kernel variant 0 (the pair sweeps as interior transposes, in place)
+        against variant 1 (per-cell gather kernels): every effective-media
+        gradient within 1e-5 of its largest entry (the same operations,
+        FP32 sums in a different order);
+      * against the FP64 gradient of the same model: every array within
+        1e-3 relative L2 (FP32 rounding of the forward and adjoint fields
+        and of the gradient sums);
+      * two FP32 gradients in a row (the boundary transposes accumulate with
+        atomics, so the order of FP32 additions can change): within 1e-5 of
+        each array's largest entry."""
+    from opesci.autograd import effective_media
+    N, steps = 14, 36
+    g64, rho, lam, mu = adjoint_model(N, order, steps)
+    with Context(g64.lower()) as ctx:
+        obs = observed_from(ctx, rho, lam, mu)
+        eff = {k: v.numpy().copy() for k, v in effective_media(torch.from_numpy(rho), torch.from_numpy(lam),
+                                                               torch.from_numpy(mu)).items()}
+        for k, v in eff.items():
+            ctx.upload_media(k, v)
+        ctx.gradient(obs, checkpoint_every=8)
+        ref = {k: ctx.download_gradient(k) for k in EFFECTIVE}
+    res = {}
+    for variant, rep in ((0, 0), (0, 1), (1, 0)):
+        g32, _, _, _ = adjoint_model(N, order, steps, double=False)
+        plan = g32.lower()
+        plan.kernel_variant = variant
+        with Context(plan) as ctx:
+            for k, v in eff.items():
+                ctx.upload_media(k, v.astype(np.float32))
+            ctx.gradient(obs, checkpoint_every=8)
+            res[(variant, rep)] = {k: ctx.download_gradient(k).astype(np.float64) for k in EFFECTIVE}
+    for k in EFFECTIVE:
+        scale = np.abs(ref[k]).max()
+        if scale == 0:
+            continue
+        a, b, c = res[(0, 0)][k], res[(0, 1)][k], res[(1, 0)][k]
+        assert np.abs(a - c).max() <= 1e-5 * scale, (k, 'variant', np.abs(a - c).max() / scale)
+        assert np.abs(a - b).max() <= 1e-5 * scale, (k, 'spread', np.abs(a - b).max() / scale)
+        rel = np.linalg.norm(a - ref[k]) / np.linalg.norm(ref[k])
+        assert rel <= 1e-3, (k, 'fp32 vs fp64', rel)

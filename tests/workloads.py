@@ -69,8 +69,8 @@ def batch_specs(count=64, seed=4, lo=48, hi=160, orders=(2, 4, 8)):
     return out
 
 
-def adjoint_model(N, order, steps, seed=5):
-    """Small inversion case: seeded smooth medium, fields at rest, FP64,
+def adjoint_model(N, order, steps, seed=5, double=True):
+    """Small inversion case: seeded smooth medium, fields at rest, FP64 (or FP32),
     Ricker source (x100) in Txx/Tyy/Tzz at the centre, a receiver patch two
     cells below the top face plus three interior receivers.  Returns the grid
     and FP64 rho, lambda, mu."""
@@ -81,7 +81,7 @@ def adjoint_model(N, order, steps, seed=5):
     dx = 1.0 / n
     dt = 0.4 * 0.495 * dx / float(vp.max())
     g = opesci.eigenwave3d((1.0, 1.0, 1.0), (n, n, n), dt, dt * (steps + 0.5), accuracy_order=[2, order, order, order],
-                           double=True, media_arrays=(rho, vp, vs), initial='zero', o_converge=False)
+                           double=double, media_arrays=(rho, vp, vs), initial='zero', o_converge=False)
     m = order // 2
     c = m + n // 2
     g.set_source(ricker(20.0, dt, steps, t0=6 * dt) * 100.0, index=(c, c, c))
