@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define OPESCI_ABI_VERSION 1
+#define OPESCI_ABI_VERSION 2
 #define OPESCI_NFIELDS 9
 #define OPESCI_MAX_HALF_ORDER 4
 
@@ -121,6 +121,14 @@ typedef struct OpesciBcDTerm {
 } OpesciBcDTerm;
 
 /* Everything the generator bakes into the reference's emitted TU. */
+/* OpesciModelDesc.flags: run both half steps of a time step in one sweep
+ * (the fused step) over the rows clear of the x and y faces, where it
+ * applies (FP32 arithmetic, constant media, half order 1 or 2, the deep
+ * z-face statements in the stress sweep).  Results equal the separate
+ * launches bit for bit; on MI355X it is slower than they are (256^3, 4th
+ * order: 694 us against 386 us per step), so it is off by default. */
+#define OPESCI_FLAG_FUSED_STEP 1
+
 typedef struct OpesciModelDesc {
     int32_t abi_version;
     int32_t dims[3];        /* dim1..dim3 incl. ghosts: grid_size + 1 + 2m */
@@ -129,7 +137,7 @@ typedef struct OpesciModelDesc {
     int32_t real_bytes;     /* OPESCI_REAL_F32 / OPESCI_REAL_F64 */
     int32_t storage;        /* OPESCI_STORE_* (16-bit storage needs real_bytes 4) */
     int32_t media_arrays;   /* 0 constant media, 1 arrays from rho/vp/vs */
-    int32_t flags;          /* reserved, 0 */
+    int32_t flags;          /* OPESCI_FLAG_* bits, 0 by default */
     double dt;
     double spacing[3];
     double media_const[OPESCI_M_COUNT];  /* constant media values (float-rounded as the reference declares them) */
@@ -248,15 +256,18 @@ int opesci_ctx_receivers(OpesciContext* ctx, float* host, int64_t capacity_float
 int opesci_ctx_receivers_f64(OpesciContext* ctx, double* host, int64_t capacity);
 /* steps completed since init */
 int opesci_ctx_steps(OpesciContext* ctx, int32_t* steps);
-/* average device time per launch of each hot kernel over the last run
- * (HIP events on the context stream): [0] stress, [1] velocity, [2] all BC launches of a step */
+/* average device time per step of the hot kernels over nsteps steps (HIP
+ * events on the context stream): [0] stress, [1] velocity, [2] all BC
+ * launches of a step; with the fused step [0] is the fused kernel and [1]
+ * the stress and velocity launches of the frame around its box */
 int opesci_ctx_kernel_times(OpesciContext* ctx, int32_t nsteps, double out_ms[3]);
-/* Boundary execution plan of a context: out[0] = boundary launches per time
- * step, out[1] = phase-4 statements (deep z-face rows of the stress phase),
+/* Execution plan of a context: out[0] = boundary launches per time step,
+ * out[1] = phase-4 statements (deep z-face rows of the stress phase),
  * out[2] = launches they take (0 or 1), out[3] = 1 when the stress sweep
- * applies them in place (FP32 pair sweeps), else 0.  No reference
- * counterpart. */
-int opesci_ctx_stats(OpesciContext* ctx, int32_t out[4]);
+ * applies them in place (FP32 pair sweeps), else 0, out[4] = the fused
+ * step's margin F (its box is [F, dim - F) in x and y; 0: separate stress
+ * and velocity launches).  No reference counterpart. */
+int opesci_ctx_stats(OpesciContext* ctx, int32_t out[5]);
 
 /* Launch configuration of the interior kernels (kernel_variant 0 = sweep,
  * two z points per lane for FP32 arithmetic, tile_y 0 | 4 | 8 | 16: stress

@@ -140,6 +140,7 @@ template <typename S>
 __device__ __forceinline__ void stnt2(S* p, float a, float b) {
     __builtin_nontemporal_store(Pair<S>::join(a, b), reinterpret_cast<uint32_t*>(p));
 }
+
 // FP32 storage: a lane's two points as one 8-byte access (z - m even is
 // 8-byte aligned: z = m starts a 128-byte line)
 typedef float f32x2_t __attribute__((ext_vector_type(2)));
@@ -208,6 +209,17 @@ struct InteriorLaunch {
     int x_chunk;
 };
 
+// Up to four (x, y) boxes of interior rows, all of z, that one launch of the
+// pair sweeps covers, blocks numbered box by box (the launch fills first,
+// tyn and xc from the tile shape).  The whole interior is one box; the
+// rows around the fused step's box are four.
+struct Boxes {
+    int n = 0;
+    int first[5] = {};                     // first logical block of each box, first[n] = blocks
+    int x0[4] = {}, x1[4] = {}, y0[4] = {}, y1[4] = {};
+    int tyn[4] = {}, xc[4] = {};           // y tiles and x chunk of each box
+};
+
 
 // Free-surface statements of the stress phase on the z faces, applied by the
 // FP32 pair stress sweep in place, for the deep rows x, y in [F, dim - F)
@@ -238,13 +250,23 @@ struct ZFaceArgs {
     const ZCoef* coef = nullptr;
 };
 
+// `boxes` (x0..y1 and n set) restricts the update to those rows; it needs
+// the pair sweeps (FP32 arithmetic, kernel variant 0)
 hipError_t launch_stress(const Geom& g, int real_bytes, int storage, bool het, void* fields, const void* media,
                          const void* coefs, int t0, int t1, const InteriorLaunch& cfg, hipStream_t s,
-                         const ZFaceArgs* zf = nullptr, const void* pool = nullptr);
+                         const ZFaceArgs* zf = nullptr, const void* pool = nullptr, const Boxes* boxes = nullptr);
 // whether launch_stress applies ZFaceArgs for this configuration
 bool stress_applies_zfaces(int real_bytes, bool het, const InteriorLaunch& cfg);
 hipError_t launch_velocity(const Geom& g, int real_bytes, int storage, bool het, void* fields, const void* media,
-                           const void* coefs, int t0, int t1, const InteriorLaunch& cfg, hipStream_t s);
+                           const void* coefs, int t0, int t1, const InteriorLaunch& cfg, hipStream_t s,
+                           const Boxes* boxes = nullptr);
+// Fused step (both half steps in one x sweep) over the rows of box (one box,
+// x0..y1 set) for FP32 arithmetic on constant media with the deep z faces
+// in the sweep; x_chunk 0 picks the chunk.  fused_step_builds(m): whether
+// it is built for half order m.
+hipError_t launch_step_fused(const Geom& g, int storage, void* fields, const void* coefs, int t0, int t1,
+                             const ZFaceArgs& zf, const Boxes& box, int x_chunk, hipStream_t s);
+bool fused_step_builds(int m);
 
 // auxiliary kernels (aux.hip)
 // Boundary program as the device runs it: one self-contained descriptor per

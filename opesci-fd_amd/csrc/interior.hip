@@ -11,6 +11,9 @@
 //   T_ii += sum_j m_ij sum_k cn[i][j][k] (V_j[+k] - V_j[-1-k])          along j
 //   T_ij += m_s (sum_k cs[s][0][k] (V_i[+1+k] - V_i[-k]) along j + ... along i)
 //   V_i  += m_i sum_j sum_k cv[i][j][k] (T[a+k] - T[a-1-k])             along j
+// The velocity sum is one fma chain over j = y, z, x: the fused step forms
+// its y and z part while the stresses of an x plane are staged and completes
+// it with the x part M planes later, in the same order.
 // with m = 1 for constant media, and lambda(+2mu) / mu_ij / beta_i arrays in
 // the read path.  Arithmetic in the compute type R (float or double); fields
 // stored as S (float, double, _Float16, bf16).
@@ -31,6 +34,8 @@
 // All three keep one accumulation order, so their results agree bit for bit.
 #include "common.h"
 
+#include <algorithm>
+
 namespace opesci {
 
 template <typename R>
@@ -44,6 +49,38 @@ __device__ __forceinline__ int logical_block(int nblocks) {
     const int per = (nblocks + 7) / 8;
     const int l = (b % 8) * per + b / 8;
     return l < nblocks ? l : -1;
+}
+
+// Origin of logical block l of a box list: z tile start, first row, row
+// end of its box, x range.  The box is picked with constant indices (the
+// list stays in kernel arguments).
+struct TileOrigin {
+    int z0, y0, yend, xb, xe;
+};
+template <int TY>
+__device__ __forceinline__ TileOrigin tile_of(const Boxes& B, int l, int tz, int M) {
+    int first = B.first[0], x0 = B.x0[0], x1 = B.x1[0], y0 = B.y0[0], y1 = B.y1[0], tyn = B.tyn[0], xc = B.xc[0];
+#pragma unroll
+    for (int i = 1; i < 4; ++i)
+        if (i < B.n && l >= B.first[i]) {
+            first = B.first[i];
+            x0 = B.x0[i];
+            x1 = B.x1[i];
+            y0 = B.y0[i];
+            y1 = B.y1[i];
+            tyn = B.tyn[i];
+            xc = B.xc[i];
+        }
+    const int loc = l - first;
+    const int bz = loc % tz, by = (loc / tz) % tyn;
+    const int bx = loc / (tz * tyn);
+    TileOrigin o;
+    o.z0 = M + bz * 128;
+    o.y0 = y0 + by * TY;
+    o.yend = y1;
+    o.xb = x0 + bx * xc;
+    o.xe = min(o.xb + xc, x1);
+    return o;
 }
 
 // ------------------------------------------------------------------ gather
@@ -123,8 +160,10 @@ __global__ __launch_bounds__(256) void velocity_gather(Geom g, FieldPtrs<S> F, M
         for (int i = 0; i < 3; ++i) {
             R acc = C::ld(F.f[i][t0] + q);
             R sum = 0;
+            // derivative terms along y, then z, then x
 #pragma unroll
-            for (int j = 0; j < 3; ++j) {
+            for (int jj = 0; jj < 3; ++jj) {
+                const int j = jj == 2 ? 0 : jj + 1;
                 const int a = (i == j) ? 1 : 0;
                 const S* t = F.f[src[i][j]][t1] + q;
 #pragma unroll
@@ -439,23 +478,23 @@ __global__ __launch_bounds__(64 * TY) void velocity_sweep(Geom g, FieldPtrs<S> F
             const int64_t q = (int64_t)x * g.sx + col;
             R su = 0, sv = 0, sw = 0;
 #pragma unroll
-            for (int k = 0; k < M; ++k) su = fmac(c.cv[0][0][k], qXX[M + k] - qXX[M - 1 - k], su);
-#pragma unroll
             for (int k = 0; k < M; ++k) su = fmac(c.cv[0][1][k], LXY[me + k * P::W] - LXY[me - (1 + k) * P::W], su);
 #pragma unroll
             for (int k = 0; k < M; ++k) su = fmac(c.cv[0][2][k], LXZ[me + k] - LXZ[me - 1 - k], su);
 #pragma unroll
-            for (int k = 0; k < M; ++k) sv = fmac(c.cv[1][0][k], qXY[M + k] - qXY[M - 1 - k], sv);
+            for (int k = 0; k < M; ++k) su = fmac(c.cv[0][0][k], qXX[M + k] - qXX[M - 1 - k], su);
 #pragma unroll
             for (int k = 0; k < M; ++k) sv = fmac(c.cv[1][1][k], LYY[me + (1 + k) * P::W] - LYY[me - k * P::W], sv);
 #pragma unroll
             for (int k = 0; k < M; ++k) sv = fmac(c.cv[1][2][k], LYZ[me + k] - LYZ[me - 1 - k], sv);
 #pragma unroll
-            for (int k = 0; k < M; ++k) sw = fmac(c.cv[2][0][k], qXZ[M + k] - qXZ[M - 1 - k], sw);
+            for (int k = 0; k < M; ++k) sv = fmac(c.cv[1][0][k], qXY[M + k] - qXY[M - 1 - k], sv);
 #pragma unroll
             for (int k = 0; k < M; ++k) sw = fmac(c.cv[2][1][k], LYZ[me + k * P::W] - LYZ[me - (1 + k) * P::W], sw);
 #pragma unroll
             for (int k = 0; k < M; ++k) sw = fmac(c.cv[2][2][k], LZZ[me + 1 + k] - LZZ[me - k], sw);
+#pragma unroll
+            for (int k = 0; k < M; ++k) sw = fmac(c.cv[2][0][k], qXZ[M + k] - qXZ[M - 1 - k], sw);
             if (HET) {
                 C::stnt(F.f[0][t1] + q, fmac(__builtin_nontemporal_load(md.m[OPESCI_M_BETA1] + q), su, u0));
                 C::stnt(F.f[1][t1] + q, fmac(__builtin_nontemporal_load(md.m[OPESCI_M_BETA2] + q), sv, v0));
@@ -562,7 +601,9 @@ __device__ __forceinline__ mpair_t media2(const float* p) {
     return __builtin_nontemporal_load(reinterpret_cast<const mpair_t*>(p));
 }
 
-// one or two of a lane's points, as one pair or a single value
+// one or two of a lane's points, as one pair or a single value (non-temporal:
+// ordinary stores, which leave the fields in the memory-side cache for the
+// next launch, measured slower for both sweeps at 256^3)
 template <typename S>
 __device__ __forceinline__ void store_points(S* p, float a, float b, bool act0, bool act1) {
     if (act1)
@@ -581,20 +622,20 @@ __device__ __forceinline__ float lane_read(float v, int lane) {
 
 template <typename S, int M, int TY, bool HET, bool ZF>
 __global__ __launch_bounds__(64 * TY) void stress_sweep2(Geom g, FieldPtrs<S> F, MediaPtrs<float> md,
-                                                         Coefs<float> c, int t0, int t1, int tz, int ty, int xchunk,
-                                                         int nblocks, ZFaceArgs zf) {
+                                                         Coefs<float> c, int t0, int t1, int tz, int nblocks,
+                                                         Boxes B, ZFaceArgs zf) {
     using R = float;
     using P = Plane2<M, TY>;
     __shared__ R sU[2][P::H * P::W], sV[2][P::H * P::W], sW[2][P::H * P::W];
     const int l = logical_block(nblocks);
     if (l < 0) return;
-    const int bz = l % tz, by = (l / tz) % ty, bx = l / (tz * ty);
+    const TileOrigin o = tile_of<TY>(B, l, tz, M);
     const int lz = threadIdx.x & 63, ly = threadIdx.x >> 6;
-    const int z0 = M + bz * 128, y0 = M + by * TY;
+    const int z0 = o.z0, y0 = o.y0;
     const int z = z0 + 2 * lz, y = y0 + ly;
-    const bool act0 = z < g.nz - M && y < g.ny - M, act1 = z + 1 < g.nz - M && y < g.ny - M;
+    const bool act0 = z < g.nz - M && y < o.yend, act1 = z + 1 < g.nz - M && y < o.yend;
     const int64_t col = (int64_t)min(y, g.ny - 1) * g.sy + pair_start(g, z);
-    const int xb = M + bx * xchunk, xe = min(xb + xchunk, g.nx - M);
+    const int xb = o.xb, xe = o.xe;
     const S* U0 = F.f[0][t0];
     const S* V0 = F.f[1][t0];
     const S* W0 = F.f[2][t0];
@@ -605,7 +646,7 @@ __global__ __launch_bounds__(64 * TY) void stress_sweep2(Geom g, FieldPtrs<S> F,
     // out of the kernel arguments, whose scalar registers it would exhaust)
     __shared__ ZCoef zl;
     const int zrh = g.nz - M - 1 - z0;                 // high face relative to the tile
-    const bool zlo = bz == 0, zhi = zrh >= 0 && zrh < 128;
+    const bool zlo = z0 == M, zhi = zrh >= 0 && zrh < 128;
     const int zlane = zlo && (threadIdx.x & 63) == 0 ? 0 : (zhi && (int)(threadIdx.x & 63) == (zrh >> 1) ? 1 : -1);
     if constexpr (ZF) {
         for (int w = threadIdx.x; w < ZCOEF_WORDS; w += blockDim.x)
@@ -826,28 +867,29 @@ __global__ __launch_bounds__(64 * TY) void stress_sweep2(Geom g, FieldPtrs<S> F,
                 }
             }
 #pragma unroll
-            for (int f = 0; f < 6; ++f) store_points(F.f[3 + f][t1] + q, out[f][0], out[f][1], act0, act1);
+            for (int f = 0; f < 6; ++f)
+                store_points(F.f[3 + f][t1] + q, out[f][0], out[f][1], act0, act1);
         }
     }
 }
 
 template <typename S, int M, int TY, bool HET>
 __global__ __launch_bounds__(64 * TY) void velocity_sweep2(Geom g, FieldPtrs<S> F, MediaPtrs<float> md,
-                                                           Coefs<float> c, int t0, int t1, int tz, int ty, int xchunk,
-                                                           int nblocks) {
+                                                           Coefs<float> c, int t0, int t1, int tz, int nblocks,
+                                                           Boxes B) {
     using R = float;
     using P = Plane2<M, TY>;
     __shared__ R sYY[2][P::H * P::W], sZZ[2][P::H * P::W], sXY[2][P::H * P::W], sYZ[2][P::H * P::W],
         sXZ[2][P::H * P::W];
     const int l = logical_block(nblocks);
     if (l < 0) return;
-    const int bz = l % tz, by = (l / tz) % ty, bx = l / (tz * ty);
+    const TileOrigin o = tile_of<TY>(B, l, tz, M);
     const int lz = threadIdx.x & 63, ly = threadIdx.x >> 6;
-    const int z0 = M + bz * 128, y0 = M + by * TY;
+    const int z0 = o.z0, y0 = o.y0;
     const int z = z0 + 2 * lz, y = y0 + ly;
-    const bool act0 = z < g.nz - M && y < g.ny - M, act1 = z + 1 < g.nz - M && y < g.ny - M;
+    const bool act0 = z < g.nz - M && y < o.yend, act1 = z + 1 < g.nz - M && y < o.yend;
     const int64_t col = (int64_t)min(y, g.ny - 1) * g.sy + pair_start(g, z);
-    const int xb = M + bx * xchunk, xe = min(xb + xchunk, g.nx - M);
+    const int xb = o.xb, xe = o.xe;
     const S* TXX = F.f[3][t1];
     const S* TYY = F.f[4][t1];
     const S* TZZ = F.f[5][t1];
@@ -936,23 +978,23 @@ __global__ __launch_bounds__(64 * TY) void velocity_sweep2(Geom g, FieldPtrs<S> 
                 const int me = me0 + v;
                 R su = 0, sv = 0, sw = 0;
 #pragma unroll
-                for (int k = 0; k < M; ++k) su = fmac(c.cv[0][0][k], qXX[v][M + k] - qXX[v][M - 1 - k], su);
-#pragma unroll
                 for (int k = 0; k < M; ++k) su = fmac(c.cv[0][1][k], LXY[me + k * P::W] - LXY[me - (1 + k) * P::W], su);
 #pragma unroll
                 for (int k = 0; k < M; ++k) su = fmac(c.cv[0][2][k], LXZ[me + k] - LXZ[me - 1 - k], su);
 #pragma unroll
-                for (int k = 0; k < M; ++k) sv = fmac(c.cv[1][0][k], qXY[v][M + k] - qXY[v][M - 1 - k], sv);
+                for (int k = 0; k < M; ++k) su = fmac(c.cv[0][0][k], qXX[v][M + k] - qXX[v][M - 1 - k], su);
 #pragma unroll
                 for (int k = 0; k < M; ++k) sv = fmac(c.cv[1][1][k], LYY[me + (1 + k) * P::W] - LYY[me - k * P::W], sv);
 #pragma unroll
                 for (int k = 0; k < M; ++k) sv = fmac(c.cv[1][2][k], LYZ[me + k] - LYZ[me - 1 - k], sv);
 #pragma unroll
-                for (int k = 0; k < M; ++k) sw = fmac(c.cv[2][0][k], qXZ[v][M + k] - qXZ[v][M - 1 - k], sw);
+                for (int k = 0; k < M; ++k) sv = fmac(c.cv[1][0][k], qXY[v][M + k] - qXY[v][M - 1 - k], sv);
 #pragma unroll
                 for (int k = 0; k < M; ++k) sw = fmac(c.cv[2][1][k], LYZ[me + k * P::W] - LYZ[me - (1 + k) * P::W], sw);
 #pragma unroll
                 for (int k = 0; k < M; ++k) sw = fmac(c.cv[2][2][k], LZZ[me + 1 + k] - LZZ[me - k], sw);
+#pragma unroll
+                for (int k = 0; k < M; ++k) sw = fmac(c.cv[2][0][k], qXZ[v][M + k] - qXZ[v][M - 1 - k], sw);
                 if (HET) {
                     ou[v] = fmac(b1[v], su, u0[v]);
                     ov[v] = fmac(b2[v], sv, v0[v]);
@@ -966,6 +1008,574 @@ __global__ __launch_bounds__(64 * TY) void velocity_sweep2(Geom g, FieldPtrs<S> 
             store_points(F.f[0][t1] + q, ou[0], ou[1], act0, act1);
             store_points(F.f[1][t1] + q, ov[0], ov[1], act0, act1);
             store_points(F.f[2][t1] + q, ow[0], ow[1], act0, act1);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ fused step
+// Both half steps of one leapfrog step in one x sweep, for FP32 arithmetic
+// on constant media with the deep z-face statements of the stress phase in
+// the sweep.  It covers a box of rows far enough from the x and y faces
+// that no stress boundary statement writes a cell its velocity update reads
+// (the host checks this against the boundary program); the frame around the
+// box keeps the separate stress and velocity launches, and the velocity
+// boundary statements run after both as before.
+//
+// A block owns a 128 (z) x TY (y) tile of an x chunk [xb, xe).  Its waves
+// hold rows y0 - M .. y0 + TY + M - 1: TY own rows between M halo rows on
+// either side.  At step xs of the sweep (xs = xb - M .. xe + M - 1) every
+// wave computes the stresses of its row at plane xs exactly as the pair
+// stress sweep does (velocity plane xs staged in LDS with 2M halo rows and
+// 2M z columns either side, x stencils in register queues), lanes 0 and 63
+// of the own waves add the z-halo pairs below and above the tile (Txz, Tyz,
+// Tzz) where a neighbouring tile, not a face, lies there, and the z-face
+// statements replace the surface values and produce the ghost cells.  The
+// stresses go to an LDS plane (rounded to the storage type, as the separate
+// velocity launch would read them back) and, for own rows and xs in
+// [xb, xe), to memory.  The own waves then form the y and z part of the
+// velocity sums of plane xs from that plane (the first part of the y, z, x
+// fma chain) and keep it M planes, until the stresses up to xs + M are
+// known: at step xs they complete the velocities of plane xs - M with the x
+// part from register queues of Txx, Txy, Txz.  So every value is the
+// separate launches' bit for bit, and a step reads U, V, W and the six
+// stresses once and writes them once (18 streams against 15 + 12).
+template <int M, int TY>
+struct FusedTile {
+    static constexpr int NW = TY + 2 * M;   // waves
+    static constexpr int NT = 64 * NW;
+    static constexpr int VW = 128 + 4 * M;  // velocity plane: rows y0-2M.., z from z0-2M
+    static constexpr int VH = TY + 4 * M;
+    static constexpr int SW = 128 + 2 * M;  // stress plane: rows y0-M.., z from z0-M
+    static constexpr int SH = TY + 2 * M;
+    static constexpr int NO = 3 * 2 * M * 64;  // velocity pairs of the 2M outer rows (3 fields)
+    static constexpr int JO = (NO + NT - 1) / NT;
+    static constexpr int NZF = SH * 2 * M;             // velocity pairs of the z columns per field (M either side)
+    static constexpr int NZP = (NZF + 63) / 64 * 64;   // padded: one field per wave
+    static constexpr int NZ = 3 * NZP;
+    static constexpr int JZ = (NZ + NT - 1) / NT;
+};
+
+// staging slots of the velocity plane beyond the waves' own pairs
+template <int M, int TY>
+struct FusedSlots {
+    using T = FusedTile<M, TY>;
+    int lo[T::JO], go[T::JO], fo[T::JO];  // outer rows: LDS slot, offset, field (-1: none)
+    int lz[T::JZ], gz[T::JZ], fz[T::JZ];  // z columns
+    __device__ void init(const Geom& g, int y0, int z0) {
+        const int t = threadIdx.x;
+#pragma unroll
+        for (int j = 0; j < T::JO; ++j) {
+            const int h = t + j * T::NT;
+            fo[j] = -1;
+            lo[j] = 0;
+            go[j] = 0;
+            if (h < T::NO) {
+                const int f = h / (2 * M * 64), r = (h / 64) % (2 * M), p = h % 64;
+                const int row = r < M ? r : r + TY + 2 * M;
+                const int yy = min(max(y0 - 2 * M + row, 0), g.ny - 1);
+                fo[j] = f;
+                lo[j] = row * T::VW + 2 * M + 2 * p;
+                go[j] = yy * (int)g.sy + pair_start(g, z0 + 2 * p);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < T::JZ; ++j) {
+            const int h = t + j * T::NT;
+            fz[j] = -1;
+            lz[j] = 0;
+            gz[j] = 0;
+            if (h < T::NZ && h % T::NZP < T::NZF) {
+                const int f = h / T::NZP, rem = h % T::NZP;
+                const int row = rem / (2 * M) + M, c = rem % (2 * M);
+                const int zz = c < M ? z0 - 2 * M + 2 * c : z0 + 128 + 2 * (c - M);
+                const int yy = min(max(y0 - 2 * M + row, 0), g.ny - 1);
+                fz[j] = f;
+                lz[j] = row * T::VW + (zz - (z0 - 2 * M));
+                gz[j] = yy * (int)g.sy + pair_start(g, zz);
+            }
+        }
+    }
+};
+
+// field pointers of one fused step, levels resolved on the host
+template <typename S>
+struct StepPtrs {
+    const S* v0[3];   // U, V, W at t0
+    const S* s0[6];   // Txx, Tyy, Tzz, Txy, Tyz, Txz at t0
+    S* s1[6];         // the stresses at t1
+    S* v1[3];         // U, V, W at t1
+};
+
+template <typename S, int M, int TY>
+__global__ __launch_bounds__(64 * (TY + 2 * M)) void step_sweep2(Geom g, StepPtrs<S> P, Coefs<float> c, int tz,
+                                                                 int nblocks, Boxes B, ZFaceArgs zf) {
+    using R = float;
+    using T = FusedTile<M, TY>;
+    using CS = Conv<S, float>;
+    constexpr int L = 2 * M + 1;    // planes kept in the LDS rings
+    constexpr int NOWN = 64 * TY;   // own-row lanes
+    __shared__ R vU[T::VH * T::VW], vV[T::VH * T::VW], vW[T::VH * T::VW];
+    __shared__ R sXY[T::SH * T::SW], sYY[T::SH * T::SW], sYZ[T::SH * T::SW], sXZ[T::SH * T::SW],
+        sZZ[T::SH * T::SW];
+    // own lanes: Txx, Txy, Txz of the last planes and the y, z part of the
+    // velocity sums, ring slot p % L for plane p, [slot][element][lane]
+    __shared__ R rXX[L * 2 * NOWN], rXY[L * 2 * NOWN], rXZ[L * 2 * NOWN], rPU[L * 2 * NOWN],
+        rPV[L * 2 * NOWN], rPW[L * 2 * NOWN];
+    // z-halo pairs: U and W of the last planes, [row][side][slot][field][element]
+    __shared__ R rI[TY * 2 * L * 2 * 2];
+    __shared__ ZCoef zl;
+    const int l = logical_block(nblocks);
+    if (l < 0) return;
+    const TileOrigin o = tile_of<TY>(B, l, tz, M);
+    const int w = threadIdx.x >> 6, lz = threadIdx.x & 63;
+    const bool own = w >= M && w < M + TY;
+    const int tw = (w - M) * 64 + lz;   // own-lane index (own waves)
+    const int z0 = o.z0, y0 = o.y0, xb = o.xb, xe = o.xe;
+    const int y = y0 - M + w, z = z0 + 2 * lz;
+    const bool rowc = y < o.yend + M, rowo = own && y < o.yend;
+    const bool zin0 = z < g.nz - M, zin1 = z + 1 < g.nz - M;
+    const bool act0 = rowc && zin0, act1 = rowc && zin1;
+    const int64_t col = (int64_t)min(y, g.ny - 1) * g.sy + pair_start(g, z);
+    const int zrh = g.nz - M - 1 - z0;
+    const bool zlo = z0 == M, zhi = zrh >= 0 && zrh < 128;
+    const int zlane = zlo && lz == 0 ? 0 : (zhi && lz == (zrh >> 1) ? 1 : -1);
+    const bool item = own && ((lz == 0 && !zlo) || (lz == 63 && !zhi && z0 + 128 < g.nz - M));
+    const int zi = lz == 0 ? z0 - 2 : z0 + 128;
+    const int64_t coli = (int64_t)min(y, g.ny - 1) * g.sy + pair_start(g, zi);
+    R* ring_i = rI + (((own ? w - M : 0) * 2 + (lz == 63 ? 1 : 0)) * L) * 4;
+    FusedSlots<M, TY> hs;
+    hs.init(g, y0, z0);
+    for (int i = threadIdx.x; i < T::SH * T::SW; i += T::NT) {
+        sXY[i] = 0.0f;
+        sYY[i] = 0.0f;
+        sYZ[i] = 0.0f;
+        sXZ[i] = 0.0f;
+        sZZ[i] = 0.0f;
+    }
+    for (int i = threadIdx.x; i < ZCOEF_WORDS; i += T::NT)
+        reinterpret_cast<int*>(&zl)[i] = reinterpret_cast<const int*>(zf.coef)[i];
+    const int xs0 = xb - M, xs1 = xe + M;
+    // stress side: qU[v][i] = U(x-M+i), qV[v][i] = V(x-M+i), qW[v][i] = W(x-M+i)
+    R qU[2][2 * M], qV[2][2 * M + 1], qW[2][2 * M + 1];
+#pragma unroll
+    for (int i = 1; i < 2 * M; ++i) ld2(P.v0[0] + (int64_t)(xs0 - M - 1 + i) * g.sx + col, qU[0][i], qU[1][i]);
+#pragma unroll
+    for (int i = 1; i < 2 * M + 1; ++i) {
+        ld2(P.v0[1] + (int64_t)(xs0 - M - 1 + i) * g.sx + col, qV[0][i], qV[1][i]);
+        ld2(P.v0[2] + (int64_t)(xs0 - M - 1 + i) * g.sx + col, qW[0][i], qW[1][i]);
+    }
+    // z-halo pairs: U(x-M .. x+M-2) and W(x-M+1 .. x+M-1) into ring slots,
+    // the newest plane arriving with each step
+    if (item) {
+        for (int p = xs0 - M; p < xs0 + M; ++p) {
+            const int sl = ((p % L) + L) % L;
+            R a0, a1;
+            if (p < xs0 + M - 1) {
+                ld2(P.v0[0] + (int64_t)p * g.sx + coli, a0, a1);
+                ring_i[(sl * 2 + 0) * 2 + 0] = a0;
+                ring_i[(sl * 2 + 0) * 2 + 1] = a1;
+            }
+            if (p > xs0 - M) {
+                ld2(P.v0[2] + (int64_t)p * g.sx + coli, a0, a1);
+                ring_i[(sl * 2 + 1) * 2 + 0] = a0;
+                ring_i[(sl * 2 + 1) * 2 + 1] = a1;
+            }
+        }
+    }
+    R nU[2], nV[2], nW[2], nT[6][2], nUi[2] = {0.0f, 0.0f}, nWi[2] = {0.0f, 0.0f}, nTi[3][2] = {}, so[T::JO][2],
+        sz[T::JZ][2];
+    auto fetch = [&](int x) {
+        const int64_t px = (int64_t)x * g.sx;
+        ld2(P.v0[0] + px + (int64_t)(M - 1) * g.sx + col, nU[0], nU[1]);
+        ld2(P.v0[1] + px + (int64_t)M * g.sx + col, nV[0], nV[1]);
+        ld2(P.v0[2] + px + (int64_t)M * g.sx + col, nW[0], nW[1]);
+#pragma unroll
+        for (int f = 0; f < 6; ++f) ldnt2(P.s0[f] + px + col, nT[f][0], nT[f][1]);
+        if (item) {
+            ld2(P.v0[0] + px + (int64_t)(M - 1) * g.sx + coli, nUi[0], nUi[1]);
+            ld2(P.v0[2] + px + (int64_t)M * g.sx + coli, nWi[0], nWi[1]);
+            ld2(P.s0[2] + px + coli, nTi[0][0], nTi[0][1]);
+            ld2(P.s0[4] + px + coli, nTi[1][0], nTi[1][1]);
+            ld2(P.s0[5] + px + coli, nTi[2][0], nTi[2][1]);
+        }
+#pragma unroll
+        for (int j = 0; j < T::JO; ++j)
+            if (hs.fo[j] >= 0) ld2(P.v0[hs.fo[j]] + px + hs.go[j], so[j][0], so[j][1]);
+#pragma unroll
+        for (int j = 0; j < T::JZ; ++j)
+            if (hs.fz[j] >= 0) ld2(P.v0[hs.fz[j]] + px + hs.gz[j], sz[j][0], sz[j][1]);
+    };
+    fetch(xs0);
+    const int vme0 = (w + M) * T::VW + 2 * M + 2 * lz;
+    const int sme0 = w * T::SW + M + 2 * lz;
+    const int vmi = (w + M) * T::VW + (zi - (z0 - 2 * M));
+    const int smi = w * T::SW + (zi - (z0 - M));
+    auto rnd = [](float v) { return CS::from(CS::to(v)); };
+    int slot = ((xs0 % L) + L) % L;   // ring slot of plane x
+    auto back = [&](int d) { return slot >= d ? slot - d : slot - d + L; };   // slot of plane x - d
+    for (int x = xs0; x < xs1; ++x, slot = slot + 1 == L ? 0 : slot + 1) {
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+#pragma unroll
+            for (int i = 0; i < 2 * M - 1; ++i) qU[v][i] = qU[v][i + 1];
+#pragma unroll
+            for (int i = 0; i < 2 * M; ++i) {
+                qV[v][i] = qV[v][i + 1];
+                qW[v][i] = qW[v][i + 1];
+            }
+            qU[v][2 * M - 1] = nU[v];
+            qV[v][2 * M] = nV[v];
+            qW[v][2 * M] = nW[v];
+        }
+        R T0[6][2], T0i[3][2];
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+            T0[f][0] = nT[f][0];
+            T0[f][1] = nT[f][1];
+        }
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            T0i[f][0] = nTi[f][0];
+            T0i[f][1] = nTi[f][1];
+        }
+        if (item) {
+            // U(x+M-1) and W(x+M) of the z-halo pair
+            const int su = back(L - (M - 1)), sw = back(L - M);
+            ring_i[(su * 2 + 0) * 2 + 0] = nUi[0];
+            ring_i[(su * 2 + 0) * 2 + 1] = nUi[1];
+            ring_i[(sw * 2 + 1) * 2 + 0] = nWi[0];
+            ring_i[(sw * 2 + 1) * 2 + 1] = nWi[1];
+        }
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+            vU[vme0 + v] = qU[v][M];
+            vV[vme0 + v] = qV[v][M];
+            vW[vme0 + v] = qW[v][M];
+        }
+#pragma unroll
+        for (int j = 0; j < T::JO; ++j)
+            if (hs.fo[j] >= 0) {
+                R* Lp = hs.fo[j] == 0 ? vU : (hs.fo[j] == 1 ? vV : vW);
+                Lp[hs.lo[j]] = so[j][0];
+                Lp[hs.lo[j] + 1] = so[j][1];
+            }
+#pragma unroll
+        for (int j = 0; j < T::JZ; ++j)
+            if (hs.fz[j] >= 0) {
+                R* Lp = hs.fz[j] == 0 ? vU : (hs.fz[j] == 1 ? vV : vW);
+                Lp[hs.lz[j]] = sz[j][0];
+                Lp[hs.lz[j] + 1] = sz[j][1];
+            }
+        if (x + 1 < xs1) fetch(x + 1);
+        __syncthreads();
+        const bool xo = x >= xb && x < xe;
+        const int64_t q = (int64_t)x * g.sx + col;
+        R out[6][2] = {};
+        if (act0) {
+#pragma unroll
+            for (int v = 0; v < 2; ++v) {
+                const int me = vme0 + v;
+                R dx[M], dy[M], dz[M];
+#pragma unroll
+                for (int k = 0; k < M; ++k) {
+                    dx[k] = qU[v][M + k] - qU[v][M - 1 - k];
+                    dy[k] = vV[me + k * T::VW] - vV[me - (1 + k) * T::VW];
+                    dz[k] = vW[me + k] - vW[me - 1 - k];
+                }
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    R acc = T0[i][v];
+                    R s = 0;
+#pragma unroll
+                    for (int k = 0; k < M; ++k) s = fmac(c.cn[i][0][k], dx[k], s);
+                    acc = acc + s;
+                    s = 0;
+#pragma unroll
+                    for (int k = 0; k < M; ++k) s = fmac(c.cn[i][1][k], dy[k], s);
+                    acc = acc + s;
+                    s = 0;
+#pragma unroll
+                    for (int k = 0; k < M; ++k) s = fmac(c.cn[i][2][k], dz[k], s);
+                    acc = acc + s;
+                    out[i][v] = acc;
+                }
+                R sxy = 0, syz = 0, sxz = 0;
+#pragma unroll
+                for (int k = 0; k < M; ++k)
+                    sxy = fmac(c.cs[0][0][k], vU[me + (1 + k) * T::VW] - vU[me - k * T::VW], sxy);
+#pragma unroll
+                for (int k = 0; k < M; ++k) sxy = fmac(c.cs[0][1][k], qV[v][M + k + 1] - qV[v][M - k], sxy);
+#pragma unroll
+                for (int k = 0; k < M; ++k) syz = fmac(c.cs[1][0][k], vV[me + 1 + k] - vV[me - k], syz);
+#pragma unroll
+                for (int k = 0; k < M; ++k)
+                    syz = fmac(c.cs[1][1][k], vW[me + (1 + k) * T::VW] - vW[me - k * T::VW], syz);
+#pragma unroll
+                for (int k = 0; k < M; ++k) sxz = fmac(c.cs[2][0][k], vU[me + 1 + k] - vU[me - k], sxz);
+#pragma unroll
+                for (int k = 0; k < M; ++k) sxz = fmac(c.cs[2][1][k], qW[v][M + k + 1] - qW[v][M - k], sxz);
+                out[3][v] = T0[3][v] + sxy;
+                out[4][v] = T0[4][v] + syz;
+                out[5][v] = T0[5][v] + sxz;
+            }
+        }
+        // z-face statements (box rows are deep rows: x and y in [F, dim - F))
+        float ghost[3][M];   // [Tzz, Tyz, Txz][k] below the low face or above the high face
+        const bool zrow = act0 && x >= zf.F && x < g.nx - zf.F && y >= zf.F && y < g.ny - zf.F;
+        // image sources [face][Tzz, Tyz, Txz][k], read from the lanes holding
+        // them while every interior lane (each source among them) is active
+        float src[2][3][M];
+        if (zrow) {
+            const int rh = zrh;
+#pragma unroll
+            for (int k = 0; k < M; ++k) {
+                if (zlo) {
+                    src[0][0][k] = k >= 1 ? lane_read<S>(out[2][k & 1], k >> 1) : 0.0f;
+                    src[0][1][k] = lane_read<S>(out[4][k & 1], k >> 1);
+                    src[0][2][k] = lane_read<S>(out[5][k & 1], k >> 1);
+                }
+                if (zhi) {
+                    const int r0 = rh - k, r1 = rh - 1 - k;
+                    src[1][0][k] = k >= 1 ? lane_read<S>((r0 & 1) ? out[2][1] : out[2][0], r0 >> 1) : 0.0f;
+                    src[1][1][k] = lane_read<S>((r1 & 1) ? out[4][1] : out[4][0], r1 >> 1);
+                    src[1][2][k] = lane_read<S>((r1 & 1) ? out[5][1] : out[5][0], r1 >> 1);
+                }
+            }
+        }
+        if (zrow && zlane >= 0) {
+            const int face = zlane;
+            const int eh = face ? (zrh & 1) : 0;
+            float rxx = 0.0f, ryy = 0.0f;
+            if constexpr (M == 2) {
+                if (zf.rec) {
+                    // Levander recompute of Txx, Tyy on the surface point
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) {
+                        const float v = k < 4 ? (eh ? qU[1][k] : qU[0][k])
+                                              : (k < 8 ? vV[vme0 + eh + (k - 6) * T::VW] : 0.0f);
+                        if (k < 8) {
+                            rxx = fmac(zl.rc[face][0][k], v, rxx);
+                            ryy = fmac(zl.rc[face][1][k], v, ryy);
+                        } else {
+                            rxx = fmac(zl.rc[face][0][k], eh ? T0[0][1] : T0[0][0], rxx);
+                            ryy = fmac(zl.rc[face][1][k], eh ? T0[1][1] : T0[1][0], ryy);
+                        }
+                    }
+                }
+            }
+            if (face == 0) {
+#pragma unroll
+                for (int k = 0; k < M; ++k) {
+                    ghost[0][k] = k >= 1 ? fmac(zl.zc[0][0][k], src[0][0][k], 0.0f) : 0.0f;
+                    ghost[1][k] = fmac(zl.zc[0][1][k], src[0][1][k], 0.0f);
+                    ghost[2][k] = fmac(zl.zc[0][2][k], src[0][2][k], 0.0f);
+                }
+                if (zf.rec) {
+                    out[0][0] = rxx;
+                    out[1][0] = ryy;
+                }
+                out[2][0] = 0.0f;
+            } else {
+                const float tyz = fmac(zl.zc[1][1][0], src[1][1][0], 0.0f);
+                const float txz = fmac(zl.zc[1][2][0], src[1][2][0], 0.0f);
+#pragma unroll
+                for (int k = 0; k < M; ++k) {
+                    ghost[0][k] = k >= 1 ? fmac(zl.zc[1][0][k], src[1][0][k], 0.0f) : 0.0f;
+                    ghost[1][k] = k >= 1 ? fmac(zl.zc[1][1][k], src[1][1][k], 0.0f) : 0.0f;
+                    ghost[2][k] = k >= 1 ? fmac(zl.zc[1][2][k], src[1][2][k], 0.0f) : 0.0f;
+                }
+#pragma unroll
+                for (int v = 0; v < 2; ++v)
+                    if (v == eh) {
+                        if (zf.rec) {
+                            out[0][v] = rxx;
+                            out[1][v] = ryy;
+                        }
+                        out[2][v] = 0.0f;
+                        out[4][v] = tyz;
+                        out[5][v] = txz;
+                    }
+            }
+        }
+        // z-halo pair of the neighbouring tile: Tzz, Tyz, Txz as its own lanes compute them
+        R outi[3][2] = {};
+        if (item && rowc) {
+            // U(x-M+i) for i < 2M, W(x-M+1+i) for i < 2M from the ring
+            R ui[2 * M][2], wi[2 * M][2];
+#pragma unroll
+            for (int i = 0; i < 2 * M; ++i) {
+                const int su = back((M - i + L) % L), sw = back((M - 1 - i + L) % L);
+                ui[i][0] = ring_i[(su * 2 + 0) * 2 + 0];
+                ui[i][1] = ring_i[(su * 2 + 0) * 2 + 1];
+                wi[i][0] = ring_i[(sw * 2 + 1) * 2 + 0];
+                wi[i][1] = ring_i[(sw * 2 + 1) * 2 + 1];
+            }
+#pragma unroll
+            for (int v = 0; v < 2; ++v) {
+                const int me = vmi + v;
+                R s = 0, acc = T0i[0][v];
+#pragma unroll
+                for (int k = 0; k < M; ++k) s = fmac(c.cn[2][0][k], ui[M + k][v] - ui[M - 1 - k][v], s);
+                acc = acc + s;
+                s = 0;
+#pragma unroll
+                for (int k = 0; k < M; ++k) s = fmac(c.cn[2][1][k], vV[me + k * T::VW] - vV[me - (1 + k) * T::VW], s);
+                acc = acc + s;
+                s = 0;
+#pragma unroll
+                for (int k = 0; k < M; ++k) s = fmac(c.cn[2][2][k], vW[me + k] - vW[me - 1 - k], s);
+                outi[0][v] = acc + s;
+                R syz = 0, sxz = 0;
+#pragma unroll
+                for (int k = 0; k < M; ++k) syz = fmac(c.cs[1][0][k], vV[me + 1 + k] - vV[me - k], syz);
+#pragma unroll
+                for (int k = 0; k < M; ++k)
+                    syz = fmac(c.cs[1][1][k], vW[me + (1 + k) * T::VW] - vW[me - k * T::VW], syz);
+#pragma unroll
+                for (int k = 0; k < M; ++k) sxz = fmac(c.cs[2][0][k], vU[me + 1 + k] - vU[me - k], sxz);
+#pragma unroll
+                for (int k = 0; k < M; ++k) sxz = fmac(c.cs[2][1][k], wi[M + k][v] - wi[M - 1 - k][v], sxz);
+                outi[1][v] = T0i[1][v] + syz;
+                outi[2][v] = T0i[2][v] + sxz;
+            }
+        }
+        // stress plane x, rounded as the separate velocity launch reads it back
+        if (act0) {
+#pragma unroll
+            for (int v = 0; v < 2; ++v)
+                if (v == 0 || act1) {
+                    const int sm = sme0 + v;
+                    sXY[sm] = rnd(out[3][v]);
+                    sYY[sm] = rnd(out[1][v]);
+                    sYZ[sm] = rnd(out[4][v]);
+                    if (own) {
+                        sXZ[sm] = rnd(out[5][v]);
+                        sZZ[sm] = rnd(out[2][v]);
+                    }
+                }
+        }
+        if (item && rowc) {
+#pragma unroll
+            for (int v = 0; v < 2; ++v) {
+                const int cz = zi + v - (z0 - M);
+                if (cz >= 0 && cz < T::SW) {
+                    sZZ[smi + v] = rnd(outi[0][v]);
+                    sYZ[smi + v] = rnd(outi[1][v]);
+                    sXZ[smi + v] = rnd(outi[2][v]);
+                }
+            }
+        }
+        if (zrow && zlane >= 0) {
+            // ghost cells: below the low face (z0 = m) columns M - k (Tzz) and
+            // M - 1 - k; above the high face columns ch + k
+            const int ch = zrh + M;
+#pragma unroll
+            for (int k = 0; k < M; ++k) {
+                if (zlane == 0) {
+                    if (k >= 1) sZZ[w * T::SW + M - k] = rnd(ghost[0][k]);
+                    sYZ[w * T::SW + M - 1 - k] = rnd(ghost[1][k]);
+                    sXZ[w * T::SW + M - 1 - k] = rnd(ghost[2][k]);
+                } else if (k >= 1) {
+                    sZZ[w * T::SW + ch + k] = rnd(ghost[0][k]);
+                    sYZ[w * T::SW + ch + k] = rnd(ghost[1][k]);
+                    sXZ[w * T::SW + ch + k] = rnd(ghost[2][k]);
+                }
+            }
+        }
+        if (rowo && xo) {
+            const int64_t row = (int64_t)x * g.sx + (int64_t)y * g.sy;
+            if (zrow && zlane >= 0) {
+                const int b = zlane == 0 ? M : g.nz - M - 1;
+#pragma unroll
+                for (int k = 0; k < M; ++k) {
+                    if (zlane == 0) {
+                        if (k >= 1) CS::st(P.s1[2] + row + b - k, ghost[0][k]);
+                        CS::st(P.s1[4] + row + b - 1 - k, ghost[1][k]);
+                        CS::st(P.s1[5] + row + b - 1 - k, ghost[2][k]);
+                    } else if (k >= 1) {
+                        CS::st(P.s1[2] + row + b + k, ghost[0][k]);
+                        CS::st(P.s1[4] + row + b + k, ghost[1][k]);
+                        CS::st(P.s1[5] + row + b + k, ghost[2][k]);
+                    }
+                }
+            }
+            if (act0) {
+#pragma unroll
+                for (int f = 0; f < 6; ++f) store_points(P.s1[f] + q, out[f][0], out[f][1], act0, act1);
+            }
+        }
+        // own rows: Txx, Txy, Txz of plane x into the rings
+        if (own && rowc) {
+#pragma unroll
+            for (int v = 0; v < 2; ++v) {
+                const int r = (slot * 2 + v) * NOWN + tw;
+                rXX[r] = rnd(out[0][v]);
+                rXY[r] = rnd(out[3][v]);
+                rXZ[r] = rnd(out[5][v]);
+            }
+        }
+        __syncthreads();
+        // velocity: y and z part of plane x, completed for plane x - M
+        if (own && rowc) {
+#pragma unroll
+            for (int v = 0; v < 2; ++v) {
+                const int sm = sme0 + v;
+                R su = 0, sv = 0, sw = 0;
+#pragma unroll
+                for (int k = 0; k < M; ++k)
+                    su = fmac(c.cv[0][1][k], sXY[sm + k * T::SW] - sXY[sm - (1 + k) * T::SW], su);
+#pragma unroll
+                for (int k = 0; k < M; ++k) su = fmac(c.cv[0][2][k], sXZ[sm + k] - sXZ[sm - 1 - k], su);
+#pragma unroll
+                for (int k = 0; k < M; ++k)
+                    sv = fmac(c.cv[1][1][k], sYY[sm + (1 + k) * T::SW] - sYY[sm - k * T::SW], sv);
+#pragma unroll
+                for (int k = 0; k < M; ++k) sv = fmac(c.cv[1][2][k], sYZ[sm + k] - sYZ[sm - 1 - k], sv);
+#pragma unroll
+                for (int k = 0; k < M; ++k)
+                    sw = fmac(c.cv[2][1][k], sYZ[sm + k * T::SW] - sYZ[sm - (1 + k) * T::SW], sw);
+#pragma unroll
+                for (int k = 0; k < M; ++k) sw = fmac(c.cv[2][2][k], sZZ[sm + 1 + k] - sZZ[sm - k], sw);
+                const int r = (slot * 2 + v) * NOWN + tw;
+                rPU[r] = su;
+                rPV[r] = sv;
+                rPW[r] = sw;
+            }
+            const int xv = x - M;
+            if (rowo && xv >= xb && xv < xe) {
+                R ou[2], ov[2], ow[2];
+#pragma unroll
+                for (int v = 0; v < 2; ++v) {
+                    const int r0 = (back(M) * 2 + v) * NOWN + tw;
+                    R su = rPU[r0], sv = rPV[r0], sw = rPW[r0];
+                    // Txx(xv-M+1 .. xv+M) = planes x-2M+1 .. x; Txy, Txz(xv-M .. xv+M-1)
+#pragma unroll
+                    for (int k = 0; k < M; ++k) {
+                        const R a = rXX[(back(M - 1 - k) * 2 + v) * NOWN + tw];
+                        const R b = rXX[(back(M + k) * 2 + v) * NOWN + tw];
+                        su = fmac(c.cv[0][0][k], a - b, su);
+                    }
+#pragma unroll
+                    for (int k = 0; k < M; ++k) {
+                        const R a = rXY[(back(M - k) * 2 + v) * NOWN + tw];
+                        const R b = rXY[(back(M + 1 + k) * 2 + v) * NOWN + tw];
+                        sv = fmac(c.cv[1][0][k], a - b, sv);
+                    }
+#pragma unroll
+                    for (int k = 0; k < M; ++k) {
+                        const R a = rXZ[(back(M - k) * 2 + v) * NOWN + tw];
+                        const R b = rXZ[(back(M + 1 + k) * 2 + v) * NOWN + tw];
+                        sw = fmac(c.cv[2][0][k], a - b, sw);
+                    }
+                    ou[v] = qU[v][0] + su;
+                    ov[v] = qV[v][0] + sv;
+                    ow[v] = qW[v][0] + sw;
+                }
+                const int64_t qv = (int64_t)xv * g.sx + col;
+                store_points(P.v1[0] + qv, ou[0], ou[1], zin0, zin1);
+                store_points(P.v1[1] + qv, ov[0], ov[1], zin0, zin1);
+                store_points(P.v1[2] + qv, ow[0], ow[1], zin0, zin1);
+            }
         }
     }
 }
@@ -998,6 +1608,34 @@ static Tiling tiling(const Geom& g, const InteriorLaunch& cfg, int ty, int tzw =
     return t;
 }
 
+// Tile counts of a box list for tiles of ty rows and 128 z points; x chunk
+// xc0 (0: about 512 blocks over all boxes, 8..64 planes, as tiling()).
+template <int M>
+static Boxes fill_boxes(const Geom& g, Boxes B, int ty, int xc0, int& tz) {
+    tz = (g.nz - 2 * M + 127) / 128;
+    int cols = 0, len = 0;
+    for (int i = 0; i < B.n; ++i) {
+        B.tyn[i] = std::max(0, (B.y1[i] - B.y0[i] + ty - 1) / ty);
+        cols += tz * B.tyn[i];
+        len = std::max(len, B.x1[i] - B.x0[i]);
+    }
+    int xc = xc0;
+    if (xc <= 0) {
+        long long work = 0;
+        for (int i = 0; i < B.n; ++i) work += (long long)tz * B.tyn[i] * std::max(0, B.x1[i] - B.x0[i]);
+        xc = (int)((work + 511) / 512);
+        xc = xc < 8 ? 8 : (xc > 64 ? 64 : xc);
+    }
+    B.first[0] = 0;
+    for (int i = 0; i < B.n; ++i) {
+        const int xl = std::max(0, B.x1[i] - B.x0[i]);
+        B.xc[i] = std::max(1, std::min(xc, xl));
+        const int tx = xl > 0 ? (xl + B.xc[i] - 1) / B.xc[i] : 0;
+        B.first[i + 1] = B.first[i] + tz * B.tyn[i] * tx;
+    }
+    return B;
+}
+
 // Rows per tile of the single-point sweeps when the caller leaves it open: 16
 // for FP32 on constant media (fewer y-halo rows; measured 2-3 % faster at
 // 256^3), else 8 (the media streams and the 16-bit conversions favour the
@@ -1009,10 +1647,13 @@ static int default_tile_y() {
 
 template <typename R, typename S, int M, bool HET, bool VEL>
 static hipError_t run_interior(const Geom& g, void* fields, const void* media, const void* coefs, int t0, int t1,
-                               const InteriorLaunch& cfg, hipStream_t s, const ZFaceArgs* zf, const void* pool) {
+                               const InteriorLaunch& cfg, hipStream_t s, const ZFaceArgs* zf, const void* pool,
+                               const Boxes* boxes) {
     const FieldPtrs<S>& F = *static_cast<const FieldPtrs<S>*>(fields);
     const MediaPtrs<R>& md = *static_cast<const MediaPtrs<R>*>(media);
     const Coefs<R>& c = *static_cast<const Coefs<R>*>(coefs);
+    // box lists run on the pair sweeps only
+    if (boxes && (sizeof(R) != 4 || cfg.variant != 0)) return hipErrorInvalidValue;
     if (cfg.variant == 1) {
         const Tiling t = tiling<M>(g, cfg, 4);
         if (VEL)
@@ -1034,18 +1675,36 @@ static hipError_t run_interior(const Geom& g, void* fields, const void* media, c
             int ty = cfg.tile_y == 4 ? 4 : 8;
             if constexpr (VEL && M <= 2)
                 if (cfg.tile_y == 16 || (cfg.tile_y == 0 && sizeof(S) == 4)) ty = 16;
-            const Tiling t = tiling<M>(g, cfg, ty, 128);
+            Boxes B;
+            int tz = 0;
+            if (boxes) {
+                B = fill_boxes<M>(g, *boxes, ty, cfg.x_chunk, tz);
+            } else {
+                const Tiling t = tiling<M>(g, cfg, ty, 128);
+                B.n = 1;
+                B.x0[0] = M;
+                B.x1[0] = g.nx - M;
+                B.y0[0] = M;
+                B.y1[0] = g.ny - M;
+                B.tyn[0] = t.tyn;
+                B.xc[0] = t.xchunk;
+                B.first[1] = t.nb;
+                tz = t.tz;
+            }
+            const int nb = B.first[B.n];
+            if (nb == 0) return hipSuccess;
+            const int grid = ((nb + 7) / 8) * 8;
 #define OPESCI_PAIR(TYV)                                                                                        \
     if (ty == TYV) {                                                                                            \
         if (VEL)                                                                                                \
-            hipLaunchKernelGGL((velocity_sweep2<S, M, TYV, HET>), dim3(t.grid), dim3(64 * TYV), 0, s, g, F, md, c, \
-                               t0, t1, t.tz, t.tyn, t.xchunk, t.nb);                                            \
+            hipLaunchKernelGGL((velocity_sweep2<S, M, TYV, HET>), dim3(grid), dim3(64 * TYV), 0, s, g, F, md, c,   \
+                               t0, t1, tz, nb, B);                                                              \
         else if (zf && zf->F > 0)                                                                               \
-            hipLaunchKernelGGL((stress_sweep2<S, M, TYV, false, true>), dim3(t.grid), dim3(64 * TYV), 0, s, g, F,   \
-                               md, c, t0, t1, t.tz, t.tyn, t.xchunk, t.nb, *zf);                              \
+            hipLaunchKernelGGL((stress_sweep2<S, M, TYV, false, true>), dim3(grid), dim3(64 * TYV), 0, s, g, F,     \
+                               md, c, t0, t1, tz, nb, B, *zf);                                                  \
         else                                                                                                    \
-            hipLaunchKernelGGL((stress_sweep2<S, M, TYV, HET, false>), dim3(t.grid), dim3(64 * TYV), 0, s, g, F, md, \
-                               c, t0, t1, t.tz, t.tyn, t.xchunk, t.nb, ZFaceArgs());                           \
+            hipLaunchKernelGGL((stress_sweep2<S, M, TYV, HET, false>), dim3(grid), dim3(64 * TYV), 0, s, g, F, md, \
+                               c, t0, t1, tz, nb, B, ZFaceArgs());                                              \
         return hipGetLastError();                                                                               \
     }
             OPESCI_PAIR(4)
@@ -1053,8 +1712,8 @@ static hipError_t run_interior(const Geom& g, void* fields, const void* media, c
 #undef OPESCI_PAIR
             if constexpr (VEL && M <= 2) {
                 if (ty == 16) {
-                    hipLaunchKernelGGL((velocity_sweep2<S, M, 16, HET>), dim3(t.grid), dim3(1024), 0, s, g, F, md, c,
-                                       t0, t1, t.tz, t.tyn, t.xchunk, t.nb);
+                    hipLaunchKernelGGL((velocity_sweep2<S, M, 16, HET>), dim3(grid), dim3(1024), 0, s, g, F, md, c,
+                                       t0, t1, tz, nb, B);
                     return hipGetLastError();
                 }
             }
@@ -1095,11 +1754,11 @@ static hipError_t run_interior(const Geom& g, void* fields, const void* media, c
 template <typename R, typename S, bool VEL>
 static hipError_t dispatch_m(const Geom& g, bool het, void* fields, const void* media, const void* coefs, int t0,
                              int t1, const InteriorLaunch& cfg, hipStream_t s, const ZFaceArgs* zf,
-                             const void* pool) {
-#define OPESCI_CASE(MM)                                                                                   \
-    case MM:                                                                                              \
-        return het ? run_interior<R, S, MM, true, VEL>(g, fields, media, coefs, t0, t1, cfg, s, zf, pool) \
-                   : run_interior<R, S, MM, false, VEL>(g, fields, media, coefs, t0, t1, cfg, s, zf, pool);
+                             const void* pool, const Boxes* boxes) {
+#define OPESCI_CASE(MM)                                                                                          \
+    case MM:                                                                                                     \
+        return het ? run_interior<R, S, MM, true, VEL>(g, fields, media, coefs, t0, t1, cfg, s, zf, pool, boxes) \
+                   : run_interior<R, S, MM, false, VEL>(g, fields, media, coefs, t0, t1, cfg, s, zf, pool, boxes);
     switch (g.m) {
         OPESCI_CASE(1)
         OPESCI_CASE(2)
@@ -1114,14 +1773,76 @@ static hipError_t dispatch_m(const Geom& g, bool het, void* fields, const void* 
 template <bool VEL>
 static hipError_t dispatch(const Geom& g, int real_bytes, int storage, bool het, void* fields, const void* media,
                            const void* coefs, int t0, int t1, const InteriorLaunch& cfg, hipStream_t s,
-                           const ZFaceArgs* zf, const void* pool) {
+                           const ZFaceArgs* zf, const void* pool, const Boxes* boxes) {
     if (real_bytes == 8)
-        return dispatch_m<double, double, VEL>(g, het, fields, media, coefs, t0, t1, cfg, s, nullptr, nullptr);
+        return dispatch_m<double, double, VEL>(g, het, fields, media, coefs, t0, t1, cfg, s, nullptr, nullptr, boxes);
     if (storage == OPESCI_STORE_F16)
-        return dispatch_m<float, _Float16, VEL>(g, het, fields, media, coefs, t0, t1, cfg, s, zf, pool);
+        return dispatch_m<float, _Float16, VEL>(g, het, fields, media, coefs, t0, t1, cfg, s, zf, pool, boxes);
     if (storage == OPESCI_STORE_BF16)
-        return dispatch_m<float, bf16_t, VEL>(g, het, fields, media, coefs, t0, t1, cfg, s, zf, pool);
-    return dispatch_m<float, float, VEL>(g, het, fields, media, coefs, t0, t1, cfg, s, zf, pool);
+        return dispatch_m<float, bf16_t, VEL>(g, het, fields, media, coefs, t0, t1, cfg, s, zf, pool, boxes);
+    return dispatch_m<float, float, VEL>(g, het, fields, media, coefs, t0, t1, cfg, s, zf, pool, boxes);
+}
+
+// ------------------------------------------------------------------ fused step launch
+#ifndef OPESCI_FUSED_TY
+#define OPESCI_FUSED_TY 4
+#endif
+bool fused_step_builds(int m) { return m == 1 || m == 2; }
+
+template <typename S, int M>
+static hipError_t run_fused(const Geom& g, void* fields, const void* coefs, int t0, int t1, const ZFaceArgs& zf,
+                            const Boxes& box, int x_chunk, hipStream_t s) {
+    constexpr int TY = OPESCI_FUSED_TY;
+    const FieldPtrs<S>& F = *static_cast<const FieldPtrs<S>*>(fields);
+    const Coefs<float>& c = *static_cast<const Coefs<float>*>(coefs);
+    StepPtrs<S> P;
+    for (int f = 0; f < 3; ++f) {
+        P.v0[f] = F.f[f][t0];
+        P.v1[f] = F.f[f][t1];
+    }
+    for (int f = 0; f < 6; ++f) {
+        P.s0[f] = F.f[3 + f][t0];
+        P.s1[f] = F.f[3 + f][t1];
+    }
+    if (box.n != 1 || box.x0[0] < 2 * M || box.y0[0] < 2 * M || box.x1[0] > g.nx - 2 * M ||
+        box.y1[0] > g.ny - 2 * M || !zf.coef || zf.F <= 0)
+        return hipErrorInvalidValue;
+    int xc = x_chunk;
+    if (xc <= 0) {
+        // one block per CU (the 12-wave blocks hold a CU's registers): split
+        // x so that the block count is close to 256
+        const int tz = (g.nz - 2 * M + 127) / 128, tyn = (box.y1[0] - box.y0[0] + TY - 1) / TY;
+        const int cols = std::max(1, tz * tyn), xl = box.x1[0] - box.x0[0];
+        const int tx = std::max(1, std::min(xl, (256 + cols / 2) / cols));
+        xc = (xl + tx - 1) / tx;
+    }
+    int tz = 0;
+    const Boxes B = fill_boxes<M>(g, box, TY, xc, tz);
+    const int nb = B.first[B.n];
+    if (nb == 0) return hipSuccess;
+    const int grid = ((nb + 7) / 8) * 8;
+    hipLaunchKernelGGL((step_sweep2<S, M, TY>), dim3(grid), dim3(64 * (TY + 2 * M)), 0, s, g, P, c, tz, nb, B, zf);
+    return hipGetLastError();
+}
+
+template <typename S>
+static hipError_t fused_m(const Geom& g, void* fields, const void* coefs, int t0, int t1, const ZFaceArgs& zf,
+                          const Boxes& box, int x_chunk, hipStream_t s) {
+    switch (g.m) {
+        case 1:
+            return run_fused<S, 1>(g, fields, coefs, t0, t1, zf, box, x_chunk, s);
+        case 2:
+            return run_fused<S, 2>(g, fields, coefs, t0, t1, zf, box, x_chunk, s);
+        default:
+            return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_step_fused(const Geom& g, int storage, void* fields, const void* coefs, int t0, int t1,
+                             const ZFaceArgs& zf, const Boxes& box, int x_chunk, hipStream_t s) {
+    if (storage == OPESCI_STORE_F16) return fused_m<_Float16>(g, fields, coefs, t0, t1, zf, box, x_chunk, s);
+    if (storage == OPESCI_STORE_BF16) return fused_m<bf16_t>(g, fields, coefs, t0, t1, zf, box, x_chunk, s);
+    return fused_m<float>(g, fields, coefs, t0, t1, zf, box, x_chunk, s);
 }
 
 bool stress_applies_zfaces(int real_bytes, bool het, const InteriorLaunch& cfg) {
@@ -1134,14 +1855,15 @@ bool stress_applies_zfaces(int real_bytes, bool het, const InteriorLaunch& cfg) 
 
 hipError_t launch_stress(const Geom& g, int real_bytes, int storage, bool het, void* fields, const void* media,
                          const void* coefs, int t0, int t1, const InteriorLaunch& cfg, hipStream_t s,
-                         const ZFaceArgs* zf, const void* pool) {
+                         const ZFaceArgs* zf, const void* pool, const Boxes* boxes) {
     if (zf && zf->F > 0 && !stress_applies_zfaces(real_bytes, het, cfg)) return hipErrorInvalidValue;
-    return dispatch<false>(g, real_bytes, storage, het, fields, media, coefs, t0, t1, cfg, s, zf, pool);
+    return dispatch<false>(g, real_bytes, storage, het, fields, media, coefs, t0, t1, cfg, s, zf, pool, boxes);
 }
 
 hipError_t launch_velocity(const Geom& g, int real_bytes, int storage, bool het, void* fields, const void* media,
-                           const void* coefs, int t0, int t1, const InteriorLaunch& cfg, hipStream_t s) {
-    return dispatch<true>(g, real_bytes, storage, het, fields, media, coefs, t0, t1, cfg, s, nullptr, nullptr);
+                           const void* coefs, int t0, int t1, const InteriorLaunch& cfg, hipStream_t s,
+                           const Boxes* boxes) {
+    return dispatch<true>(g, real_bytes, storage, het, fields, media, coefs, t0, t1, cfg, s, nullptr, nullptr, boxes);
 }
 
 }  // namespace opesci

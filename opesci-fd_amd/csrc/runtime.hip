@@ -93,6 +93,9 @@ struct OpesciContext {
     size_t field_bytes = 0;
     InteriorLaunch cfg{0, 0, 0};
     ZFaceArgs zf;                   // deep z-face statements the pair stress sweep applies (F = 0: none)
+    std::vector<OpesciBcTerm> h_terms;  // boundary terms as given (host)
+    int fused_F = 0;                // fused step: box [F, dim - F) in x and y (0: off)
+    Boxes fbox, frame;              // its box and the four boxes of rows around it
     int grad_every = 0;             // checkpoint interval of the last forward pass (0: none pending)
     bool grad_keep_t0 = false;
     ZCoef* d_zcoef = nullptr;       // their coefficient block (device)
@@ -428,6 +431,85 @@ bool zfaces_in_sweep(const OpesciContext* c) {
     return c->zf.F > 0 && stress_applies_zfaces(c->real_bytes, c->het, c->cfg);
 }
 
+// The fused step's box: the largest [F, dim - F) in x and y such that no
+// stress boundary statement left in phase 2 writes a cell the velocity
+// update of a point in the box reads (x and y in [F - M, dim - F + M)), and
+// no such statement reads a velocity at t1 (which the fused step would have
+// written before it).  The box rows are then deep z-face rows at every halo
+// plane and row (F - M >= the z faces' F), and the x and y stencils of the
+// sweep stay inside the grid (F >= 2M).
+int build_fused(OpesciContext* c) {
+    c->fused_F = 0;
+    c->fbox = Boxes();
+    c->frame = Boxes();
+    const Geom& g = c->g;
+    const int M = g.m;
+    if (!(c->desc.flags & OPESCI_FLAG_FUSED_STEP) || c->real_bytes != 4 || c->het || c->zf.F <= 0 ||
+        !fused_step_builds(M))
+        return 0;
+    for (const OpesciBcStmt& a : c->stmts) {
+        if (a.phase != 2) continue;
+        if (a.field < 3) return 0;
+        for (int k = 0; k < a.nterms; ++k) {
+            const OpesciBcTerm& t = c->h_terms[a.term0 + k];
+            if (t.field < 3 && t.level == 1) return 0;
+        }
+    }
+    const int fmax = std::min(g.nx, g.ny) / 2 - 1;
+    for (int F = std::max(c->zf.F + M, 2 * M); F <= fmax; ++F) {
+        const int lo = F - M, hix = g.nx - F + M, hiy = g.ny - F + M;
+        bool clear = true;
+        for (const OpesciBcStmt& a : c->stmts) {
+            if (a.phase != 2 || a.hi[0] <= a.lo[0] || a.hi[1] <= a.lo[1]) continue;
+            const int o0 = a.axis == 0 ? 1 : 0, o1 = a.axis == 2 ? 1 : 2;
+            int rx[2] = {0, 0}, ry[2] = {0, 0};
+            auto range = [&](int axis, int* r) {
+                if (axis == a.axis) {
+                    r[0] = a.plane;
+                    r[1] = a.plane + 1;
+                } else {
+                    const int j = axis == o0 ? 0 : 1;
+                    r[0] = a.lo[j];
+                    r[1] = a.hi[j];
+                }
+            };
+            (void)o1;
+            range(0, rx);
+            range(1, ry);
+            if (rx[0] < hix && lo < rx[1] && ry[0] < hiy && lo < ry[1]) {
+                clear = false;
+                break;
+            }
+        }
+        if (!clear) continue;
+        if (g.nx - 2 * F < 1 || g.ny - 2 * F < 1) return 0;
+        c->fused_F = F;
+        Boxes& b = c->fbox;
+        b.n = 1;
+        b.x0[0] = F;
+        b.x1[0] = g.nx - F;
+        b.y0[0] = F;
+        b.y1[0] = g.ny - F;
+        Boxes& f = c->frame;
+        f.n = 4;
+        const int X0[4] = {M, g.nx - F, F, F}, X1[4] = {F, g.nx - M, g.nx - F, g.nx - F};
+        const int Y0[4] = {M, M, M, g.ny - F}, Y1[4] = {g.ny - M, g.ny - M, F, g.ny - M};
+        for (int i = 0; i < 4; ++i) {
+            f.x0[i] = X0[i];
+            f.x1[i] = X1[i];
+            f.y0[i] = Y0[i];
+            f.y1[i] = Y1[i];
+        }
+        return 0;
+    }
+    return 0;
+}
+
+// whether steps run the fused kernel (FP32 pair sweeps with the z faces in place)
+bool fused_step(const OpesciContext* c) {
+    return c->fused_F > 0 && c->cfg.variant == 0 && zfaces_in_sweep(c);
+}
+
 int run_bc_phase(OpesciContext* c, int phase, int t0, int t1) {
     for (const BcLevel& lv : c->levels[phase])
         OPESCI_HIP_TRY(launch_bc(c->g, c->real_bytes, c->storage, c->fptr, c->d_desc, c->d_pool, lv, t0, t1, c->stream));
@@ -436,6 +518,20 @@ int run_bc_phase(OpesciContext* c, int phase, int t0, int t1) {
 
 int step_once(OpesciContext* c, int t0) {
     const int t1 = 1 - t0;
+    if (fused_step(c)) {
+        OPESCI_HIP_TRY(launch_step_fused(c->g, c->storage, c->fptr, c->coefs(), t0, t1, c->zf, c->fbox, 0, c->stream));
+        OPESCI_HIP_TRY(launch_stress(c->g, c->real_bytes, c->storage, c->het, c->fptr, c->mptr, c->coefs(), t0, t1,
+                                     c->cfg, c->stream, &c->zf, c->d_pool, &c->frame));
+        if (int e = run_bc_phase(c, 2, t0, t1)) return e;
+        OPESCI_HIP_TRY(launch_velocity(c->g, c->real_bytes, c->storage, c->het, c->fptr, c->mptr, c->coefs(), t0, t1,
+                                       c->cfg, c->stream, &c->frame));
+        if (int e = run_bc_phase(c, 3, t0, t1)) return e;
+        if (c->nrec > 0 || c->src_ntr)
+            OPESCI_HIP_TRY(launch_record_source(c->g, c->real_bytes, c->storage, c->fptr, t1, c->nrec, c->d_rec,
+                                                c->d_trace, c->src_ntr, c->src_idx, c->d_wavelet, c->d_step,
+                                                c->desc.ntsteps, c->stream));
+        return 0;
+    }
     const bool zin = zfaces_in_sweep(c);
     OPESCI_HIP_TRY(launch_stress(c->g, c->real_bytes, c->storage, c->het, c->fptr, c->mptr, c->coefs(), t0, t1,
                                  c->cfg, c->stream, zin ? &c->zf : nullptr, c->d_pool));
@@ -560,8 +656,10 @@ int create(const OpesciModelDesc* d, OpesciContext** out) {
                 c->cf.cs[s][gq][k] = (float)d->cs[s][gq][k];
             }
     c->stmts.assign(d->stmts, d->stmts + d->nstmts);
+    c->h_terms.assign(d->terms, d->terms + d->nterms);
     if (int e = build_program(c.get(), d->terms, d->dterms, d->ndterms, d->nterms)) return e;
     if (int e = build_zfaces(c.get())) return e;
+    if (int e = build_fused(c.get())) return e;
     if (int e = upload(&c->d_dterms, d->dterms, (size_t)d->ndterms, c->stream)) return e;
     if (int e = upload(&c->d_code, d->bc_code, (size_t)d->ncode, c->stream)) return e;
     // time-invariant media-dependent coefficients, once (the media are ready)
@@ -830,10 +928,35 @@ int opesci_ctx_receivers_f64(OpesciContext* ctx, double* host, int64_t capacity)
 int opesci_ctx_kernel_times(OpesciContext* ctx, int32_t nsteps, double out_ms[3]) {
     if (!ctx || !out_ms || nsteps <= 0) return fail(OPESCI_EINVAL, "bad arguments");
     if (!ctx->initialised) return fail(OPESCI_ESTATE, "opesci_ctx_init must run first");
-    std::vector<hipEvent_t> ev((size_t)nsteps * 5);
+    std::vector<hipEvent_t> ev((size_t)nsteps * 5 + 1);
     for (auto& e : ev) OPESCI_HIP_TRY(hipEventCreate(&e));
     int rc = 0;
-    for (int s = 0; s < nsteps && !rc; ++s) {
+    const bool fused = fused_step(ctx);
+    for (int s = 0; s < nsteps && !rc && fused; ++s) {
+        // fused kernel | frame stress | stress BCs | frame velocity | velocity BCs, sources
+        const int t0 = ctx->steps % 2, t1 = 1 - t0;
+        hipEvent_t* e = &ev[(size_t)s * 5];
+        OPESCI_HIP_TRY(hipEventRecord(e[0], ctx->stream));
+        OPESCI_HIP_TRY(launch_step_fused(ctx->g, ctx->storage, ctx->fptr, ctx->coefs(), t0, t1, ctx->zf, ctx->fbox, 0,
+                                         ctx->stream));
+        OPESCI_HIP_TRY(hipEventRecord(e[1], ctx->stream));
+        OPESCI_HIP_TRY(launch_stress(ctx->g, ctx->real_bytes, ctx->storage, ctx->het, ctx->fptr, ctx->mptr,
+                                     ctx->coefs(), t0, t1, ctx->cfg, ctx->stream, &ctx->zf, ctx->d_pool, &ctx->frame));
+        OPESCI_HIP_TRY(hipEventRecord(e[2], ctx->stream));
+        rc = run_bc_phase(ctx, 2, t0, t1);
+        OPESCI_HIP_TRY(hipEventRecord(e[3], ctx->stream));
+        OPESCI_HIP_TRY(launch_velocity(ctx->g, ctx->real_bytes, ctx->storage, ctx->het, ctx->fptr, ctx->mptr,
+                                       ctx->coefs(), t0, t1, ctx->cfg, ctx->stream, &ctx->frame));
+        OPESCI_HIP_TRY(hipEventRecord(e[4], ctx->stream));
+        if (!rc) rc = run_bc_phase(ctx, 3, t0, t1);
+        if (!rc && (ctx->nrec > 0 || ctx->src_ntr))
+            OPESCI_HIP_TRY(launch_record_source(ctx->g, ctx->real_bytes, ctx->storage, ctx->fptr, t1, ctx->nrec,
+                                                ctx->d_rec, ctx->d_trace, ctx->src_ntr, ctx->src_idx,
+                                                ctx->d_wavelet, ctx->d_step, ctx->desc.ntsteps, ctx->stream));
+        OPESCI_HIP_TRY(hipEventRecord(e[5], ctx->stream));
+        ++ctx->steps;
+    }
+    for (int s = 0; s < nsteps && !rc && !fused; ++s) {
         const int t0 = ctx->steps % 2, t1 = 1 - t0;
         hipEvent_t* e = &ev[(size_t)s * 5];
         OPESCI_HIP_TRY(hipEventRecord(e[0], ctx->stream));
@@ -858,7 +981,19 @@ int opesci_ctx_kernel_times(OpesciContext* ctx, int32_t nsteps, double out_ms[3]
     }
     OPESCI_HIP_TRY(hipStreamSynchronize(ctx->stream));
     double acc[3] = {0, 0, 0};
-    for (int s = 0; s < nsteps; ++s) {
+    for (int s = 0; s < nsteps && fused; ++s) {
+        hipEvent_t* e = &ev[(size_t)s * 5];
+        float a = 0, b = 0, c2 = 0, d2 = 0, e2 = 0;
+        OPESCI_HIP_TRY(hipEventElapsedTime(&a, e[0], e[1]));
+        OPESCI_HIP_TRY(hipEventElapsedTime(&b, e[1], e[2]));
+        OPESCI_HIP_TRY(hipEventElapsedTime(&c2, e[2], e[3]));
+        OPESCI_HIP_TRY(hipEventElapsedTime(&d2, e[3], e[4]));
+        OPESCI_HIP_TRY(hipEventElapsedTime(&e2, e[4], e[5]));
+        acc[0] += a;
+        acc[1] += b + d2;
+        acc[2] += c2 + e2;
+    }
+    for (int s = 0; s < nsteps && !fused; ++s) {
         hipEvent_t* e = &ev[(size_t)s * 5];
         float a = 0, b = 0, c2 = 0, d2 = 0;
         OPESCI_HIP_TRY(hipEventElapsedTime(&a, e[0], e[1]));
@@ -874,7 +1009,7 @@ int opesci_ctx_kernel_times(OpesciContext* ctx, int32_t nsteps, double out_ms[3]
     return rc;
 }
 
-int opesci_ctx_stats(OpesciContext* ctx, int32_t out[4]) {
+int opesci_ctx_stats(OpesciContext* ctx, int32_t out[5]) {
     if (!ctx || !out) return fail(OPESCI_EINVAL, "null argument");
     int n4 = 0;
     for (const BcLevel& lv : ctx->levels[4]) n4 += lv.count;
@@ -883,6 +1018,7 @@ int opesci_ctx_stats(OpesciContext* ctx, int32_t out[4]) {
     out[1] = n4;
     out[2] = zin ? 0 : (int)ctx->levels[4].size();
     out[3] = zin ? 1 : 0;
+    out[4] = fused_step(ctx) ? ctx->fused_F : 0;
     return 0;
 }
 

@@ -15,7 +15,8 @@ import numpy as np
 
 from .compilation import library_path
 
-ABI_VERSION = 1
+ABI_VERSION = 2
+FLAG_FUSED_STEP = 1
 FIELDS = ('U', 'V', 'W', 'Txx', 'Tyy', 'Tzz', 'Txy', 'Tyz', 'Txz')
 MEDIA_IDS = {'rho': 0, 'vp': 1, 'vs': 2, 'beta': 3, 'lambda': 4, 'mu': 5, 'beta1': 6, 'beta2': 7, 'beta3': 8,
              'mu12': 9, 'mu13': 10, 'mu23': 11}
@@ -144,6 +145,7 @@ def make_desc(plan):
     d.real_bytes = plan.real_bytes
     d.storage = plan.storage_code
     d.media_arrays = 1 if plan.media_arrays else 0
+    d.flags = FLAG_FUSED_STEP if getattr(plan, 'fuse_step', False) else 0
     d.dt = plan.dt
     d.spacing[:] = plan.spacing
     d.media_const[:] = [float(v) for v in plan.media_const]
@@ -286,9 +288,16 @@ class Context(object):
         """(boundary launches per step, deep z-face statements of the stress
         phase, launches they take[, 1 if the stress sweep applies them in
         place])."""
-        out = (_i32 * 4)()
+        out = (_i32 * 5)()
         _check(self.lib.opesci_ctx_stats(self.h, out), 'opesci_ctx_stats')
         return tuple(out[:4] if full else out[:3])
+
+    def fused_margin(self):
+        """F of the fused step's box [F, dim - F) in x and y, 0 when the
+        stress and velocity updates run as separate launches."""
+        out = (_i32 * 5)()
+        _check(self.lib.opesci_ctx_stats(self.h, out), 'opesci_ctx_stats')
+        return int(out[4])
 
     def configure(self, kernel_variant=0, tile_y=0, x_chunk=0):
         """Interior launch configuration: kernel_variant 0 sweep (two z points

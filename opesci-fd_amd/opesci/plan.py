@@ -664,6 +664,9 @@ def build_plan(grid):
     p.tile_y = getattr(grid, 'tile_y', 0)
     p.x_chunk = getattr(grid, 'x_chunk', 0)
     p.kernel_variant = getattr(grid, 'kernel_variant', 0)
+    # both half steps in one sweep where it applies (off by default: slower
+    # than the separate launches on MI355X)
+    p.fuse_step = bool(getattr(grid, 'fuse_step', False))
     p.use_graph = 1 if grid.use_graph else 0
     p.flops_per_point = None
     p.field_names = tuple(str(f.label) for f in slots)

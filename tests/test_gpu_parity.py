@@ -305,3 +305,56 @@ def test_fused_z_faces_equal_separate_launches(order, het, n, double, storage, t
     b, _ = gpu_run(plan2)
     for f in range(9):
         assert np.array_equal(a[f], b[f]), FIELDS[f]
+
+
+@pytest.mark.parametrize("sizes,order,storage,tile_y", [
+    ((70, 70, 70), 4, 'native', 0), ((140, 140, 140), 4, 'native', 0), ((61, 61, 61), 2, 'native', 0),
+    ((45, 52, 200), 2, 'native', 0), ((90, 90, 140), 4, 'fp16', 0), ((64, 64, 100), 4, 'bf16', 0),
+    ((40, 33, 150), 4, 'native', 4), ((23, 45, 131), 2, 'native', 8), ((57, 38, 131), 4, "native", 0)])
+def test_fused_step_equals_separate_launches(sizes, order, storage, tile_y):
+    """The fused step (stress and velocity of the box [F, dim - F) in one
+    sweep, the frame rows and every boundary statement as separate launches)
+    gives the separate stress / boundary / velocity launches' bits: one z
+    tile and several (z-halo points of the neighbouring tile computed in the
+    sweep), half orders 1 and 2, 16-bit storage, ragged y tiles, and a high
+    face just past a 128-point z tile boundary."""
+    g, _ = box_model(sizes, order, 9, storage=storage)
+    g.tile_y = tile_y
+    plan = g.lower()
+    plan.fuse_step = True
+    with Context(plan) as ctx:
+        F = ctx.fused_margin()
+        ctx.init()
+        ctx.run()
+        a = ctx.fields()
+    assert F > 0, 'fused step not applied'
+    plan.fuse_step = False
+    with Context(plan) as ctx:
+        assert ctx.fused_margin() == 0
+        ctx.init()
+        ctx.run()
+        b = ctx.fields()
+    for f in range(9):
+        assert np.array_equal(a[f], b[f]), (FIELDS[f], F)
+
+
+def test_fused_step_only_where_it_applies():
+    """Asked for, the fused step runs for FP32 arithmetic on constant media at
+    half orders 1 and 2; FP64, the read path, half order 3 and the gather
+    kernels keep separate launches, as does every model by default."""
+    cases = [(dict(double=True), 4), (dict(hetero=True), 4), ({}, 6), ({}, 4)]
+    for kw, order in cases:
+        g, _ = box_model((40, 40, 40), order, 3, **kw)
+        plan = g.lower()
+        plan.fuse_step = True
+        with Context(plan) as ctx:
+            want = not kw and order == 4
+            assert (ctx.fused_margin() > 0) == want, (kw, order)
+    g, _ = box_model((40, 40, 40), 4, 3)
+    plan = g.lower()
+    with Context(plan) as ctx:
+        assert ctx.fused_margin() == 0
+    plan.fuse_step = True
+    plan.kernel_variant = 1
+    with Context(plan) as ctx:
+        assert ctx.fused_margin() == 0
