@@ -95,7 +95,10 @@ def main():
         l2 = ctx.l2()
         kt = ctx.kernel_times(10)   # per-launch averages, after the timed region
         bc_stats = ctx.stats(full=True)
-    workload_key = 'stress_sweep %d^3 order %d %s%s' % (N, order, args.storage, ' hetero' if args.hetero else '')
+    # the traffic record of the stress launch that ran: the pair sweep with
+    # or without the deep z-face statements
+    workload_key = 'stress_sweep %d^3 order %d %s%s%s' % (N, order, args.storage, ' hetero' if args.hetero else '',
+                                                         ' z-faces in sweep' if bc_stats[3] else '')
     # flops per point-step: ADD + MUL of the stress and velocity kernels in the
     # reference's own operation count (get_*_kernel_ai)
     flops_pt = sum(ai[2] + ai[3] for ai in (g.get_stress_kernel_ai(), g.get_velocity_kernel_ai()))
@@ -142,7 +145,8 @@ def main():
             'unit': 'GB/s',
             'frac': achieved / HBM_PEAK,
             'traffic': committed_traffic(workload_key),
-            'kernel': 'stress update (6 stresses at t1 from U,V,W and stresses at t0)',
+            'kernel': 'stress update (6 stresses at t1 from U,V,W and stresses at t0%s)'
+                      % (', deep z-face statements in place' if bc_stats[3] else ''),
             'algorithmic_bytes_per_launch': stress_bytes,
         },
     }
@@ -191,27 +195,52 @@ def committed_traffic(key):
     return None
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or None."""
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(N, order, steps, hetero=False, runs=3):
     """Median of `runs` timed runs of the oracle's time loop, built with the
-    reference GNU flags, and of its -march=native build beside it."""
+    reference GNU flags, and of its -march=native build beside it, on the
+    cores this process may run on (its affinity mask, at most 16: the host
+    share of one GPU).  The whole call (allocation, initial state, the time
+    loop, the norms) is timed beside the loop."""
     sys.path.insert(0, os.path.join(ROOT, 'oracle'))
     import oracle
     from workloads import eigen_model, hetero_model
     _, kw = (hetero_model if hetero else eigen_model)(N, order, steps)
-    cores = min(16, os.cpu_count() or 1)
+    cores = min(16, len(os.sched_getaffinity(0)))
     kw['nthreads'] = cores
     kw['want_fields'] = False
     pts = float(N) ** 3 * steps
-    rates = {}
+    rates, whole = {}, {}
     for native in (False, True):
-        secs = sorted(oracle.run(native=native, **kw)['seconds'] for _ in range(runs))
-        rates[native] = pts / secs[len(secs) // 2]
+        loop, call = [], []
+        for _ in range(runs):
+            r = oracle.run(native=native, **kw)
+            loop.append(r['seconds'])
+            call.append(r['seconds_total'])
+        rates[native] = pts / sorted(loop)[len(loop) // 2]
+        whole[native] = sorted(call)[len(call) // 2]
     return {'value': rates[False], 'unit': 'grid-pt updates/s', 'cores': cores, 'kind': 'port',
+            'cpu': cpu_model(),
             'sample': 'median of %d runs of %d steps of the same %d^3 order-%d FP32 %s, oracle/elastic_oracle.cpp '
                       'built with the reference GNU flags (-O3 -fno-tree-vectorize -fopenmp), time loop only, '
                       'OpenMP on %d threads' % (runs, steps, N, order,
                                                 'heterogeneous model' if hetero else 'eigenwave', cores),
+            'whole_run_s': whole[False],
+            'whole_run_sample': 'the same runs timed around the whole call: allocation, initial state and '
+                                'boundary values, the time loop, the L2 norms (execute() and convergence())',
             'native_value': rates[True],
+            'native_whole_run_s': whole[True],
             'native_sample': 'the same built with -O3 -march=native -fopenmp (the stronger CPU baseline)'}
 
 
