@@ -7,10 +7,11 @@ small grids of different shapes and orders run side by side.
 
 The autotuner replaces the reference's Pluto tile-size sweep
 (opesci/regulargrid.py:21-23, tests/eigenwave3d.py --tile): for every
-distinct (grid shape, order, precision, media) it times a few interior
-launch configurations over two steps and keeps the fastest.  The kernels
-use the same arithmetic in every configuration, so tuning never changes the
-results.
+distinct (grid shape, order, precision, media) it times the interior launch
+configurations of CONFIGS -- kernel variant, rows per tile, x chunk -- each
+after one warm-up step, as the median of five single steps, and keeps the
+fastest.  The kernels use the same arithmetic in every configuration, so
+tuning never changes the results.
 """
 import ctypes
 
@@ -19,18 +20,22 @@ from . import backend
 __all__ = ['Batch', 'autotune', 'CONFIGS']
 
 # (kernel_variant, tile_y, x_chunk): the default sweep (two z points per lane
-# for FP32 arithmetic) with its default rows, 8 or 4 rows per tile, and the
-# one-point-per-lane sweep with 16 or 8 rows
-CONFIGS = ((0, 0, 0), (0, 8, 0), (0, 4, 0), (2, 16, 0), (2, 8, 0))
+# for FP32 arithmetic) with its default rows, 8 or 4 rows per tile, and x
+# chunks of 8, 16 or 32 planes beside the automatic choice; the
+# one-point-per-lane sweep with 16 or 8 rows, automatic or 16-plane chunks
+CONFIGS = ((0, 0, 0), (0, 8, 0), (0, 4, 0), (0, 0, 8), (0, 0, 16), (0, 0, 32), (0, 4, 16),
+           (2, 16, 0), (2, 8, 0), (2, 8, 16))
 
 
 def _key(plan):
     return (tuple(plan.dims), plan.order, plan.real_bytes, plan.storage, bool(plan.media_arrays))
 
 
-def autotune(ctx, configs=CONFIGS, steps=2, cache=None):
+def autotune(ctx, configs=CONFIGS, steps=5, cache=None):
     """Pick the fastest interior launch configuration for a context (the
-    context is re-initialised afterwards).  Returns (config, times_ms)."""
+    context is re-initialised afterwards): per configuration one warm-up
+    step, then the median of `steps` single steps of the stress and velocity
+    launches.  Returns (config, times_ms)."""
     key = _key(ctx.plan)
     if cache is not None and key in cache:
         best = cache[key]
@@ -40,8 +45,9 @@ def autotune(ctx, configs=CONFIGS, steps=2, cache=None):
     for cfg in configs:
         ctx.configure(*cfg)
         ctx.init()
-        kt = ctx.kernel_times(steps)
-        times[cfg] = kt[0] + kt[1]
+        ctx.kernel_times(1)
+        samples = sorted(sum(ctx.kernel_times(1)[:2]) for _ in range(steps))
+        times[cfg] = samples[len(samples) // 2]
     best = min(times, key=times.get)
     ctx.configure(*best)
     ctx.init()
