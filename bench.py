@@ -54,6 +54,9 @@ def main():
     ap.add_argument('--copy-probe', type=int, default=0, help='MiB copied device-to-device 3x first')
     ap.add_argument('--hetero', action='store_true',
                     help='seeded smooth heterogeneous medium through the read path (8 more media streams)')
+    ap.add_argument('--fused-step', action='store_true',
+                    help='both half steps in one sweep over the rows clear of the x and y faces (opt-in, '
+                         'slower than the separate sweeps on MI355X)')
     ap.add_argument('--no-split-z', action='store_true',
                     help='keep the deep rows of the stress z-face loops in the boundary launches '
                          '(default: split off and applied by the stress sweep)')
@@ -78,6 +81,7 @@ def main():
     if args.no_split_z:
         g.fuse_boundary = False
     plan = g.lower()
+    plan.fuse_step = args.fused_step
     if args.variant is not None:
         plan.kernel_variant = args.variant
     plan.x_chunk = args.x_chunk
@@ -95,6 +99,7 @@ def main():
         l2 = ctx.l2()
         kt = ctx.kernel_times(10)   # per-launch averages, after the timed region
         bc_stats = ctx.stats(full=True)
+        fused_F = ctx.fused_margin()
     # the traffic record of the stress launch that ran: the pair sweep with
     # or without the deep z-face statements
     workload_key = 'stress_sweep %d^3 order %d %s%s%s' % (N, order, args.storage, ' hetero' if args.hetero else '',
@@ -137,6 +142,7 @@ def main():
             'boundary_launches_per_step': bc_stats[0],
             'deep_z_face_statements': bc_stats[1],
             'deep_z_faces_in_stress_sweep': bool(bc_stats[3]),
+            'fused_step_margin': fused_F,
         },
         'roofline': {
             'bound': 'hbm',

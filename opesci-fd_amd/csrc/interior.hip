@@ -11,9 +11,10 @@
 //   T_ii += sum_j m_ij sum_k cn[i][j][k] (V_j[+k] - V_j[-1-k])          along j
 //   T_ij += m_s (sum_k cs[s][0][k] (V_i[+1+k] - V_i[-k]) along j + ... along i)
 //   V_i  += m_i sum_j sum_k cv[i][j][k] (T[a+k] - T[a-1-k])             along j
-// The velocity sum is one fma chain over j = y, z, x: the fused step forms
-// its y and z part while the stresses of an x plane are staged and completes
-// it with the x part M planes later, in the same order.
+// The velocity sum is one fma chain over j = x, y, z (the order that keeps
+// FP32 closest to FP64 on the 512^3 8th-order sweep; y, z, x measured 20-40 %
+// farther): the fused step keeps the y and z differences of an x plane while
+// its stresses are staged and sums the chain M planes later.
 // with m = 1 for constant media, and lambda(+2mu) / mu_ij / beta_i arrays in
 // the read path.  Arithmetic in the compute type R (float or double); fields
 // stored as S (float, double, _Float16, bf16).
@@ -160,10 +161,8 @@ __global__ __launch_bounds__(256) void velocity_gather(Geom g, FieldPtrs<S> F, M
         for (int i = 0; i < 3; ++i) {
             R acc = C::ld(F.f[i][t0] + q);
             R sum = 0;
-            // derivative terms along y, then z, then x
 #pragma unroll
-            for (int jj = 0; jj < 3; ++jj) {
-                const int j = jj == 2 ? 0 : jj + 1;
+            for (int j = 0; j < 3; ++j) {
                 const int a = (i == j) ? 1 : 0;
                 const S* t = F.f[src[i][j]][t1] + q;
 #pragma unroll
@@ -478,23 +477,23 @@ __global__ __launch_bounds__(64 * TY) void velocity_sweep(Geom g, FieldPtrs<S> F
             const int64_t q = (int64_t)x * g.sx + col;
             R su = 0, sv = 0, sw = 0;
 #pragma unroll
+            for (int k = 0; k < M; ++k) su = fmac(c.cv[0][0][k], qXX[M + k] - qXX[M - 1 - k], su);
+#pragma unroll
             for (int k = 0; k < M; ++k) su = fmac(c.cv[0][1][k], LXY[me + k * P::W] - LXY[me - (1 + k) * P::W], su);
 #pragma unroll
             for (int k = 0; k < M; ++k) su = fmac(c.cv[0][2][k], LXZ[me + k] - LXZ[me - 1 - k], su);
 #pragma unroll
-            for (int k = 0; k < M; ++k) su = fmac(c.cv[0][0][k], qXX[M + k] - qXX[M - 1 - k], su);
+            for (int k = 0; k < M; ++k) sv = fmac(c.cv[1][0][k], qXY[M + k] - qXY[M - 1 - k], sv);
 #pragma unroll
             for (int k = 0; k < M; ++k) sv = fmac(c.cv[1][1][k], LYY[me + (1 + k) * P::W] - LYY[me - k * P::W], sv);
 #pragma unroll
             for (int k = 0; k < M; ++k) sv = fmac(c.cv[1][2][k], LYZ[me + k] - LYZ[me - 1 - k], sv);
 #pragma unroll
-            for (int k = 0; k < M; ++k) sv = fmac(c.cv[1][0][k], qXY[M + k] - qXY[M - 1 - k], sv);
+            for (int k = 0; k < M; ++k) sw = fmac(c.cv[2][0][k], qXZ[M + k] - qXZ[M - 1 - k], sw);
 #pragma unroll
             for (int k = 0; k < M; ++k) sw = fmac(c.cv[2][1][k], LYZ[me + k * P::W] - LYZ[me - (1 + k) * P::W], sw);
 #pragma unroll
             for (int k = 0; k < M; ++k) sw = fmac(c.cv[2][2][k], LZZ[me + 1 + k] - LZZ[me - k], sw);
-#pragma unroll
-            for (int k = 0; k < M; ++k) sw = fmac(c.cv[2][0][k], qXZ[M + k] - qXZ[M - 1 - k], sw);
             if (HET) {
                 C::stnt(F.f[0][t1] + q, fmac(__builtin_nontemporal_load(md.m[OPESCI_M_BETA1] + q), su, u0));
                 C::stnt(F.f[1][t1] + q, fmac(__builtin_nontemporal_load(md.m[OPESCI_M_BETA2] + q), sv, v0));
@@ -978,23 +977,23 @@ __global__ __launch_bounds__(64 * TY) void velocity_sweep2(Geom g, FieldPtrs<S> 
                 const int me = me0 + v;
                 R su = 0, sv = 0, sw = 0;
 #pragma unroll
+                for (int k = 0; k < M; ++k) su = fmac(c.cv[0][0][k], qXX[v][M + k] - qXX[v][M - 1 - k], su);
+#pragma unroll
                 for (int k = 0; k < M; ++k) su = fmac(c.cv[0][1][k], LXY[me + k * P::W] - LXY[me - (1 + k) * P::W], su);
 #pragma unroll
                 for (int k = 0; k < M; ++k) su = fmac(c.cv[0][2][k], LXZ[me + k] - LXZ[me - 1 - k], su);
 #pragma unroll
-                for (int k = 0; k < M; ++k) su = fmac(c.cv[0][0][k], qXX[v][M + k] - qXX[v][M - 1 - k], su);
+                for (int k = 0; k < M; ++k) sv = fmac(c.cv[1][0][k], qXY[v][M + k] - qXY[v][M - 1 - k], sv);
 #pragma unroll
                 for (int k = 0; k < M; ++k) sv = fmac(c.cv[1][1][k], LYY[me + (1 + k) * P::W] - LYY[me - k * P::W], sv);
 #pragma unroll
                 for (int k = 0; k < M; ++k) sv = fmac(c.cv[1][2][k], LYZ[me + k] - LYZ[me - 1 - k], sv);
 #pragma unroll
-                for (int k = 0; k < M; ++k) sv = fmac(c.cv[1][0][k], qXY[v][M + k] - qXY[v][M - 1 - k], sv);
+                for (int k = 0; k < M; ++k) sw = fmac(c.cv[2][0][k], qXZ[v][M + k] - qXZ[v][M - 1 - k], sw);
 #pragma unroll
                 for (int k = 0; k < M; ++k) sw = fmac(c.cv[2][1][k], LYZ[me + k * P::W] - LYZ[me - (1 + k) * P::W], sw);
 #pragma unroll
                 for (int k = 0; k < M; ++k) sw = fmac(c.cv[2][2][k], LZZ[me + 1 + k] - LZZ[me - k], sw);
-#pragma unroll
-                for (int k = 0; k < M; ++k) sw = fmac(c.cv[2][0][k], qXZ[v][M + k] - qXZ[v][M - 1 - k], sw);
                 if (HET) {
                     ou[v] = fmac(b1[v], su, u0[v]);
                     ov[v] = fmac(b2[v], sv, v0[v]);
@@ -1032,13 +1031,14 @@ __global__ __launch_bounds__(64 * TY) void velocity_sweep2(Geom g, FieldPtrs<S> 
 // statements replace the surface values and produce the ghost cells.  The
 // stresses go to an LDS plane (rounded to the storage type, as the separate
 // velocity launch would read them back) and, for own rows and xs in
-// [xb, xe), to memory.  The own waves then form the y and z part of the
-// velocity sums of plane xs from that plane (the first part of the y, z, x
-// fma chain) and keep it M planes, until the stresses up to xs + M are
-// known: at step xs they complete the velocities of plane xs - M with the x
-// part from register queues of Txx, Txy, Txz.  So every value is the
-// separate launches' bit for bit, and a step reads U, V, W and the six
-// stresses once and writes them once (18 streams against 15 + 12).
+// [xb, xe), to memory.  The own waves then keep the y and z stencil
+// differences of the velocity sums of plane xs from that plane for M
+// planes, until the stresses up to xs + M are known: at step xs they form
+// the velocities of plane xs - M, x part from rings of Txx, Txy, Txz first,
+// then the kept differences, the separate velocity sweep's fma chain.  So
+// every value is the separate launches' bit for bit, and a step reads U, V,
+// W and the six stresses once and writes them once (18 streams against
+// 15 + 12).
 template <int M, int TY>
 struct FusedTile {
     static constexpr int NW = TY + 2 * M;   // waves
@@ -1117,10 +1117,12 @@ __global__ __launch_bounds__(64 * (TY + 2 * M)) void step_sweep2(Geom g, StepPtr
     __shared__ R vU[T::VH * T::VW], vV[T::VH * T::VW], vW[T::VH * T::VW];
     __shared__ R sXY[T::SH * T::SW], sYY[T::SH * T::SW], sYZ[T::SH * T::SW], sXZ[T::SH * T::SW],
         sZZ[T::SH * T::SW];
-    // own lanes: Txx, Txy, Txz of the last planes and the y, z part of the
-    // velocity sums, ring slot p % L for plane p, [slot][element][lane]
-    __shared__ R rXX[L * 2 * NOWN], rXY[L * 2 * NOWN], rXZ[L * 2 * NOWN], rPU[L * 2 * NOWN],
-        rPV[L * 2 * NOWN], rPW[L * 2 * NOWN];
+    // own lanes: Txx, Txy, Txz of the last planes, ring slot p % L for plane
+    // p, [slot][element][lane]; the y and z differences of the velocity
+    // sums of the last M + 1 planes, [p % (M + 1)][element][difference][lane]
+    // (U: Txy along y, Txz along z; V: Tyy, Tyz; W: Tyz, Tzz; M each)
+    constexpr int LD = M + 1, ND = 6 * M;
+    __shared__ R rXX[L * 2 * NOWN], rXY[L * 2 * NOWN], rXZ[L * 2 * NOWN], rD[LD * 2 * ND * NOWN];
     // z-halo pairs: U and W of the last planes, [row][side][slot][field][element]
     __shared__ R rI[TY * 2 * L * 2 * 2];
     __shared__ ZCoef zl;
@@ -1211,9 +1213,10 @@ __global__ __launch_bounds__(64 * (TY + 2 * M)) void step_sweep2(Geom g, StepPtr
     const int vmi = (w + M) * T::VW + (zi - (z0 - 2 * M));
     const int smi = w * T::SW + (zi - (z0 - M));
     auto rnd = [](float v) { return CS::from(CS::to(v)); };
-    int slot = ((xs0 % L) + L) % L;   // ring slot of plane x
+    int slot = ((xs0 % L) + L) % L;     // ring slot of plane x
+    int dslot = ((xs0 % LD) + LD) % LD;  // difference ring slot of plane x
     auto back = [&](int d) { return slot >= d ? slot - d : slot - d + L; };   // slot of plane x - d
-    for (int x = xs0; x < xs1; ++x, slot = slot + 1 == L ? 0 : slot + 1) {
+    for (int x = xs0; x < xs1; ++x, slot = slot + 1 == L ? 0 : slot + 1, dslot = dslot + 1 == LD ? 0 : dslot + 1) {
 #pragma unroll
         for (int v = 0; v < 2; ++v) {
 #pragma unroll
@@ -1515,39 +1518,31 @@ __global__ __launch_bounds__(64 * (TY + 2 * M)) void step_sweep2(Geom g, StepPtr
             }
         }
         __syncthreads();
-        // velocity: y and z part of plane x, completed for plane x - M
+        // velocity: y and z differences of plane x kept; the sums of plane
+        // x - M formed in the x, y, z order of the separate velocity sweep
         if (own && rowc) {
 #pragma unroll
             for (int v = 0; v < 2; ++v) {
                 const int sm = sme0 + v;
-                R su = 0, sv = 0, sw = 0;
+                R* d = rD + (dslot * 2 + v) * ND * NOWN + tw;
 #pragma unroll
-                for (int k = 0; k < M; ++k)
-                    su = fmac(c.cv[0][1][k], sXY[sm + k * T::SW] - sXY[sm - (1 + k) * T::SW], su);
-#pragma unroll
-                for (int k = 0; k < M; ++k) su = fmac(c.cv[0][2][k], sXZ[sm + k] - sXZ[sm - 1 - k], su);
-#pragma unroll
-                for (int k = 0; k < M; ++k)
-                    sv = fmac(c.cv[1][1][k], sYY[sm + (1 + k) * T::SW] - sYY[sm - k * T::SW], sv);
-#pragma unroll
-                for (int k = 0; k < M; ++k) sv = fmac(c.cv[1][2][k], sYZ[sm + k] - sYZ[sm - 1 - k], sv);
-#pragma unroll
-                for (int k = 0; k < M; ++k)
-                    sw = fmac(c.cv[2][1][k], sYZ[sm + k * T::SW] - sYZ[sm - (1 + k) * T::SW], sw);
-#pragma unroll
-                for (int k = 0; k < M; ++k) sw = fmac(c.cv[2][2][k], sZZ[sm + 1 + k] - sZZ[sm - k], sw);
-                const int r = (slot * 2 + v) * NOWN + tw;
-                rPU[r] = su;
-                rPV[r] = sv;
-                rPW[r] = sw;
+                for (int k = 0; k < M; ++k) {
+                    d[(0 * M + k) * NOWN] = sXY[sm + k * T::SW] - sXY[sm - (1 + k) * T::SW];
+                    d[(1 * M + k) * NOWN] = sXZ[sm + k] - sXZ[sm - 1 - k];
+                    d[(2 * M + k) * NOWN] = sYY[sm + (1 + k) * T::SW] - sYY[sm - k * T::SW];
+                    d[(3 * M + k) * NOWN] = sYZ[sm + k] - sYZ[sm - 1 - k];
+                    d[(4 * M + k) * NOWN] = sYZ[sm + k * T::SW] - sYZ[sm - (1 + k) * T::SW];
+                    d[(5 * M + k) * NOWN] = sZZ[sm + 1 + k] - sZZ[sm - k];
+                }
             }
             const int xv = x - M;
             if (rowo && xv >= xb && xv < xe) {
                 R ou[2], ov[2], ow[2];
+                const int ds = dslot >= M ? dslot - M : dslot - M + LD;   // plane x - M
 #pragma unroll
                 for (int v = 0; v < 2; ++v) {
-                    const int r0 = (back(M) * 2 + v) * NOWN + tw;
-                    R su = rPU[r0], sv = rPV[r0], sw = rPW[r0];
+                    const R* d = rD + (ds * 2 + v) * ND * NOWN + tw;
+                    R su = 0, sv = 0, sw = 0;
                     // Txx(xv-M+1 .. xv+M) = planes x-2M+1 .. x; Txy, Txz(xv-M .. xv+M-1)
 #pragma unroll
                     for (int k = 0; k < M; ++k) {
@@ -1556,17 +1551,29 @@ __global__ __launch_bounds__(64 * (TY + 2 * M)) void step_sweep2(Geom g, StepPtr
                         su = fmac(c.cv[0][0][k], a - b, su);
                     }
 #pragma unroll
+                    for (int k = 0; k < M; ++k) su = fmac(c.cv[0][1][k], d[(0 * M + k) * NOWN], su);
+#pragma unroll
+                    for (int k = 0; k < M; ++k) su = fmac(c.cv[0][2][k], d[(1 * M + k) * NOWN], su);
+#pragma unroll
                     for (int k = 0; k < M; ++k) {
                         const R a = rXY[(back(M - k) * 2 + v) * NOWN + tw];
                         const R b = rXY[(back(M + 1 + k) * 2 + v) * NOWN + tw];
                         sv = fmac(c.cv[1][0][k], a - b, sv);
                     }
 #pragma unroll
+                    for (int k = 0; k < M; ++k) sv = fmac(c.cv[1][1][k], d[(2 * M + k) * NOWN], sv);
+#pragma unroll
+                    for (int k = 0; k < M; ++k) sv = fmac(c.cv[1][2][k], d[(3 * M + k) * NOWN], sv);
+#pragma unroll
                     for (int k = 0; k < M; ++k) {
                         const R a = rXZ[(back(M - k) * 2 + v) * NOWN + tw];
                         const R b = rXZ[(back(M + 1 + k) * 2 + v) * NOWN + tw];
                         sw = fmac(c.cv[2][0][k], a - b, sw);
                     }
+#pragma unroll
+                    for (int k = 0; k < M; ++k) sw = fmac(c.cv[2][1][k], d[(4 * M + k) * NOWN], sw);
+#pragma unroll
+                    for (int k = 0; k < M; ++k) sw = fmac(c.cv[2][2][k], d[(5 * M + k) * NOWN], sw);
                     ou[v] = qU[v][0] + su;
                     ov[v] = qV[v][0] + sv;
                     ow[v] = qW[v][0] + sw;
