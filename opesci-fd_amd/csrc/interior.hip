@@ -780,87 +780,84 @@ __global__ __launch_bounds__(64 * TY) void stress_sweep2(Geom g, FieldPtrs<S> F,
                 // wave is one row: x and y are wave-uniform)
                 if (x >= zf.F && x < g.nx - zf.F && y >= zf.F && y < g.ny - zf.F) {
                     const int64_t row = (int64_t)x * g.sx + (int64_t)y * g.sy;
-                    const int blo = M, bhi = g.nz - M - 1, rh = zrh;
-                    const int eh = rh & 1;
-                    // image sources (interior values, before any replacement):
-                    // [Tzz, Tyz, Txz][k] of the low and the high face
-                    float slo[3][M], shi[3][M];
+                    const int rh = zrh;
+                    // image sources [face][Tzz, Tyz, Txz][k] (interior values,
+                    // before any replacement), read from the lanes holding
+                    // them while every interior lane is active
+                    float src[2][3][M];
 #pragma unroll
                     for (int k = 0; k < M; ++k) {
                         if (zlo) {
-                            slo[0][k] = k >= 1 ? lane_read<S>(out[2][k & 1], k >> 1) : 0.0f;
-                            slo[1][k] = lane_read<S>(out[4][k & 1], k >> 1);
-                            slo[2][k] = lane_read<S>(out[5][k & 1], k >> 1);
+                            src[0][0][k] = k >= 1 ? lane_read<S>(out[2][k & 1], k >> 1) : 0.0f;
+                            src[0][1][k] = lane_read<S>(out[4][k & 1], k >> 1);
+                            src[0][2][k] = lane_read<S>(out[5][k & 1], k >> 1);
                         }
                         if (zhi) {
                             const int r0 = rh - k, r1 = rh - 1 - k;
-                            shi[0][k] = k >= 1 ? lane_read<S>((r0 & 1) ? out[2][1] : out[2][0], r0 >> 1) : 0.0f;
-                            shi[1][k] = lane_read<S>((r1 & 1) ? out[4][1] : out[4][0], r1 >> 1);
-                            shi[2][k] = lane_read<S>((r1 & 1) ? out[5][1] : out[5][0], r1 >> 1);
+                            src[1][0][k] = k >= 1 ? lane_read<S>((r0 & 1) ? out[2][1] : out[2][0], r0 >> 1) : 0.0f;
+                            src[1][1][k] = lane_read<S>((r1 & 1) ? out[4][1] : out[4][0], r1 >> 1);
+                            src[1][2][k] = lane_read<S>((r1 & 1) ? out[5][1] : out[5][0], r1 >> 1);
                         }
                     }
-                    // Levander recompute of Txx (i = 0) or Tyy (i = 1) at the
-                    // surface plane, point element e of this (writer) lane
-                    auto recompute = [&](int face, int i, int e) -> float {
-                        float acc = 0.0f;
+                    // the lane holding a surface point: its face's Levander
+                    // recompute of Txx and Tyy (one pass over the nine terms,
+                    // point element e), surface values and ghost cells
+                    if (zlane >= 0) {
+                        const int face = zlane, e = face ? (rh & 1) : 0;
+                        float rxx = 0.0f, ryy = 0.0f;
                         if constexpr (M == 2) {
-#pragma unroll
-                            for (int k = 0; k < 9; ++k) {
-                                float v;
-                                if (k < 4)
-                                    v = e ? qU[1][k] : qU[0][k];
-                                else if (k < 8)
-                                    v = LV[me0 + e + (k - 6) * P::W];
-                                else
-                                    v = e ? T0[i][1] : T0[i][0];
-                                acc = fmac(zl.rc[face][i][k], v, acc);
-                            }
-                        }
-                        return acc;
-                    };
-                    // ghost cells: ordinary stores, so the few cells of each
-                    // face line merge in L2 before they reach HBM
-                    using CS = Conv<S, float>;
-                    if (zlane == 0) {
-                        if (zf.rec) {
-                            out[0][0] = recompute(0, 0, 0);
-                            out[1][0] = recompute(0, 1, 0);
-                        }
-                        out[2][0] = 0.0f;
-#pragma unroll
-                        for (int k = 0; k < M; ++k) {
-                            if (k >= 1) CS::st(F.f[5][t1] + row + blo - k, fmac(zl.zc[0][0][k], slo[0][k], 0.0f));
-                            CS::st(F.f[7][t1] + row + blo - 1 - k, fmac(zl.zc[0][1][k], slo[1][k], 0.0f));
-                            CS::st(F.f[8][t1] + row + blo - 1 - k, fmac(zl.zc[0][2][k], slo[2][k], 0.0f));
-                        }
-                    }
-                    if (zlane == 1) {
-                        const float rxx = zf.rec ? recompute(1, 0, eh) : 0.0f;
-                        const float ryy = zf.rec ? recompute(1, 1, eh) : 0.0f;
-                        const float tyz = fmac(zl.zc[1][1][0], shi[1][0], 0.0f);
-                        const float txz = fmac(zl.zc[1][2][0], shi[2][0], 0.0f);
-                        if (eh) {
                             if (zf.rec) {
-                                out[0][1] = rxx;
-                                out[1][1] = ryy;
+#pragma unroll
+                                for (int k = 0; k < 8; ++k) {
+                                    const float v = k < 4 ? (e ? qU[1][k] : qU[0][k]) : LV[me0 + e + (k - 6) * P::W];
+                                    rxx = fmac(zl.rc[face][0][k], v, rxx);
+                                    ryy = fmac(zl.rc[face][1][k], v, ryy);
+                                }
+                                rxx = fmac(zl.rc[face][0][8], e ? T0[0][1] : T0[0][0], rxx);
+                                ryy = fmac(zl.rc[face][1][8], e ? T0[1][1] : T0[1][0], ryy);
                             }
-                            out[2][1] = 0.0f;
-                            out[4][1] = tyz;
-                            out[5][1] = txz;
-                        } else {
+                        }
+                        float sv[3][M];
+#pragma unroll
+                        for (int f = 0; f < 3; ++f)
+#pragma unroll
+                            for (int k = 0; k < M; ++k) sv[f][k] = face ? src[1][f][k] : src[0][f][k];
+                        // ghost cells: ordinary stores, so the few cells of
+                        // each face line merge in L2 before they reach HBM
+                        using CS = Conv<S, float>;
+                        if (face == 0) {
+#pragma unroll
+                            for (int k = 0; k < M; ++k) {
+                                if (k >= 1) CS::st(F.f[5][t1] + row + M - k, fmac(zl.zc[0][0][k], sv[0][k], 0.0f));
+                                CS::st(F.f[7][t1] + row + M - 1 - k, fmac(zl.zc[0][1][k], sv[1][k], 0.0f));
+                                CS::st(F.f[8][t1] + row + M - 1 - k, fmac(zl.zc[0][2][k], sv[2][k], 0.0f));
+                            }
                             if (zf.rec) {
                                 out[0][0] = rxx;
                                 out[1][0] = ryy;
                             }
                             out[2][0] = 0.0f;
-                            out[4][0] = tyz;
-                            out[5][0] = txz;
-                        }
+                        } else {
+                            const int bhi = g.nz - M - 1;
 #pragma unroll
-                        for (int k = 1; k < M; ++k) {
-                            CS::st(F.f[5][t1] + row + bhi + k, fmac(zl.zc[1][0][k], shi[0][k], 0.0f));
-                            CS::st(F.f[7][t1] + row + bhi + k, fmac(zl.zc[1][1][k], shi[1][k], 0.0f));
-                            CS::st(F.f[8][t1] + row + bhi + k, fmac(zl.zc[1][2][k], shi[2][k], 0.0f));
+                            for (int k = 1; k < M; ++k) {
+                                CS::st(F.f[5][t1] + row + bhi + k, fmac(zl.zc[1][0][k], sv[0][k], 0.0f));
+                                CS::st(F.f[7][t1] + row + bhi + k, fmac(zl.zc[1][1][k], sv[1][k], 0.0f));
+                                CS::st(F.f[8][t1] + row + bhi + k, fmac(zl.zc[1][2][k], sv[2][k], 0.0f));
+                            }
+                            const float tyz = fmac(zl.zc[1][1][0], sv[1][0], 0.0f);
+                            const float txz = fmac(zl.zc[1][2][0], sv[2][0], 0.0f);
+#pragma unroll
+                            for (int v = 0; v < 2; ++v)
+                                if (v == e) {
+                                    if (zf.rec) {
+                                        out[0][v] = rxx;
+                                        out[1][v] = ryy;
+                                    }
+                                    out[2][v] = 0.0f;
+                                    out[4][v] = tyz;
+                                    out[5][v] = txz;
+                                }
                         }
                     }
                 }
