@@ -15,10 +15,12 @@
 // collects, at each velocity cell q, the terms of every interior p whose
 // stencil reads q; likewise for the shear and velocity updates (stencils in
 // interior.hip).  Boundary statements dst = sum c_k src_k are transposed per
-// point: a = bar[dst]; bar[dst] = 0; bar[src_k] += c_k a (atomic: points of a
-// statement may share a source cell; a level's sources are never its
-// destinations).  Their forward operands are the state before the level, which
-// an undo log restores level by level.
+// point: a = bar[dst]; bar[dst] = 0; bar[src_k] += c_k a.  Points of a level
+// may share a source cell (a level's sources are never its destinations), so
+// the additions are gathered: each point saves a in a buffer, then each
+// source cell sums its contributions in a fixed order (no atomics, results
+// independent of scheduling).  Their forward operands are the state before
+// the level, which an undo log restores level by level.
 //
 // Media gradients: lambda, mu (normal stresses), mu12/mu23/mu13 (shear),
 // beta1..3 (velocities) from the interior updates; media-dependent boundary
@@ -468,12 +470,16 @@ __global__ __launch_bounds__(256) void bc_undo_kernel(Geom g, FieldPtrs<R> F, co
     F.f[fl >> 1][(fl & 1) ? t1 : t0][dst] = log[(int64_t)blockIdx.y * maxpts + p];
 }
 
-// Transposed statements of one level; coefficient gradients into gpool.
+// Transposed statements of one level, first pass: each point's adjoint
+// value a = bar[dst] into the level's buffer, bar[dst] = 0, and the
+// coefficient gradients into gpool (one slot per point and term).
 template <typename R>
 __global__ __launch_bounds__(256) void adj_bc_kernel(Geom g, FieldPtrs<R> F, AdjPtrs<R> B,
                                                      const BcDesc* __restrict__ desc, const R* __restrict__ pool,
-                                                     R* gpool, int first, int t0, int t1) {
-    const BcDesc* D = desc + first + blockIdx.y;
+                                                     R* gpool, int first, const int32_t* __restrict__ boff, R* buf,
+                                                     int t0, int t1) {
+    const int d = first + blockIdx.y;
+    const BcDesc* D = desc + d;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= D->npts) return;
     const int n1 = D->n1, i0 = D->lo0 + p / n1, i1 = D->lo1 + p % n1;
@@ -484,57 +490,105 @@ __global__ __launch_bounds__(256) void adj_bc_kernel(Geom g, FieldPtrs<R> F, Adj
     R* bd = B.f[fl >> 1][(fl & 1) ? t1 : t0] + dst;
     const R a = *bd;
     *bd = 0;
+    buf[boff[d] + p] = a;
     if (a == (R)0) return;
-    for (int k = 0; k < D->nterms; ++k) {
-        const int sfl = D->t[k].fl;
-        const int lv = (sfl & 1) ? t1 : t0;
-        const int64_t src = dst + D->t[k].disp;
-        R coef = (R)D->t[k].coef;
+    for (int k = 0; k < D->nterms; ++k)
         if (D->t[k].cslot >= 0) {
+            const int sfl = D->t[k].fl;
             const int64_t ci = D->cpool + (int64_t)D->t[k].cslot * D->npts + p;
-            coef = pool[ci];
-            gpool[ci] += a * F.f[sfl >> 1][lv][src];
+            gpool[ci] += a * F.f[sfl >> 1][(sfl & 1) ? t1 : t0][dst + D->t[k].disp];
         }
-        atomicAdd(B.f[sfl >> 1][lv] + src, coef * a);
-    }
 }
 
-// gpool -> media gradients through the derivative programs of each term.
+// Second pass: bar[src] += c_k a over the source cell's contributions, in
+// (statement, point, term) order.
 template <typename R>
-__global__ __launch_bounds__(256) void adj_coef_media_kernel(Geom g, MediaPtrs<R> md, GradPtrs<R> gm,
-                                                             const BcDesc* __restrict__ desc,
+__global__ __launch_bounds__(256) void adj_bc_gather_kernel(AdjPtrs<R> B, const BcDesc* __restrict__ desc,
+                                                            const R* __restrict__ pool, AdjGather ga, int t0,
+                                                            int t1) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ga.ntgt) return;
+    const R* buf = static_cast<const R*>(ga.buf);
+    const int fl = ga.key[t];
+    R* q = B.f[fl >> 1][(fl & 1) ? t1 : t0] + ga.off[t];
+    R acc = *q;
+    for (int e = ga.start[t]; e < ga.start[t + 1]; ++e) {
+        const int ci = ga.cinfo[e];
+        const R coef = ci >= 0 ? pool[ci] : (R)desc[(-1 - ci) / BC_MAXT].t[(-1 - ci) % BC_MAXT].coef;
+        acc += coef * buf[ga.src[e]];
+    }
+    *q = acc;
+}
+
+// gpool -> media gradients through the derivative programs of each term,
+// first pass: every (point, term, derivative program) value into its slot
+// of the level's buffer (point p of descriptor d: slots boff[d] + p * n ..,
+// n the descriptor's derivative programs).
+template <typename R>
+__global__ __launch_bounds__(256) void adj_coef_media_kernel(Geom g, MediaPtrs<R> md, const BcDesc* __restrict__ desc,
                                                              const OpesciExprOp* __restrict__ code,
                                                              const OpesciBcDTerm* __restrict__ dterms,
-                                                             const R* __restrict__ gpool, int first) {
-    const BcDesc* D = desc + first + blockIdx.y;
+                                                             const R* __restrict__ gpool, int first,
+                                                             const int32_t* __restrict__ boff, R* buf) {
+    const int dd = first + blockIdx.y;
+    const BcDesc* D = desc + dd;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= D->npts) return;
+    int n = 0;
+    for (int k = 0; k < D->nterms; ++k)
+        if (D->t[k].cslot >= 0) n += D->t[k].ndterm;
+    if (n == 0) return;
     const int n1 = D->n1, axis = D->axis;
     const int i0 = D->lo0 + p / n1, i1 = D->lo1 + p % n1;
     const int ax = axis == 0 ? 0 : i0;
     const int ay = axis == 0 ? i0 : (axis == 1 ? 0 : i1);
     const int az = axis == 2 ? 0 : i1;
     const double c0[3] = {0, 0, 0};
+    R* out = buf + boff[dd] + (int64_t)p * n;
+    int j = 0;
     for (int k = 0; k < D->nterms; ++k) {
         if (D->t[k].cslot < 0) continue;
         const R gc = gpool[D->cpool + (int64_t)D->t[k].cslot * D->npts + p];
-        if (gc == (R)0) continue;
-        for (int d = D->t[k].dterm0; d < D->t[k].dterm0 + D->t[k].ndterm; ++d) {
+        for (int d = D->t[k].dterm0; d < D->t[k].dterm0 + D->t[k].ndterm; ++d, ++j) {
+            if (gc == (R)0) {
+                out[j] = (R)0;
+                continue;
+            }
             const OpesciBcDTerm& T = dterms[d];
             const double dv = (double)D->t[k].coef * vm_eval<R>(code + T.prog, T.prog_len, c0, 0.0, md, g, ax, ay, az);
-            atomicAdd(gm.m[T.media] + g.at(ax + T.off[0], ay + T.off[1], az + T.off[2]), (R)(gc * (R)dv));
+            out[j] = (R)(gc * (R)dv);
         }
     }
 }
 
-// Receiver misfit residual: bar U,V,W(t1) += field - observed at each receiver.
+// Second pass: each media cell sums its slots in (statement, point, term,
+// program) order.
 template <typename R>
-__global__ void adj_receiver_kernel(Geom g, FieldPtrs<R> F, AdjPtrs<R> B, int t1, int nrec,
-                                    const int32_t* __restrict__ rec, const float* __restrict__ obs) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nrec) return;
-    const int64_t q = g.at(rec[3 * r], rec[3 * r + 1], rec[3 * r + 2]);
-    for (int c = 0; c < 3; ++c) atomicAdd(B.f[c][t1] + q, F.f[c][t1][q] - (R)obs[3 * r + c]);
+__global__ __launch_bounds__(256) void adj_coef_gather_kernel(GradPtrs<R> gm, AdjGather ga) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ga.ntgt) return;
+    const R* buf = static_cast<const R*>(ga.buf);
+    R* q = gm.m[ga.key[t]] + ga.off[t];
+    R acc = *q;
+    for (int e = ga.start[t]; e < ga.start[t + 1]; ++e) acc += buf[ga.src[e]];
+    *q = acc;
+}
+
+// Receiver misfit residual: bar U,V,W(t1) += field - observed, one thread
+// per receiver cell summing its receivers in receiver order.
+template <typename R>
+__global__ void adj_receiver_kernel(FieldPtrs<R> F, AdjPtrs<R> B, int t1, int ngroups,
+                                    const int32_t* __restrict__ start, const int64_t* __restrict__ cell,
+                                    const int32_t* __restrict__ order, const float* __restrict__ obs) {
+    const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= ngroups) return;
+    const int64_t q = cell[gi];
+    for (int c = 0; c < 3; ++c) {
+        R acc = B.f[c][t1][q];
+        const R v = F.f[c][t1][q];
+        for (int e = start[gi]; e < start[gi + 1]; ++e) acc += v - (R)obs[3 * order[e] + c];
+        B.f[c][t1][q] = acc;
+    }
 }
 
 __global__ void set_step_kernel(int* step, int value) { *step = value; }
@@ -755,51 +809,60 @@ hipError_t launch_bc_logged(const Geom& g, int real_bytes, void* fields, const B
 
 template <typename R>
 static hipError_t adj_bc_t(const Geom& g, void* fields, void* bar, const BcDesc* desc, const void* pool, void* gpool,
-                           const BcLevel& lv, int t0, int t1, hipStream_t s) {
+                           const BcLevel& lv, const AdjGather& ga, int t0, int t1, hipStream_t s) {
     if (lv.count <= 0 || lv.maxpts <= 0) return hipSuccess;
     dim3 grid((lv.maxpts + 255) / 256, lv.count);
-    hipLaunchKernelGGL((adj_bc_kernel<R>), grid, dim3(256), 0, s, g, *static_cast<const FieldPtrs<R>*>(fields),
-                       *static_cast<const AdjPtrs<R>*>(bar), desc, static_cast<const R*>(pool),
-                       static_cast<R*>(gpool), lv.first, t0, t1);
+    const AdjPtrs<R>& B = *static_cast<const AdjPtrs<R>*>(bar);
+    hipLaunchKernelGGL((adj_bc_kernel<R>), grid, dim3(256), 0, s, g, *static_cast<const FieldPtrs<R>*>(fields), B,
+                       desc, static_cast<const R*>(pool), static_cast<R*>(gpool), lv.first, ga.boff,
+                       static_cast<R*>(ga.buf), t0, t1);
+    if (hipError_t e = hipGetLastError()) return e;
+    if (ga.ntgt <= 0) return hipSuccess;
+    hipLaunchKernelGGL((adj_bc_gather_kernel<R>), dim3((ga.ntgt + 255) / 256), dim3(256), 0, s, B, desc,
+                       static_cast<const R*>(pool), ga, t0, t1);
     return hipGetLastError();
 }
 
 hipError_t launch_adj_bc(const Geom& g, int real_bytes, void* fields, void* bar, const BcDesc* desc, const void* pool,
-                         void* gpool, const BcLevel& lv, int t0, int t1, hipStream_t s) {
-    return real_bytes == 8 ? adj_bc_t<double>(g, fields, bar, desc, pool, gpool, lv, t0, t1, s)
-                           : adj_bc_t<float>(g, fields, bar, desc, pool, gpool, lv, t0, t1, s);
+                         void* gpool, const BcLevel& lv, const AdjGather& ga, int t0, int t1, hipStream_t s) {
+    return real_bytes == 8 ? adj_bc_t<double>(g, fields, bar, desc, pool, gpool, lv, ga, t0, t1, s)
+                           : adj_bc_t<float>(g, fields, bar, desc, pool, gpool, lv, ga, t0, t1, s);
 }
 
 template <typename R>
 static hipError_t adj_coef_t(const Geom& g, const void* media, void* grad, const BcDesc* desc,
                              const OpesciExprOp* code, const OpesciBcDTerm* dterms, const void* gpool,
-                             const BcLevel& lv, hipStream_t s) {
-    if (lv.count <= 0 || lv.maxpts <= 0) return hipSuccess;
+                             const BcLevel& lv, const AdjGather& ga, hipStream_t s) {
+    if (lv.count <= 0 || lv.maxpts <= 0 || ga.ntgt <= 0) return hipSuccess;
     dim3 grid((lv.maxpts + 255) / 256, lv.count);
     hipLaunchKernelGGL((adj_coef_media_kernel<R>), grid, dim3(256), 0, s, g, *static_cast<const MediaPtrs<R>*>(media),
-                       *static_cast<const GradPtrs<R>*>(grad), desc, code, dterms, static_cast<const R*>(gpool),
-                       lv.first);
+                       desc, code, dterms, static_cast<const R*>(gpool), lv.first, ga.boff, static_cast<R*>(ga.buf));
+    if (hipError_t e = hipGetLastError()) return e;
+    hipLaunchKernelGGL((adj_coef_gather_kernel<R>), dim3((ga.ntgt + 255) / 256), dim3(256), 0, s,
+                       *static_cast<const GradPtrs<R>*>(grad), ga);
     return hipGetLastError();
 }
 
 hipError_t launch_adj_coef_media(const Geom& g, int real_bytes, const void* media, void* grad, const BcDesc* desc,
                                  const OpesciExprOp* code, const OpesciBcDTerm* dterms, const void* gpool,
-                                 const BcLevel& lv, hipStream_t s) {
-    return real_bytes == 8 ? adj_coef_t<double>(g, media, grad, desc, code, dterms, gpool, lv, s)
-                           : adj_coef_t<float>(g, media, grad, desc, code, dterms, gpool, lv, s);
+                                 const BcLevel& lv, const AdjGather& ga, hipStream_t s) {
+    return real_bytes == 8 ? adj_coef_t<double>(g, media, grad, desc, code, dterms, gpool, lv, ga, s)
+                           : adj_coef_t<float>(g, media, grad, desc, code, dterms, gpool, lv, ga, s);
 }
 
-hipError_t launch_adj_receivers(const Geom& g, int real_bytes, void* fields, void* bar, int t1, int nrec,
-                                const int32_t* rec, const float* obs, hipStream_t s) {
-    if (nrec <= 0) return hipSuccess;
+hipError_t launch_adj_receivers(const Geom& g, int real_bytes, void* fields, void* bar, int t1, int ngroups,
+                                const int32_t* start, const int64_t* cell, const int32_t* order, const float* obs,
+                                hipStream_t s) {
+    (void)g;
+    if (ngroups <= 0) return hipSuccess;
     if (real_bytes == 8)
-        hipLaunchKernelGGL((adj_receiver_kernel<double>), dim3((nrec + 255) / 256), dim3(256), 0, s, g,
+        hipLaunchKernelGGL((adj_receiver_kernel<double>), dim3((ngroups + 255) / 256), dim3(256), 0, s,
                            *static_cast<const FieldPtrs<double>*>(fields), *static_cast<const AdjPtrs<double>*>(bar),
-                           t1, nrec, rec, obs);
+                           t1, ngroups, start, cell, order, obs);
     else
-        hipLaunchKernelGGL((adj_receiver_kernel<float>), dim3((nrec + 255) / 256), dim3(256), 0, s, g,
+        hipLaunchKernelGGL((adj_receiver_kernel<float>), dim3((ngroups + 255) / 256), dim3(256), 0, s,
                            *static_cast<const FieldPtrs<float>*>(fields), *static_cast<const AdjPtrs<float>*>(bar),
-                           t1, nrec, rec, obs);
+                           t1, ngroups, start, cell, order, obs);
     return hipGetLastError();
 }
 

@@ -11,10 +11,12 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "common.h"
@@ -99,6 +101,17 @@ struct OpesciContext {
     int grad_every = 0;             // checkpoint interval of the last forward pass (0: none pending)
     bool grad_keep_t0 = false;
     ZCoef* d_zcoef = nullptr;       // their coefficient block (device)
+    // adjoint: fixed-order gathers per boundary level (transposes of phases
+    // 2-4, media maps of every phase) and the receivers grouped by cell
+    std::vector<OpesciBcDTerm> h_dterms;
+    std::vector<int32_t> h_rec;
+    std::vector<AdjGather> gat_bc[5], gat_md[5];
+    char* d_gather = nullptr;       // their index arrays, one allocation
+    void* d_abuf = nullptr;         // the per-level slot buffer (compute type)
+    int nrgroups = 0;
+    const int32_t* rg_start = nullptr;
+    const int64_t* rg_cell = nullptr;
+    const int32_t* rg_order = nullptr;
 
     const void* coefs() const { return real_bytes == 8 ? (const void*)&cd : (const void*)&cf; }
 };
@@ -581,7 +594,7 @@ void destroy(OpesciContext* c) {
     void* ptrs[] = {c->d_desc,  c->d_code,    c->d_sol,  c->d_partial, c->d_stage, c->d_rec,
                     c->d_trace, c->d_wavelet, c->d_step, c->d_pool,    c->d_dterms, c->d_gpool,
                     c->d_log,   c->d_ckpt,    c->d_seg,  c->d_obs,     c->d_srclog, c->d_tcells,
-                    c->d_vcells, c->d_scells, c->d_zcoef};
+                    c->d_vcells, c->d_scells, c->d_zcoef, c->d_gather, c->d_abuf};
     for (int f = 0; f < NF; ++f)
         for (int l = 0; l < 2; ++l)
             if (c->abase[f][l]) (void)hipFree(c->abase[f][l]);
@@ -661,6 +674,7 @@ int create(const OpesciModelDesc* d, OpesciContext** out) {
     if (int e = build_zfaces(c.get())) return e;
     if (int e = build_fused(c.get())) return e;
     if (int e = upload(&c->d_dterms, d->dterms, (size_t)d->ndterms, c->stream)) return e;
+    c->h_dterms.assign(d->dterms, d->dterms + d->ndterms);
     if (int e = upload(&c->d_code, d->bc_code, (size_t)d->ncode, c->stream)) return e;
     // time-invariant media-dependent coefficients, once (the media are ready)
     if (c->pool_elems > 0) {
@@ -684,6 +698,7 @@ int create(const OpesciModelDesc* d, OpesciContext** out) {
         if (int e = alloc_zero(c.get(), (void**)&c->d_step, 256)) return e;
         if (c->nrec) {
             if (int e = upload(&c->d_rec, d->rec_idx, (size_t)3 * c->nrec, c->stream)) return e;
+            c->h_rec.assign(d->rec_idx, d->rec_idx + (size_t)3 * c->nrec);
             const size_t tb = (size_t)std::max(d->ntsteps, 1) * c->nrec * 4 * d->real_bytes;
             if (int e = alloc_zero(c.get(), (void**)&c->d_trace, tb)) return e;
         }
@@ -1040,6 +1055,269 @@ int opesci_ctx_configure(OpesciContext* ctx, int32_t kernel_variant, int32_t til
 }
 
 // ---------------------------------------------------------------- adjoint
+// Index arrays of the fixed-order gathers (AdjGather): per level, every
+// scattered addition of the transposes (target: the source cell of a term,
+// value: coefficient x the point's adjoint value) and of the media maps
+// (target: the media cell of a derivative program, value: its slot),
+// generated in (statement, point, term[, program]) order and stably sorted by
+// target, so each target's sum runs in that order on every call.  Receivers
+// sharing a cell are grouped the same way, in receiver order.
+struct GatherEntry {
+    int32_t key;
+    int32_t src;
+    int64_t off;
+    int32_t cinfo;
+};
+struct GatherLevel {   // one level's gather, level-local indices
+    std::vector<int32_t> start, key, src, cinfo;
+    std::vector<int64_t> off;
+    int64_t slots = 0;
+    int err = 0;
+};
+
+// Sort one level's entries by (key, offset), ties in generation order: one
+// 64-bit sort key per entry, (target rank << ibits) | entry index.
+static void finish_level(std::vector<GatherEntry>& es, GatherLevel& out) {
+    const size_t n = es.size();
+    out.start.clear();
+    if (n == 0) {
+        out.start.push_back(0);
+        return;
+    }
+    int64_t omin = es[0].off, omax = es[0].off;
+    int32_t kmax = 0;
+    for (const GatherEntry& e : es) {
+        omin = std::min(omin, e.off);
+        omax = std::max(omax, e.off);
+        kmax = std::max(kmax, e.key);
+    }
+    int ibits = 1;
+    while (((size_t)1 << ibits) < n) ++ibits;
+    const uint64_t span = (uint64_t)(omax - omin) + 1;
+    std::vector<uint64_t> sk(n);
+    if ((unsigned __int128)((uint64_t)kmax + 1) * span < ((unsigned __int128)1 << (64 - ibits))) {
+        for (size_t i = 0; i < n; ++i)
+            sk[i] = (((uint64_t)es[i].key * span + (uint64_t)(es[i].off - omin)) << ibits) | i;
+        std::sort(sk.begin(), sk.end());
+        for (uint64_t& v : sk) v &= ((uint64_t)1 << ibits) - 1;
+    } else {
+        for (size_t i = 0; i < n; ++i) sk[i] = i;
+        std::stable_sort(sk.begin(), sk.end(), [&](uint64_t a, uint64_t b) {
+            return es[a].key != es[b].key ? es[a].key < es[b].key : es[a].off < es[b].off;
+        });
+    }
+    out.src.resize(n);
+    out.cinfo.resize(n);
+    for (size_t i = 0; i < n; ++i) {
+        const GatherEntry& e = es[sk[i]];
+        if (i == 0 || e.key != es[sk[i - 1]].key || e.off != es[sk[i - 1]].off) {
+            out.start.push_back((int32_t)i);
+            out.key.push_back(e.key);
+            out.off.push_back(e.off);
+        }
+        out.src[i] = e.src;
+        out.cinfo[i] = e.cinfo;
+    }
+    out.start.push_back((int32_t)n);
+}
+
+// transposes of one level: target = source cell of term k, value = its
+// coefficient (pool slot or literal) x the point's adjoint value
+static void gather_bc_level(const OpesciContext* c, const BcLevel& lv, int32_t* boff, GatherLevel& out) {
+    const Geom& g = c->g;
+    std::vector<GatherEntry> es;
+    int64_t slots = 0;
+    for (int di = lv.first; di < lv.first + lv.count; ++di) {
+        const BcDesc& d = c->h_desc[di];
+        boff[di] = (int32_t)slots;
+        const int o0 = d.axis == 0 ? 1 : 0, o1 = d.axis == 2 ? 1 : 2;
+        const int64_t so0 = o0 == 0 ? g.sx : g.sy, so1 = o1 == 1 ? g.sy : 1;
+        for (int p = 0; p < d.npts; ++p) {
+            const int64_t dst = (int64_t)(d.lo0 + p / d.n1) * so0 + (int64_t)(d.lo1 + p % d.n1) * so1 + d.disp;
+            for (int k = 0; k < d.nterms; ++k) {
+                const int32_t ci = d.t[k].cslot >= 0 ? (int32_t)(d.cpool + (int64_t)d.t[k].cslot * d.npts + p)
+                                                     : -(1 + di * BC_MAXT + k);
+                es.push_back({d.t[k].fl, (int32_t)(slots + p), dst + d.t[k].disp, ci});
+            }
+        }
+        slots += d.npts;
+        if (slots > INT32_MAX || es.size() > (size_t)INT32_MAX) {
+            out.err = 1;
+            return;
+        }
+    }
+    out.slots = slots;
+    finish_level(es, out);
+}
+
+// media maps of one level: target = the media cell a derivative program
+// differentiates, value = its slot (point p of descriptor d: boff[d] + p * n
+// + j, n the descriptor's programs)
+static void gather_md_level(const OpesciContext* c, const BcLevel& lv, int32_t* boff, GatherLevel& out) {
+    const Geom& g = c->g;
+    std::vector<GatherEntry> es;
+    int64_t slots = 0;
+    for (int di = lv.first; di < lv.first + lv.count; ++di) {
+        const BcDesc& d = c->h_desc[di];
+        boff[di] = (int32_t)slots;
+        int n = 0;
+        for (int k = 0; k < d.nterms; ++k)
+            if (d.t[k].cslot >= 0) n += d.t[k].ndterm;
+        if (n == 0) continue;
+        for (int p = 0; p < d.npts; ++p) {
+            const int i0 = d.lo0 + p / d.n1, i1 = d.lo1 + p % d.n1;
+            const int ax = d.axis == 0 ? 0 : i0;
+            const int ay = d.axis == 0 ? i0 : (d.axis == 1 ? 0 : i1);
+            const int az = d.axis == 2 ? 0 : i1;
+            int j = 0;
+            for (int k = 0; k < d.nterms; ++k) {
+                if (d.t[k].cslot < 0) continue;
+                for (int q = d.t[k].dterm0; q < d.t[k].dterm0 + d.t[k].ndterm; ++q, ++j) {
+                    const OpesciBcDTerm& T = c->h_dterms[q];
+                    es.push_back({T.media, (int32_t)(slots + (int64_t)p * n + j),
+                                  g.at(ax + T.off[0], ay + T.off[1], az + T.off[2]), 0});
+                }
+            }
+        }
+        slots += (int64_t)d.npts * n;
+        if (slots > INT32_MAX || es.size() > (size_t)INT32_MAX) {
+            out.err = 1;
+            return;
+        }
+    }
+    out.slots = slots;
+    finish_level(es, out);
+}
+
+static int build_adj_gathers(OpesciContext* c) {
+    const Geom& g = c->g;
+    const size_t rb = c->real_bytes;
+    const int nd = (int)c->h_desc.size();
+    std::vector<int32_t> boff_bc(nd, 0), boff_md(nd, 0);
+    // one task per (phase, level, kind), run on a few host threads
+    struct Task {
+        int ph, L, md;
+    };
+    std::vector<Task> tasks;
+    for (int ph = 0; ph < 5; ++ph)
+        for (int L = 0; L < (int)c->levels[ph].size(); ++L) {
+            if (ph >= 2) tasks.push_back({ph, L, 0});
+            tasks.push_back({ph, L, 1});
+        }
+    std::vector<GatherLevel> res(tasks.size());
+    {
+        std::atomic<size_t> next{0};
+        auto work = [&]() {
+            for (size_t t; (t = next.fetch_add(1)) < tasks.size();) {
+                const Task& k = tasks[t];
+                const BcLevel& lv = c->levels[k.ph][k.L];
+                if (k.md)
+                    gather_md_level(c, lv, boff_md.data(), res[t]);
+                else
+                    gather_bc_level(c, lv, boff_bc.data(), res[t]);
+            }
+        };
+        const int nt = (int)std::min<size_t>(tasks.size(), std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+        std::vector<std::thread> pool;
+        for (int i = 1; i < nt; ++i) pool.emplace_back(work);
+        work();
+        for (std::thread& th : pool) th.join();
+    }
+    // concatenate in task order
+    std::vector<int32_t> start, key, src, cinfo, rstart, rorder;
+    std::vector<int64_t> off, rcell;
+    struct Span {
+        size_t e0, s0, t0;   // first entry, first start, first target
+        int ntgt;
+    };
+    std::vector<Span> sp_bc[5], sp_md[5];
+    for (int ph = 0; ph < 2; ++ph) sp_bc[ph].assign(c->levels[ph].size(), Span{0, 0, 0, 0});
+    int64_t maxslots = 0;
+    for (size_t t = 0; t < tasks.size(); ++t) {
+        GatherLevel& r = res[t];
+        if (r.err) return fail(OPESCI_EUNSUPPORTED, "boundary level too large for the adjoint");
+        (tasks[t].md ? sp_md : sp_bc)[tasks[t].ph].push_back(Span{src.size(), start.size(), key.size(), (int)r.key.size()});
+        maxslots = std::max(maxslots, r.slots);
+        start.insert(start.end(), r.start.begin(), r.start.end());
+        key.insert(key.end(), r.key.begin(), r.key.end());
+        off.insert(off.end(), r.off.begin(), r.off.end());
+        src.insert(src.end(), r.src.begin(), r.src.end());
+        cinfo.insert(cinfo.end(), r.cinfo.begin(), r.cinfo.end());
+        r = GatherLevel();
+    }
+    // receivers by cell
+    {
+        std::vector<std::pair<int64_t, int32_t>> rc((size_t)c->nrec);
+        for (int r = 0; r < c->nrec; ++r)
+            rc[r] = {g.at(c->h_rec[3 * r], c->h_rec[3 * r + 1], c->h_rec[3 * r + 2]), r};
+        std::sort(rc.begin(), rc.end());
+        for (size_t i = 0; i < rc.size(); ++i) {
+            if (i == 0 || rc[i].first != rc[i - 1].first) {
+                rstart.push_back((int32_t)i);
+                rcell.push_back(rc[i].first);
+            }
+            rorder.push_back(rc[i].second);
+        }
+        rstart.push_back((int32_t)rc.size());
+        c->nrgroups = (int)rcell.size();
+    }
+    // one allocation: sections aligned to 256 bytes
+    size_t bytes = 0;
+    auto sec = [&](size_t n) {
+        const size_t o = bytes;
+        bytes += (n + 255) & ~size_t(255);
+        return o;
+    };
+    const size_t o_start = sec(start.size() * 4), o_key = sec(key.size() * 4), o_off = sec(off.size() * 8),
+                 o_src = sec(src.size() * 4), o_ci = sec(cinfo.size() * 4), o_bbc = sec((size_t)nd * 4),
+                 o_bmd = sec((size_t)nd * 4), o_rs = sec(rstart.size() * 4), o_rc = sec(rcell.size() * 8),
+                 o_ro = sec(rorder.size() * 4);
+    OPESCI_HIP_TRY(hipMalloc((void**)&c->d_gather, std::max(bytes, (size_t)256)));
+    std::vector<char> blob(bytes);
+    auto put = [&](size_t o, const void* p, size_t n) {
+        if (n) memcpy(blob.data() + o, p, n);
+    };
+    put(o_start, start.data(), start.size() * 4);
+    put(o_key, key.data(), key.size() * 4);
+    put(o_off, off.data(), off.size() * 8);
+    put(o_src, src.data(), src.size() * 4);
+    put(o_ci, cinfo.data(), cinfo.size() * 4);
+    put(o_bbc, boff_bc.data(), (size_t)nd * 4);
+    put(o_bmd, boff_md.data(), (size_t)nd * 4);
+    put(o_rs, rstart.data(), rstart.size() * 4);
+    put(o_rc, rcell.data(), rcell.size() * 8);
+    put(o_ro, rorder.data(), rorder.size() * 4);
+    if (bytes) {
+        OPESCI_HIP_TRY(hipMemcpyAsync(c->d_gather, blob.data(), bytes, hipMemcpyHostToDevice, c->stream));
+        OPESCI_HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    if (int e = alloc_zero(c, &c->d_abuf, ((size_t)std::max<int64_t>(maxslots, 1) * rb + 255) & ~size_t(255))) return e;
+    char* base = c->d_gather;
+    auto views = [&](const std::vector<Span>& sp, size_t o_boff, std::vector<AdjGather>& out) {
+        out.clear();
+        for (const Span& s : sp) {
+            AdjGather a;
+            a.ntgt = s.ntgt;
+            a.start = (const int32_t*)(base + o_start) + s.s0;
+            a.key = (const int32_t*)(base + o_key) + s.t0;
+            a.off = (const int64_t*)(base + o_off) + s.t0;
+            a.src = (const int32_t*)(base + o_src) + s.e0;
+            a.cinfo = (const int32_t*)(base + o_ci) + s.e0;
+            a.boff = (const int32_t*)(base + o_boff);
+            a.buf = c->d_abuf;
+            out.push_back(a);
+        }
+    };
+    for (int ph = 0; ph < 5; ++ph) {
+        views(sp_bc[ph], o_bbc, c->gat_bc[ph]);
+        views(sp_md[ph], o_bmd, c->gat_md[ph]);
+    }
+    c->rg_start = (const int32_t*)(base + o_rs);
+    c->rg_cell = (const int64_t*)(base + o_rc);
+    c->rg_order = (const int32_t*)(base + o_ro);
+    return 0;
+}
+
 static int adj_alloc(OpesciContext* c, int ckpt, size_t seg_bytes, int log_steps) {
     const Geom& g = c->g;
     const size_t rb = c->real_bytes;
@@ -1120,6 +1398,7 @@ static int adj_alloc(OpesciContext* c, int ckpt, size_t seg_bytes, int log_steps
             c->n_vcells = (int)vc.size();
             c->n_scells = (int)sc.size();
         }
+        if (int e = build_adj_gathers(c)) return e;
         int64_t off = 0;
         for (int ph = 2; ph < 5; ++ph) {
             c->log_off[ph].clear();
@@ -1199,15 +1478,15 @@ static int step_adjoint(OpesciContext* c, int n, void* fp, char* log) {
     auto levels = [&](int ph) -> int {
         for (int L = (int)c->levels[ph].size() - 1; L >= 0; --L) {
             const BcLevel& lv = c->levels[ph][L];
-            OPESCI_HIP_TRY(launch_adj_bc(c->g, c->real_bytes, fp, c->aptr, c->d_desc, c->d_pool, c->d_gpool, lv, t0,
-                                         t1, c->stream));
+            OPESCI_HIP_TRY(launch_adj_bc(c->g, c->real_bytes, fp, c->aptr, c->d_desc, c->d_pool, c->d_gpool, lv,
+                                         c->gat_bc[ph][L], t0, t1, c->stream));
             OPESCI_HIP_TRY(launch_bc_logged(c->g, c->real_bytes, fp, c->d_desc, c->d_pool, lv,
                                             log + c->log_off[ph][L] * c->real_bytes, t0, t1, 1, c->stream));
         }
         return 0;
     };
-    OPESCI_HIP_TRY(launch_adj_receivers(c->g, c->real_bytes, fp, c->aptr, t1, c->nrec, c->d_rec,
-                                        c->d_obs + (size_t)n * c->nrec * 3, c->stream));
+    OPESCI_HIP_TRY(launch_adj_receivers(c->g, c->real_bytes, fp, c->aptr, t1, c->nrgroups, c->rg_start, c->rg_cell,
+                                        c->rg_order, c->d_obs + (size_t)n * c->nrec * 3, c->stream));
     if (int e = levels(3)) return e;
     OPESCI_HIP_TRY(launch_adj_velocity(c->g, c->real_bytes, fp, c->aptr, c->mptr, c->gptr, c->coefs(), c->sptr,
                                        AdjCells{c->d_vcells, c->n_vcells}, c->cfg, t0, t1, c->stream));
@@ -1403,9 +1682,10 @@ int opesci_ctx_gradient_reverse(OpesciContext* c) {
     }
     // media-dependent boundary coefficients -> media cells
     for (int ph = 0; ph < 5; ++ph)
-        for (const BcLevel& lv : c->levels[ph])
+        for (size_t L = 0; L < c->levels[ph].size(); ++L)
             OPESCI_HIP_TRY(launch_adj_coef_media(c->g, c->real_bytes, c->mptr, c->gptr, c->d_desc, c->d_code,
-                                                 c->d_dterms, c->d_gpool, lv, c->stream));
+                                                 c->d_dterms, c->d_gpool, c->levels[ph][L], c->gat_md[ph][L],
+                                                 c->stream));
     OPESCI_HIP_TRY(hipStreamSynchronize(c->stream));
     OPESCI_HIP_TRY(hipGetLastError());
     return 0;

@@ -24,7 +24,7 @@ ARCH = os.environ.get('OPESCI_OFFLOAD_ARCH', 'gfx950')
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 HIP_FLAGS = ['-O3', '-std=c++17', '-fPIC', '--offload-arch=%s' % ARCH,
              '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
-             '-munsafe-fp-atomics', '-Wno-unused-result', '-I', INCLUDE, '-I', CSRC]
+             '-Wno-unused-result', '-I', INCLUDE, '-I', CSRC]
 
 __all__ = ['HIPCompiler', 'get_compiler', 'library_path', 'build_library']
 
