@@ -128,6 +128,13 @@ typedef struct OpesciBcDTerm {
  * launches bit for bit; on MI355X it is slower than they are (256^3, 4th
  * order: 694 us against 386 us per step), so it is off by default. */
 #define OPESCI_FLAG_FUSED_STEP 1
+/* OpesciModelDesc.flags: evaluate the analytic-solution programs (initial
+ * state, L2 norms) at every point as given.  By default every part of a
+ * program that reads one coordinate (or none) and no media is evaluated once
+ * per index along that axis into a table the point evaluation reads: the
+ * same double operations in the same order, so the values are bit for bit
+ * those of this flag, at a fraction of the cost. */
+#define OPESCI_FLAG_POINTWISE_SOLUTION 2
 
 typedef struct OpesciModelDesc {
     int32_t abi_version;

@@ -49,33 +49,103 @@ __global__ __launch_bounds__(256) void bc_coef_kernel(Geom g, MediaPtrs<R> md, c
 }
 
 struct SolTable {
-    int off[NF];
-    int len[NF];
     double dx[3];
     double t[NF];
 };
 
-// ------------------------------------------------------------------ initial state
+// The tables of the split analytic programs: slot blockIdx.y at every index
+// i along its axis, coordinates as the point kernels compute them.
+template <typename R>
+__global__ __launch_bounds__(256) void sol_table_kernel(Geom g, MediaPtrs<R> md, SolProg sp, SolTable tab) {
+    const SolSlot S = sp.slots[blockIdx.y];
+    const int n = S.axis == 0 ? g.nx : (S.axis == 1 ? g.ny : g.nz);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double c[3] = {0.0, 0.0, 0.0};
+    c[S.axis] = coord_of<R>(tab.dx[S.axis], i, g.m, stag(S.field, S.axis));
+    sp.tab[S.tab + i] = vm_eval<R>(sp.code + S.prog, S.len, c, tab.t[S.field], md, g, 0, 0, 0);
+}
+
+// Field f's tables for the point kernels: copied into the block's LDS when
+// they fit lds_cap doubles (the launch's dynamic LDS), else read in place.
+constexpr int SOL_LDS_CAP = 4096;   // doubles (32 KiB)
+__device__ __forceinline__ const double* stage_tables(const SolProg& sol, int f, int lds_cap) {
+    extern __shared__ double stab[];
+    const int n = sol.tab_len[f];
+    if (n == 0 || n > lds_cap) return sol.tab + sol.tab_base[f];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) stab[i] = sol.tab[sol.tab_base[f] + i];
+    __syncthreads();
+    return stab;
+}
+
+// ------------------------------------------------------------------ initial state, L2 norms
+// Field f's analytic values at z on K rows (x[k], y[k]) of a block: the K
+// points of a shallow program in one pass over its ops, the others one
+// point at a time.
+constexpr int SOL_ROWS = 4;
+template <typename R>
+__device__ __forceinline__ void sol_points(const SolProg& sol, const OpesciExprOp* __restrict__ code, int f,
+                                           const SolTable& tab, const MediaPtrs<R>& md, const Geom& g,
+                                           const int x[SOL_ROWS], const int y[SOL_ROWS], int z,
+                                           const double* tabs, double v[SOL_ROWS]) {
+    const int m = g.m;
+    double cx[SOL_ROWS], cy[SOL_ROWS];
+#pragma unroll
+    for (int k = 0; k < SOL_ROWS; ++k) {
+        cx[k] = coord_of<R>(tab.dx[0], x[k], m, stag(f, 0));
+        cy[k] = coord_of<R>(tab.dx[1], y[k], m, stag(f, 1));
+    }
+    const double cz = coord_of<R>(tab.dx[2], z, m, stag(f, 2));
+    if (sol.shallow[f]) {
+        vm_eval_k<R, SOL_ROWS>(code + sol.off[f], sol.len[f], cx, cy, cz, tab.t[f], md, g, x, y, z, tabs, v);
+    } else {
+        for (int k = 0; k < SOL_ROWS; ++k) {
+            const double c[3] = {cx[k], cy[k], cz};
+            v[k] = vm_eval_tos<R>(code + sol.off[f], sol.len[f], c, tab.t[f], md, g, x[k], y[k], z, tabs);
+        }
+    }
+}
+
+// rows r0 .. r0 + SOL_ROWS - 1 of field f's interior (x, y) rows; past the
+// last row, repeats of it flagged not ok
+__device__ __forceinline__ void sol_rows(int r0, int rows, int ny, int m, int x[SOL_ROWS], int y[SOL_ROWS],
+                                         bool ok[SOL_ROWS]) {
+#pragma unroll
+    for (int k = 0; k < SOL_ROWS; ++k) {
+        const int r = min(r0 + k, rows - 1);
+        ok[k] = r0 + k < rows;
+        x[k] = m + r / ny;
+        y[k] = m + r % ny;
+    }
+}
+
 // StaggeredGrid.initialise (opesci/staggeredgrid.py:612-659): level 0,
 // ranges [m, dim-m) or [m, dim-m-1) when staggered, t = dt/2 for velocities.
 template <typename R, typename S>
-__global__ __launch_bounds__(256) void initial_kernel(Geom g, FieldPtrs<S> F, MediaPtrs<R> md,
-                                                      const OpesciExprOp* __restrict__ sol, SolTable tab) {
+__global__ __launch_bounds__(256) void initial_kernel(Geom g, FieldPtrs<S> F, MediaPtrs<R> md, SolProg sol,
+                                                      const OpesciExprOp* __restrict__ code, SolTable tab,
+                                                      int lds_cap) {
     using C = Conv<S, R>;
     const int f = blockIdx.y;
     const int m = g.m;
     const int nx = g.nx - 2 * m - stag(f, 0), ny = g.ny - 2 * m - stag(f, 1), nz = g.nz - 2 * m - stag(f, 2);
-    const int64_t n = (int64_t)nx * ny * nz;
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
-        const int z = m + (int)(p % nz), y = m + (int)((p / nz) % ny), x = m + (int)(p / ((int64_t)nz * ny));
-        const double c[3] = {coord_of<R>(tab.dx[0], x, m, stag(f, 0)), coord_of<R>(tab.dx[1], y, m, stag(f, 1)),
-                             coord_of<R>(tab.dx[2], z, m, stag(f, 2))};
-        const double v = vm_eval<R>(sol + tab.off[f], tab.len[f], c, tab.t[f], md, g, x, y, z);
-        C::st(F.f[f][0] + g.at(x, y, z), (R)v);
+    const int rows = nx * ny;
+    const double* tabs = stage_tables(sol, f, lds_cap);
+    // SOL_ROWS (x, y) rows per block iteration (block-uniform indices), z across threads
+    for (int r0 = blockIdx.x * SOL_ROWS; r0 < rows; r0 += gridDim.x * SOL_ROWS) {
+        int x[SOL_ROWS], y[SOL_ROWS];
+        bool ok[SOL_ROWS];
+        sol_rows(r0, rows, ny, m, x, y, ok);
+        for (int z = m + (int)threadIdx.x; z < m + nz; z += blockDim.x) {
+            double v[SOL_ROWS];
+            sol_points<R>(sol, code, f, tab, md, g, x, y, z, tabs, v);
+#pragma unroll
+            for (int k = 0; k < SOL_ROWS; ++k)
+                if (ok[k]) C::st(F.f[f][0] + g.at(x[k], y[k], z), (R)v[k]);
+        }
     }
 }
 
-// ------------------------------------------------------------------ L2 norms
 // sum over the interior of (F - F_exact(t_n))^2 in double
 // (StaggeredGrid.converge_test, opesci/staggeredgrid.py:892-945); wavefront
 // reduction by shuffles, one partial per block.
@@ -85,23 +155,31 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 template <typename R, typename S>
-__global__ __launch_bounds__(256) void l2_kernel(Geom g, FieldPtrs<S> F, MediaPtrs<R> md,
-                                                 const OpesciExprOp* __restrict__ sol, SolTable tab, int level,
-                                                 double* __restrict__ partial) {
+__global__ __launch_bounds__(256) void l2_kernel(Geom g, FieldPtrs<S> F, MediaPtrs<R> md, SolProg sol,
+                                                 const OpesciExprOp* __restrict__ code, SolTable tab, int level,
+                                                 double* __restrict__ partial, int lds_cap) {
     using C = Conv<S, R>;
     __shared__ double red[4];
     const int f = blockIdx.y;
     const int m = g.m;
     const int nx = g.nx - 2 * m - stag(f, 0), ny = g.ny - 2 * m - stag(f, 1), nz = g.nz - 2 * m - stag(f, 2);
-    const int64_t n = (int64_t)nx * ny * nz;
+    const int rows = nx * ny;
+    const double* tabs = stage_tables(sol, f, lds_cap);
     double acc = 0.0;
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
-        const int z = m + (int)(p % nz), y = m + (int)((p / nz) % ny), x = m + (int)(p / ((int64_t)nz * ny));
-        const double c[3] = {coord_of<R>(tab.dx[0], x, m, stag(f, 0)), coord_of<R>(tab.dx[1], y, m, stag(f, 1)),
-                             coord_of<R>(tab.dx[2], z, m, stag(f, 2))};
-        const double e = (double)C::ld(F.f[f][level] + g.at(x, y, z)) -
-                         vm_eval<R>(sol + tab.off[f], tab.len[f], c, tab.t[f], md, g, x, y, z);
-        acc += e * e;
+    for (int r0 = blockIdx.x * SOL_ROWS; r0 < rows; r0 += gridDim.x * SOL_ROWS) {
+        int x[SOL_ROWS], y[SOL_ROWS];
+        bool ok[SOL_ROWS];
+        sol_rows(r0, rows, ny, m, x, y, ok);
+        for (int z = m + (int)threadIdx.x; z < m + nz; z += blockDim.x) {
+            double v[SOL_ROWS];
+            sol_points<R>(sol, code, f, tab, md, g, x, y, z, tabs, v);
+#pragma unroll
+            for (int k = 0; k < SOL_ROWS; ++k)
+                if (ok[k]) {
+                    const double e = (double)C::ld(F.f[f][level] + g.at(x[k], y[k], z)) - v[k];
+                    acc += e * e;
+                }
+        }
     }
     acc = wave_sum(acc);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
@@ -271,56 +349,75 @@ hipError_t launch_bc(const Geom& g, int real_bytes, int storage, void* fields, c
                         (bc_t<float, bf16_t>(g, fields, d_desc, pool, lvl, t0, t1, s)));
 }
 
-static SolTable make_tab(const int* off, const int* len, const double* dx) {
+static SolTable make_tab(const double* dx) {
     SolTable t;
-    for (int f = 0; f < NF; ++f) {
-        t.off[f] = off[f];
-        t.len[f] = len[f];
-        t.t[f] = 0.0;
-    }
+    for (int f = 0; f < NF; ++f) t.t[f] = 0.0;
     for (int a = 0; a < 3; ++a) t.dx[a] = dx[a];
     return t;
 }
 
+// doubles of dynamic LDS for the point kernels: the largest field's tables
+// when within SOL_LDS_CAP (fields with larger tables read them in place)
+static int sol_lds(const SolProg& sol) {
+    int n = 0;
+    for (int f = 0; f < NF; ++f)
+        if (sol.tab_len[f] <= SOL_LDS_CAP) n = std::max(n, sol.tab_len[f]);
+    return n;
+}
+
+// the tables for this launch's times (stream-ordered before the point kernel)
+template <typename R>
+static hipError_t sol_tables_t(const Geom& g, const void* media, const SolProg& sol, const SolTable& tab,
+                               hipStream_t s) {
+    if (sol.nslots <= 0) return hipSuccess;
+    hipLaunchKernelGGL((sol_table_kernel<R>), dim3((sol.maxdim + 255) / 256, sol.nslots), dim3(256), 0, s, g,
+                       *static_cast<const MediaPtrs<R>*>(media), sol, tab);
+    return hipGetLastError();
+}
+
 template <typename R, typename S>
-static hipError_t init_t(const Geom& g, void* fields, const void* media, const OpesciExprOp* sol, SolTable tab,
+static hipError_t init_t(const Geom& g, void* fields, const void* media, const SolProg& sol, SolTable tab,
                          hipStream_t s) {
-    hipLaunchKernelGGL((initial_kernel<R, S>), dim3(512, NF), dim3(256), 0, s, g,
-                       *static_cast<const FieldPtrs<S>*>(fields), *static_cast<const MediaPtrs<R>*>(media), sol, tab);
+    if (hipError_t e = sol_tables_t<R>(g, media, sol, tab, s)) return e;
+    const int cap = sol_lds(sol);
+    hipLaunchKernelGGL((initial_kernel<R, S>), dim3(512, NF), dim3(256), cap * sizeof(double), s, g,
+                       *static_cast<const FieldPtrs<S>*>(fields), *static_cast<const MediaPtrs<R>*>(media), sol,
+                       sol.code, tab, cap);
     return hipGetLastError();
 }
 
 hipError_t launch_initial(const Geom& g, int real_bytes, int storage, void* fields, const void* media,
-                          const OpesciExprOp* d_sol, const int* sol_off, const int* sol_len, const double* spacing,
-                          double dt, hipStream_t s) {
-    SolTable tab = make_tab(sol_off, sol_len, spacing);
+                          const SolProg& sol, const double* spacing, double dt, hipStream_t s) {
+    SolTable tab = make_tab(spacing);
     for (int f = 0; f < NF; ++f) tab.t[f] = time_stag(f) ? dt / 2.0 : 0.0;
-    return OPESCI_TYPED(real_bytes, storage, (init_t<float, float>(g, fields, media, d_sol, tab, s)),
-                        (init_t<double, double>(g, fields, media, d_sol, tab, s)),
-                        (init_t<float, _Float16>(g, fields, media, d_sol, tab, s)),
-                        (init_t<float, bf16_t>(g, fields, media, d_sol, tab, s)));
+    return OPESCI_TYPED(real_bytes, storage, (init_t<float, float>(g, fields, media, sol, tab, s)),
+                        (init_t<double, double>(g, fields, media, sol, tab, s)),
+                        (init_t<float, _Float16>(g, fields, media, sol, tab, s)),
+                        (init_t<float, bf16_t>(g, fields, media, sol, tab, s)));
 }
 
 template <typename R, typename S>
-static hipError_t l2_t(const Geom& g, void* fields, const void* media, const OpesciExprOp* sol, SolTable tab,
+static hipError_t l2_t(const Geom& g, void* fields, const void* media, const SolProg& sol, SolTable tab,
                        int level, double* partial, int nblk, hipStream_t s) {
-    hipLaunchKernelGGL((l2_kernel<R, S>), dim3(nblk, NF), dim3(256), 0, s, g, *static_cast<const FieldPtrs<S>*>(fields),
-                       *static_cast<const MediaPtrs<R>*>(media), sol, tab, level, partial);
+    if (hipError_t e = sol_tables_t<R>(g, media, sol, tab, s)) return e;
+    const int cap = sol_lds(sol);
+    hipLaunchKernelGGL((l2_kernel<R, S>), dim3(nblk, NF), dim3(256), cap * sizeof(double), s, g,
+                       *static_cast<const FieldPtrs<S>*>(fields), *static_cast<const MediaPtrs<R>*>(media), sol,
+                       sol.code, tab, level, partial, cap);
     return hipGetLastError();
 }
 
 hipError_t launch_l2(const Geom& g, int real_bytes, int storage, void* fields, const void* media,
-                     const OpesciExprOp* d_sol, const int* sol_off, const int* sol_len, const double* spacing,
-                     int level, const double t_field[NF], double* d_partial, int partial_cap, double* out_host,
-                     hipStream_t s) {
-    SolTable tab = make_tab(sol_off, sol_len, spacing);
+                     const SolProg& sol, const double* spacing, int level, const double t_field[NF],
+                     double* d_partial, int partial_cap, double* out_host, hipStream_t s) {
+    SolTable tab = make_tab(spacing);
     for (int f = 0; f < NF; ++f) tab.t[f] = t_field[f];
     const int nblk = partial_cap / NF;
     hipError_t e = OPESCI_TYPED(real_bytes, storage,
-                                (l2_t<float, float>(g, fields, media, d_sol, tab, level, d_partial, nblk, s)),
-                                (l2_t<double, double>(g, fields, media, d_sol, tab, level, d_partial, nblk, s)),
-                                (l2_t<float, _Float16>(g, fields, media, d_sol, tab, level, d_partial, nblk, s)),
-                                (l2_t<float, bf16_t>(g, fields, media, d_sol, tab, level, d_partial, nblk, s)));
+                                (l2_t<float, float>(g, fields, media, sol, tab, level, d_partial, nblk, s)),
+                                (l2_t<double, double>(g, fields, media, sol, tab, level, d_partial, nblk, s)),
+                                (l2_t<float, _Float16>(g, fields, media, sol, tab, level, d_partial, nblk, s)),
+                                (l2_t<float, bf16_t>(g, fields, media, sol, tab, level, d_partial, nblk, s)));
     if (e != hipSuccess) return e;
     std::vector<double> h((size_t)NF * nblk);
     e = hipMemcpyAsync(h.data(), d_partial, h.size() * sizeof(double), hipMemcpyDeviceToHost, s);
@@ -380,19 +477,18 @@ static hipError_t l2_scratch_of_current_device(double** out) {
 int l2_noalloc_partials() { return L2_BLOCKS; }
 
 hipError_t launch_l2_noalloc(const Geom& g, int real_bytes, int storage, void* fields, const void* media,
-                             const OpesciExprOp* d_sol, const int* sol_off, const int* sol_len,
-                             const double* spacing, int level, const double t_field[NF], double* d_out,
-                             hipStream_t s) {
+                             const SolProg& sol, const double* spacing, int level, const double t_field[NF],
+                             double* d_out, hipStream_t s) {
     double* partial = nullptr;
     hipError_t e = l2_scratch_of_current_device(&partial);
     if (e != hipSuccess) return e;
-    SolTable tab = make_tab(sol_off, sol_len, spacing);
+    SolTable tab = make_tab(spacing);
     for (int f = 0; f < NF; ++f) tab.t[f] = t_field[f];
     e = OPESCI_TYPED(real_bytes, storage,
-                     (l2_t<float, float>(g, fields, media, d_sol, tab, level, partial, L2_BLOCKS, s)),
-                     (l2_t<double, double>(g, fields, media, d_sol, tab, level, partial, L2_BLOCKS, s)),
-                     (l2_t<float, _Float16>(g, fields, media, d_sol, tab, level, partial, L2_BLOCKS, s)),
-                     (l2_t<float, bf16_t>(g, fields, media, d_sol, tab, level, partial, L2_BLOCKS, s)));
+                     (l2_t<float, float>(g, fields, media, sol, tab, level, partial, L2_BLOCKS, s)),
+                     (l2_t<double, double>(g, fields, media, sol, tab, level, partial, L2_BLOCKS, s)),
+                     (l2_t<float, _Float16>(g, fields, media, sol, tab, level, partial, L2_BLOCKS, s)),
+                     (l2_t<float, bf16_t>(g, fields, media, sol, tab, level, partial, L2_BLOCKS, s)));
     if (e != hipSuccess) return e;
     const double vol = spacing[0] * spacing[1] * spacing[2];
     hipLaunchKernelGGL(l2_finish_kernel, dim3(1), dim3(64), 0, s, (const double*)partial, L2_BLOCKS, vol, d_out);

@@ -10,9 +10,12 @@ namespace opesci {
 // ------------------------------------------------------------------ bytecode
 // Stack machine over OpesciExprOp (opcodes in include/opesci_hip.h),
 // evaluated in double.  MEDIA reads media a[0] at anchor + (a[1],a[2],a[3]).
+// OPX_TABLE (library-internal, in programs the library rewrites) reads
+// tabs[a[0] + anchor index along axis a[1]].
 template <typename R>
 __device__ double vm_eval(const OpesciExprOp* code, int n, const double coord[3], double t,
-                          const MediaPtrs<R>& md, const Geom& g, int ax, int ay, int az) {
+                          const MediaPtrs<R>& md, const Geom& g, int ax, int ay, int az,
+                          const double* __restrict__ tabs = nullptr) {
     double st[16];
     int sp = 0;
     for (int i = 0; i < n; ++i) {
@@ -38,10 +41,124 @@ __device__ double vm_eval(const OpesciExprOp* code, int n, const double coord[3]
             case OPESCI_OP_EXP: st[sp - 1] = exp(st[sp - 1]); break;
             case OPESCI_OP_LOG: st[sp - 1] = log(st[sp - 1]); break;
             case OPESCI_OP_ROUNDF: st[sp - 1] = (double)(float)st[sp - 1]; break;
+            case OPX_TABLE: st[sp++] = tabs[o.a[0] + (o.a[1] == 0 ? ax : (o.a[1] == 1 ? ay : az))]; break;
             default: break;
         }
     }
     return sp > 0 ? st[0] : 0.0;
+}
+
+// The same machine with the top two stack entries in fixed registers and
+// only deeper entries in the indexed array (the point kernels' fallback for
+// programs deeper than vm_eval_k below takes): the same double operations
+// in the same order as vm_eval.
+template <typename R>
+__device__ double vm_eval_tos(const OpesciExprOp* code, int n, const double coord[3], double t,
+                              const MediaPtrs<R>& md, const Geom& g, int ax, int ay, int az,
+                              const double* __restrict__ tabs) {
+    double t0 = 0.0, t1 = 0.0, rest[14];
+    int depth = 0;
+    for (int i = 0; i < n; ++i) {
+        const OpesciExprOp o = code[i];
+        const int op = o.op;
+        if (op <= OPESCI_OP_MEDIA || op == OPX_TABLE) {
+            double v;
+            if (op == OPX_TABLE)
+                v = tabs[o.a[0] + (o.a[1] == 0 ? ax : (o.a[1] == 1 ? ay : az))];
+            else if (op == OPESCI_OP_CONST)
+                v = o.val;
+            else if (op == OPESCI_OP_COORD)
+                v = coord[o.a[0]];
+            else if (op == OPESCI_OP_TIME)
+                v = t;
+            else {
+                const R* p = md.m[o.a[0]];
+                v = p ? (double)p[g.at(ax + o.a[1], ay + o.a[2], az + o.a[3])] : 0.0;
+            }
+            if (depth >= 2) rest[depth - 2] = t1;
+            t1 = t0;
+            t0 = v;
+            ++depth;
+        } else if (op <= OPESCI_OP_DIV) {
+            t0 = op == OPESCI_OP_ADD ? t1 + t0 : (op == OPESCI_OP_SUB ? t1 - t0 : (op == OPESCI_OP_MUL ? t1 * t0 : t1 / t0));
+            --depth;
+            t1 = depth >= 2 ? rest[depth - 2] : 0.0;
+        } else {
+            switch (op) {
+                case OPESCI_OP_NEG: t0 = -t0; break;
+                case OPESCI_OP_POWC: t0 = pow(t0, o.val); break;
+                case OPESCI_OP_SQRT: t0 = sqrt(t0); break;
+                case OPESCI_OP_SIN: t0 = sin(t0); break;
+                case OPESCI_OP_COS: t0 = cos(t0); break;
+                case OPESCI_OP_EXP: t0 = exp(t0); break;
+                case OPESCI_OP_LOG: t0 = log(t0); break;
+                case OPESCI_OP_ROUNDF: t0 = (double)(float)t0; break;
+                default: break;
+            }
+        }
+    }
+    return depth > 0 ? (depth == 1 ? t0 : rest[0]) : 0.0;
+}
+
+// K points of one program at once (the same ops, decoded once for all K),
+// for programs whose stack never exceeds depth 3: t0/t1/t2 are the top
+// three entries of each point's stack.  Per point, the same double
+// operations in the same order as vm_eval.
+template <typename R, int K>
+__device__ __forceinline__ void vm_eval_k(const OpesciExprOp* __restrict__ code, int n, const double cx[K],
+                                          const double cy[K], double cz, double t, const MediaPtrs<R>& md,
+                                          const Geom& g, const int ax[K], const int ay[K], int az,
+                                          const double* __restrict__ tabs, double out[K]) {
+    double t0[K], t1[K], t2[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) t0[k] = t1[k] = t2[k] = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const OpesciExprOp o = code[i];
+        const int op = o.op;
+        if (op <= OPESCI_OP_MEDIA || op == OPX_TABLE) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                double v;
+                if (op == OPX_TABLE)
+                    v = tabs[o.a[0] + (o.a[1] == 0 ? ax[k] : (o.a[1] == 1 ? ay[k] : az))];
+                else if (op == OPESCI_OP_CONST)
+                    v = o.val;
+                else if (op == OPESCI_OP_COORD)
+                    v = o.a[0] == 0 ? cx[k] : (o.a[0] == 1 ? cy[k] : cz);
+                else if (op == OPESCI_OP_TIME)
+                    v = t;
+                else {
+                    const R* p = md.m[o.a[0]];
+                    v = p ? (double)p[g.at(ax[k] + o.a[1], ay[k] + o.a[2], az + o.a[3])] : 0.0;
+                }
+                t2[k] = t1[k];
+                t1[k] = t0[k];
+                t0[k] = v;
+            }
+        } else {
+            switch (op) {
+#define OPESCI_K_LOOP(expr) _Pragma("unroll") for (int k = 0; k < K; ++k) t0[k] = (expr); break
+#define OPESCI_K_POP(expr) _Pragma("unroll") for (int k = 0; k < K; ++k) { t0[k] = (expr); t1[k] = t2[k]; } break
+                case OPESCI_OP_ADD: OPESCI_K_POP(t1[k] + t0[k]);
+                case OPESCI_OP_SUB: OPESCI_K_POP(t1[k] - t0[k]);
+                case OPESCI_OP_MUL: OPESCI_K_POP(t1[k] * t0[k]);
+                case OPESCI_OP_DIV: OPESCI_K_POP(t1[k] / t0[k]);
+                case OPESCI_OP_NEG: OPESCI_K_LOOP(-t0[k]);
+                case OPESCI_OP_POWC: OPESCI_K_LOOP(pow(t0[k], o.val));
+                case OPESCI_OP_SQRT: OPESCI_K_LOOP(sqrt(t0[k]));
+                case OPESCI_OP_SIN: OPESCI_K_LOOP(sin(t0[k]));
+                case OPESCI_OP_COS: OPESCI_K_LOOP(cos(t0[k]));
+                case OPESCI_OP_EXP: OPESCI_K_LOOP(exp(t0[k]));
+                case OPESCI_OP_LOG: OPESCI_K_LOOP(log(t0[k]));
+                case OPESCI_OP_ROUNDF: OPESCI_K_LOOP((double)(float)t0[k]);
+#undef OPESCI_K_LOOP
+#undef OPESCI_K_POP
+                default: break;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) out[k] = t0[k];
 }
 
 // One boundary statement at point p of its loop: dst = sum_k coef_k * src_k,

@@ -359,22 +359,39 @@ hipError_t launch_copy_cells(int real_bytes, void* dst, int td, void* src, int t
 hipError_t launch_src_cells(const Geom& g, int real_bytes, void* fields, int t, int x, int y, int z, void* buf,
                             int mode, hipStream_t s);
 
+// Analytic solution programs, split: every maximal subtree of a field's
+// program that reads at most one coordinate and no media is evaluated once
+// per index along that axis (slot: code [prog, prog + len) into tab[tab +
+// i], i < dim(axis)) and the per-point program [off[f], off[f] + len[f])
+// reads it back through OPX_TABLE.  The same double operations run in the
+// same order, so the values equal per-point evaluation bit for bit.
+constexpr int OPX_TABLE = 64;
+struct SolSlot {
+    int32_t field, axis, prog, len, tab;
+};
+struct SolProg {
+    const OpesciExprOp* code = nullptr;  // per-point programs, then the slots' programs
+    int off[NF] = {}, len[NF] = {};
+    int tab_base[NF] = {}, tab_len[NF] = {};  // field f's tables: tab[tab_base[f] ..], OPX_TABLE offsets relative to it
+    int shallow[NF] = {};                      // field f's point program never exceeds stack depth 3
+    const SolSlot* slots = nullptr;
+    int nslots = 0;
+    int maxdim = 0;
+    double* tab = nullptr;               // the tables (device, written by every launch below)
+};
 hipError_t launch_initial(const Geom& g, int real_bytes, int storage, void* fields, const void* media,
-                          const OpesciExprOp* d_sol, const int* sol_off, const int* sol_len, const double* spacing,
-                          double dt, hipStream_t s);
+                          const SolProg& sol, const double* spacing, double dt, hipStream_t s);
 hipError_t launch_l2(const Geom& g, int real_bytes, int storage, void* fields, const void* media,
-                     const OpesciExprOp* d_sol, const int* sol_off, const int* sol_len, const double* spacing,
-                     int level, const double t_field[NF], double* d_partial, int partial_cap, double* out_host,
-                     hipStream_t s);
+                     const SolProg& sol, const double* spacing, int level, const double t_field[NF],
+                     double* d_partial, int partial_cap, double* out_host, hipStream_t s);
 // the same L2 with its partial sums in a fixed library-owned device array
 // and the norms written to d_out (9 doubles, device memory), stream-ordered,
 // no allocation and no host synchronisation; the block count equals
 // launch_l2's, so the two agree
 int l2_noalloc_partials();
 hipError_t launch_l2_noalloc(const Geom& g, int real_bytes, int storage, void* fields, const void* media,
-                             const OpesciExprOp* d_sol, const int* sol_off, const int* sol_len,
-                             const double* spacing, int level, const double t_field[NF], double* d_out,
-                             hipStream_t s);
+                             const SolProg& sol, const double* spacing, int level, const double t_field[NF],
+                             double* d_out, hipStream_t s);
 hipError_t launch_media(const Geom& g, int real_bytes, const float* d_rho, const float* d_vp, const float* d_vs,
                         void* const* d_media, hipStream_t s);
 hipError_t launch_pack(const Geom& g, int real_bytes, int storage, const void* dense, void* field, hipStream_t s);

@@ -77,8 +77,10 @@ struct OpesciContext {
     void* d_srclog = nullptr;         // Txx/Tyy/Tzz at the source before and after injection, per step
     float* d_obs = nullptr;
     OpesciExprOp* d_code = nullptr;
-    OpesciExprOp* d_sol = nullptr;
-    int sol_off[NF], sol_len[NF];
+    OpesciExprOp* d_sol = nullptr;    // the analytic programs, split (SolProg)
+    SolSlot* d_solslots = nullptr;
+    double* d_soltab = nullptr;
+    SolProg sol;
     double* d_partial = nullptr;
     int partial_cap = 0;
     void* d_stage = nullptr;          // dense staging buffer (one field level, real_t)
@@ -349,6 +351,88 @@ int build_program(OpesciContext* c, const OpesciBcTerm* terms, const OpesciBcDTe
     return 0;
 }
 
+// Split the analytic programs (SolProg): parse each field's postfix program
+// into subtrees (a subtree is a contiguous range of ops ending at its root),
+// mark which coordinates and whether media each reads, and replace every
+// maximal subtree that reads at most one coordinate and no media (and is not
+// a lone constant) by a table read along that coordinate (axis 0 for none).
+// OPESCI_FLAG_POINTWISE_SOLUTION keeps the programs whole.
+int build_solution(OpesciContext* c, const OpesciModelDesc* d) {
+    std::vector<OpesciExprOp> pc, tc;    // point programs, slot programs
+    std::vector<SolSlot> slots;
+    const int dims[3] = {d->dims[0], d->dims[1], d->dims[2]};
+    int64_t ntab = 0;
+    const bool split = !(d->flags & OPESCI_FLAG_POINTWISE_SOLUTION);
+    for (int f = 0; f < NF; ++f) {
+        const OpesciExprOp* ops = d->sol_code + d->sol_off[f];
+        const int n = d->sol_len[f];
+        std::vector<int> start(n), mask(n), stk;
+        for (int i = 0; i < n; ++i) {
+            const int op = ops[i].op;
+            const int arity = op <= OPESCI_OP_MEDIA ? 0 : (op <= OPESCI_OP_DIV ? 2 : 1);
+            int m = op == OPESCI_OP_COORD ? 1 << ops[i].a[0] : (op == OPESCI_OP_MEDIA ? 8 : 0), st = i;
+            for (int k = 0; k < arity; ++k) {
+                const int r = stk.back();
+                stk.pop_back();
+                m |= mask[r];
+                st = std::min(st, start[r]);
+            }
+            start[i] = st;
+            mask[i] = m;
+            stk.push_back(i);
+        }
+        c->sol.off[f] = (int)pc.size();
+        c->sol.tab_base[f] = (int)ntab;
+        auto emit = [&](auto&& self, int i) -> void {
+            const int op = ops[i].op, m = mask[i];
+            const bool one_axis = !(m & 8) && (m & (m - 1)) == 0;
+            if (split && one_axis && (i > start[i] || op == OPESCI_OP_COORD)) {
+                const int axis = m == 0 ? 0 : (m == 1 ? 0 : (m == 2 ? 1 : 2));
+                slots.push_back(SolSlot{f, axis, (int32_t)tc.size(), i - start[i] + 1, (int32_t)ntab});
+                tc.insert(tc.end(), ops + start[i], ops + i + 1);
+                OpesciExprOp t;
+                memset(&t, 0, sizeof(t));
+                t.op = OPX_TABLE;
+                t.a[0] = (int32_t)(ntab - c->sol.tab_base[f]);
+                t.a[1] = axis;
+                pc.push_back(t);
+                ntab += dims[axis];
+                return;
+            }
+            const int arity = op <= OPESCI_OP_MEDIA ? 0 : (op <= OPESCI_OP_DIV ? 2 : 1);
+            if (arity == 2) {
+                self(self, start[i - 1] - 1);   // left operand ends just before the right one starts
+                self(self, i - 1);
+            } else if (arity == 1) {
+                self(self, i - 1);
+            }
+            pc.push_back(ops[i]);
+        };
+        if (n > 0) emit(emit, n - 1);
+        if (ntab >= ((int64_t)1 << 31)) return fail(OPESCI_EUNSUPPORTED, "analytic-solution tables too large");
+        c->sol.len[f] = (int)pc.size() - c->sol.off[f];
+        c->sol.tab_len[f] = (int)ntab - c->sol.tab_base[f];
+        int depth = 0, maxd = 0;
+        for (int i = c->sol.off[f]; i < (int)pc.size(); ++i) {
+            const int op = pc[i].op;
+            depth += (op <= OPESCI_OP_MEDIA || op == OPX_TABLE) ? 1 : (op <= OPESCI_OP_DIV ? -1 : 0);
+            maxd = std::max(maxd, depth);
+        }
+        c->sol.shallow[f] = maxd <= 3;
+    }
+    for (SolSlot& s : slots) s.prog += (int32_t)pc.size();
+    pc.insert(pc.end(), tc.begin(), tc.end());
+    if (int e = upload(&c->d_sol, pc.data(), pc.size(), c->stream)) return e;
+    if (int e = upload(&c->d_solslots, slots.data(), slots.size(), c->stream)) return e;
+    if (ntab > 0) OPESCI_HIP_TRY(hipMalloc((void**)&c->d_soltab, (size_t)ntab * sizeof(double)));
+    c->sol.code = c->d_sol;
+    c->sol.slots = c->d_solslots;
+    c->sol.nslots = (int)slots.size();
+    c->sol.maxdim = std::max(dims[0], std::max(dims[1], dims[2]));
+    c->sol.tab = c->d_soltab;
+    return 0;
+}
+
 int alloc_zero(OpesciContext* c, void** p, size_t bytes) {
     OPESCI_HIP_TRY(hipMalloc(p, bytes));
     OPESCI_HIP_TRY(launch_zero(*p, bytes, c->stream));
@@ -591,7 +675,7 @@ void destroy(OpesciContext* c) {
             if (c->base[f][l]) (void)hipFree(c->base[f][l]);
     for (int i = 0; i < OPESCI_M_COUNT; ++i)
         if (c->mbase[i]) (void)hipFree(c->mbase[i]);
-    void* ptrs[] = {c->d_desc,  c->d_code,    c->d_sol,  c->d_partial, c->d_stage, c->d_rec,
+    void* ptrs[] = {c->d_desc,  c->d_code,    c->d_sol,  c->d_solslots, c->d_soltab, c->d_partial, c->d_stage, c->d_rec,
                     c->d_trace, c->d_wavelet, c->d_step, c->d_pool,    c->d_dterms, c->d_gpool,
                     c->d_log,   c->d_ckpt,    c->d_seg,  c->d_obs,     c->d_srclog, c->d_tcells,
                     c->d_vcells, c->d_scells, c->d_zcoef, c->d_gather, c->d_abuf};
@@ -685,11 +769,7 @@ int create(const OpesciModelDesc* d, OpesciContext** out) {
                                               lv.count, lv.maxpts, c->stream));
         OPESCI_HIP_TRY(hipStreamSynchronize(c->stream));
     }
-    if (int e = upload(&c->d_sol, d->sol_code, (size_t)d->nsol, c->stream)) return e;
-    for (int f = 0; f < NF; ++f) {
-        c->sol_off[f] = d->sol_off[f];
-        c->sol_len[f] = d->sol_len[f];
-    }
+    if (int e = build_solution(c.get(), d)) return e;
     c->partial_cap = NF * 512;
     OPESCI_HIP_TRY(hipMalloc((void**)&c->d_partial, c->partial_cap * sizeof(double)));
     c->nrec = d->nrec;
@@ -725,8 +805,8 @@ int ctx_init(OpesciContext* c) {
     c->grad_every = 0;
     for (int f = 0; f < NF; ++f)
         for (int l = 0; l < 2; ++l) OPESCI_HIP_TRY(launch_zero(c->base[f][l], c->field_bytes, c->stream));
-    OPESCI_HIP_TRY(launch_initial(c->g, c->real_bytes, c->storage, c->fptr, c->mptr, c->d_sol, c->sol_off,
-                                  c->sol_len, c->desc.spacing, c->desc.dt, c->stream));
+    OPESCI_HIP_TRY(launch_initial(c->g, c->real_bytes, c->storage, c->fptr, c->mptr, c->sol, c->desc.spacing,
+                                  c->desc.dt, c->stream));
     if (int e = run_bc_phase(c, 0, 0, 0)) return e;
     if (int e = run_bc_phase(c, 1, 0, 0)) return e;
     if (c->d_step) OPESCI_HIP_TRY(launch_zero(c->d_step, 256, c->stream));
@@ -781,8 +861,8 @@ int ctx_l2(OpesciContext* c, double out[NF]) {
     if (!c->initialised) return fail(OPESCI_ESTATE, "no state: call opesci_ctx_init first");
     double t[NF];
     for (int f = 0; f < NF; ++f) t[f] = c->desc.dt * c->steps + (time_stag(f) ? c->desc.dt / 2.0 : 0.0);
-    OPESCI_HIP_TRY(launch_l2(c->g, c->real_bytes, c->storage, c->fptr, c->mptr, c->d_sol, c->sol_off, c->sol_len,
-                             c->desc.spacing, c->steps % 2, t, c->d_partial, c->partial_cap, out, c->stream));
+    OPESCI_HIP_TRY(launch_l2(c->g, c->real_bytes, c->storage, c->fptr, c->mptr, c->sol, c->desc.spacing,
+                             c->steps % 2, t, c->d_partial, c->partial_cap, out, c->stream));
     return 0;
 }
 
@@ -886,8 +966,8 @@ int opesci_ctx_l2_noalloc(OpesciContext* ctx, double* d_out, void* stream) {
     double t[NF];
     for (int f = 0; f < NF; ++f) t[f] = ctx->desc.dt * ctx->steps + (time_stag(f) ? ctx->desc.dt / 2.0 : 0.0);
     const hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
-    OPESCI_HIP_TRY(launch_l2_noalloc(ctx->g, ctx->real_bytes, ctx->storage, ctx->fptr, ctx->mptr, ctx->d_sol,
-                                     ctx->sol_off, ctx->sol_len, ctx->desc.spacing, ctx->steps % 2, t, d_out, s));
+    OPESCI_HIP_TRY(launch_l2_noalloc(ctx->g, ctx->real_bytes, ctx->storage, ctx->fptr, ctx->mptr, ctx->sol,
+                                     ctx->desc.spacing, ctx->steps % 2, t, d_out, s));
     return 0;
 }
 

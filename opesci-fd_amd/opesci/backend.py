@@ -17,6 +17,7 @@ from .compilation import library_path
 
 ABI_VERSION = 2
 FLAG_FUSED_STEP = 1
+FLAG_POINTWISE_SOLUTION = 2
 FIELDS = ('U', 'V', 'W', 'Txx', 'Tyy', 'Tzz', 'Txy', 'Tyz', 'Txz')
 MEDIA_IDS = {'rho': 0, 'vp': 1, 'vs': 2, 'beta': 3, 'lambda': 4, 'mu': 5, 'beta1': 6, 'beta2': 7, 'beta3': 8,
              'mu12': 9, 'mu13': 10, 'mu23': 11}
@@ -109,8 +110,29 @@ def load(path=None):
     return _LIB[path]
 
 
+def _hip_runtimes():
+    """Files of the HIP runtimes mapped into this process."""
+    try:
+        with open('/proc/self/maps') as f:
+            return sorted({ln.split()[-1] for ln in f if 'libamdhip64' in ln})
+    except OSError:
+        return []
+
+
 def _is_device_tensor(a):
-    return hasattr(a, 'data_ptr') and getattr(a, 'is_cuda', False)
+    """A torch device tensor, usable here only when torch and this library
+    share one HIP runtime.  torch's wheel carries its own runtime under the
+    SONAME this library links against: imported first, torch's runtime is
+    the one this library binds to; imported after the library, a second
+    runtime (and HSA runtime) loads beside the first, and their device
+    pointers and streams are not each other's."""
+    if not (hasattr(a, 'data_ptr') and getattr(a, 'is_cuda', False)):
+        return False
+    rts = _hip_runtimes()
+    if len(rts) > 1:
+        raise RuntimeError('two HIP runtimes in this process (%s): import torch before loading the opesci '
+                           'library so that both use one' % ', '.join(rts))
+    return True
 
 
 def _check(rc, what):
@@ -145,7 +167,8 @@ def make_desc(plan):
     d.real_bytes = plan.real_bytes
     d.storage = plan.storage_code
     d.media_arrays = 1 if plan.media_arrays else 0
-    d.flags = FLAG_FUSED_STEP if getattr(plan, 'fuse_step', False) else 0
+    d.flags = ((FLAG_FUSED_STEP if getattr(plan, 'fuse_step', False) else 0) |
+               (FLAG_POINTWISE_SOLUTION if getattr(plan, 'pointwise_solution', False) else 0))
     d.dt = plan.dt
     d.spacing[:] = plan.spacing
     d.media_const[:] = [float(v) for v in plan.media_const]

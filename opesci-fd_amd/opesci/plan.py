@@ -667,6 +667,9 @@ def build_plan(grid):
     # both half steps in one sweep where it applies (off by default: slower
     # than the separate launches on MI355X)
     p.fuse_step = bool(getattr(grid, 'fuse_step', False))
+    # the analytic solution evaluated per point as given, without the
+    # per-axis tables of its one-coordinate parts (same values, slower)
+    p.pointwise_solution = bool(getattr(grid, 'pointwise_solution', False))
     p.use_graph = 1 if grid.use_graph else 0
     p.flops_per_point = None
     p.field_names = tuple(str(f.label) for f in slots)

@@ -3,6 +3,14 @@ import sys
 
 import pytest
 
+try:
+    # torch's wheel carries its own HIP runtime; imported before the library
+    # loads, it is the one runtime of the test process (the library binds to
+    # it), whichever test files run
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "opesci-fd_amd")
 for p in (PKG, os.path.join(ROOT, "oracle"), ROOT):

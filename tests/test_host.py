@@ -5,6 +5,7 @@ import ctypes
 import os
 import re
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -174,6 +175,31 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, n), n
     assert set(names) == set(backend.EXPORTS)
     assert lib.opesci_abi_version() == backend.ABI_VERSION
+
+
+def test_one_hip_runtime_with_torch():
+    """torch imported before the library (tests/conftest.py): the library
+    binds to torch's HIP runtime, so the process maps exactly one; a fresh
+    process that loads the library first and torch after maps two, and the
+    device-tensor paths refuse to run there."""
+    pytest.importorskip('torch')
+    backend.load()
+    assert len(backend._hip_runtimes()) == 1, backend._hip_runtimes()
+    code = ('import sys; sys.path.insert(0, %r)\n'
+            'from opesci import backend\n'
+            'backend.load()\n'
+            'import torch\n'
+            'assert len(backend._hip_runtimes()) == 2, backend._hip_runtimes()\n'
+            'class T:\n'
+            '    is_cuda = True\n'
+            '    def data_ptr(self): return 0\n'
+            'try:\n'
+            '    backend._is_device_tensor(T())\n'
+            'except RuntimeError as e:\n'
+            '    assert "import torch before" in str(e)\n'
+            'else:\n'
+            '    raise AssertionError("no refusal")\n' % os.path.dirname(os.path.dirname(backend.__file__)))
+    subprocess.run([sys.executable, '-c', code], check=True, timeout=300)
 
 
 def test_struct_layout_matches_header(tmp_path):
