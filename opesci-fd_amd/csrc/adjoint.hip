@@ -520,58 +520,33 @@ __global__ __launch_bounds__(256) void adj_bc_gather_kernel(AdjPtrs<R> B, const 
     *q = acc;
 }
 
-// gpool -> media gradients through the derivative programs of each term,
-// first pass: every (point, term, derivative program) value into its slot
-// of the level's buffer (point p of descriptor d: slots boff[d] + p * n ..,
-// n the descriptor's derivative programs).
+// gpool -> media gradients through one derivative program: term k of
+// descriptor `d`, program q, at every point of the statement.  Distinct
+// points reach distinct media cells (the cell is the point's anchor plus the
+// program's fixed offset), so one launch has no conflicting updates; the
+// launches run one after another in (statement, term, program) order, which
+// fixes the order of every cell's additions.
 template <typename R>
-__global__ __launch_bounds__(256) void adj_coef_media_kernel(Geom g, MediaPtrs<R> md, const BcDesc* __restrict__ desc,
+__global__ __launch_bounds__(256) void adj_coef_media_kernel(Geom g, MediaPtrs<R> md, GradPtrs<R> gm,
+                                                             const BcDesc* __restrict__ desc,
                                                              const OpesciExprOp* __restrict__ code,
                                                              const OpesciBcDTerm* __restrict__ dterms,
-                                                             const R* __restrict__ gpool, int first,
-                                                             const int32_t* __restrict__ boff, R* buf) {
-    const int dd = first + blockIdx.y;
-    const BcDesc* D = desc + dd;
+                                                             const R* __restrict__ gpool, int d, int k, int q) {
+    const BcDesc* D = desc + d;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= D->npts) return;
-    int n = 0;
-    for (int k = 0; k < D->nterms; ++k)
-        if (D->t[k].cslot >= 0) n += D->t[k].ndterm;
-    if (n == 0) return;
+    const R gc = gpool[D->cpool + (int64_t)D->t[k].cslot * D->npts + p];
+    if (gc == (R)0) return;
     const int n1 = D->n1, axis = D->axis;
     const int i0 = D->lo0 + p / n1, i1 = D->lo1 + p % n1;
     const int ax = axis == 0 ? 0 : i0;
     const int ay = axis == 0 ? i0 : (axis == 1 ? 0 : i1);
     const int az = axis == 2 ? 0 : i1;
     const double c0[3] = {0, 0, 0};
-    R* out = buf + boff[dd] + (int64_t)p * n;
-    int j = 0;
-    for (int k = 0; k < D->nterms; ++k) {
-        if (D->t[k].cslot < 0) continue;
-        const R gc = gpool[D->cpool + (int64_t)D->t[k].cslot * D->npts + p];
-        for (int d = D->t[k].dterm0; d < D->t[k].dterm0 + D->t[k].ndterm; ++d, ++j) {
-            if (gc == (R)0) {
-                out[j] = (R)0;
-                continue;
-            }
-            const OpesciBcDTerm& T = dterms[d];
-            const double dv = (double)D->t[k].coef * vm_eval<R>(code + T.prog, T.prog_len, c0, 0.0, md, g, ax, ay, az);
-            out[j] = (R)(gc * (R)dv);
-        }
-    }
-}
-
-// Second pass: each media cell sums its slots in (statement, point, term,
-// program) order.
-template <typename R>
-__global__ __launch_bounds__(256) void adj_coef_gather_kernel(GradPtrs<R> gm, AdjGather ga) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ga.ntgt) return;
-    const R* buf = static_cast<const R*>(ga.buf);
-    R* q = gm.m[ga.key[t]] + ga.off[t];
-    R acc = *q;
-    for (int e = ga.start[t]; e < ga.start[t + 1]; ++e) acc += buf[ga.src[e]];
-    *q = acc;
+    const OpesciBcDTerm& T = dterms[q];
+    const double dv = (double)D->t[k].coef * vm_eval<R>(code + T.prog, T.prog_len, c0, 0.0, md, g, ax, ay, az);
+    R* out = gm.m[T.media] + g.at(ax + T.off[0], ay + T.off[1], az + T.off[2]);
+    *out += (R)(gc * (R)dv);
 }
 
 // Receiver misfit residual: bar U,V,W(t1) += field - observed, one thread
@@ -831,23 +806,20 @@ hipError_t launch_adj_bc(const Geom& g, int real_bytes, void* fields, void* bar,
 
 template <typename R>
 static hipError_t adj_coef_t(const Geom& g, const void* media, void* grad, const BcDesc* desc,
-                             const OpesciExprOp* code, const OpesciBcDTerm* dterms, const void* gpool,
-                             const BcLevel& lv, const AdjGather& ga, hipStream_t s) {
-    if (lv.count <= 0 || lv.maxpts <= 0 || ga.ntgt <= 0) return hipSuccess;
-    dim3 grid((lv.maxpts + 255) / 256, lv.count);
-    hipLaunchKernelGGL((adj_coef_media_kernel<R>), grid, dim3(256), 0, s, g, *static_cast<const MediaPtrs<R>*>(media),
-                       desc, code, dterms, static_cast<const R*>(gpool), lv.first, ga.boff, static_cast<R*>(ga.buf));
-    if (hipError_t e = hipGetLastError()) return e;
-    hipLaunchKernelGGL((adj_coef_gather_kernel<R>), dim3((ga.ntgt + 255) / 256), dim3(256), 0, s,
-                       *static_cast<const GradPtrs<R>*>(grad), ga);
+                             const OpesciExprOp* code, const OpesciBcDTerm* dterms, const void* gpool, int d,
+                             int npts, int k, int q, hipStream_t s) {
+    if (npts <= 0) return hipSuccess;
+    hipLaunchKernelGGL((adj_coef_media_kernel<R>), dim3((npts + 255) / 256), dim3(256), 0, s, g,
+                       *static_cast<const MediaPtrs<R>*>(media), *static_cast<const GradPtrs<R>*>(grad), desc, code,
+                       dterms, static_cast<const R*>(gpool), d, k, q);
     return hipGetLastError();
 }
 
 hipError_t launch_adj_coef_media(const Geom& g, int real_bytes, const void* media, void* grad, const BcDesc* desc,
-                                 const OpesciExprOp* code, const OpesciBcDTerm* dterms, const void* gpool,
-                                 const BcLevel& lv, const AdjGather& ga, hipStream_t s) {
-    return real_bytes == 8 ? adj_coef_t<double>(g, media, grad, desc, code, dterms, gpool, lv, ga, s)
-                           : adj_coef_t<float>(g, media, grad, desc, code, dterms, gpool, lv, ga, s);
+                                 const OpesciExprOp* code, const OpesciBcDTerm* dterms, const void* gpool, int d,
+                                 int npts, int k, int q, hipStream_t s) {
+    return real_bytes == 8 ? adj_coef_t<double>(g, media, grad, desc, code, dterms, gpool, d, npts, k, q, s)
+                           : adj_coef_t<float>(g, media, grad, desc, code, dterms, gpool, d, npts, k, q, s);
 }
 
 hipError_t launch_adj_receivers(const Geom& g, int real_bytes, void* fields, void* bar, int t1, int ngroups,

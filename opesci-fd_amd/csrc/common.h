@@ -325,26 +325,28 @@ hipError_t launch_adj_velocity(const Geom& g, int real_bytes, void* fields, void
                                const InteriorLaunch& cfg, int t0, int t1, hipStream_t s);
 hipError_t launch_bc_logged(const Geom& g, int real_bytes, void* fields, const BcDesc* desc, const void* pool,
                             const BcLevel& lv, void* log, int t0, int t1, int undo, hipStream_t s);
-// Fixed-order accumulation of one level's transposes (adjoint fields) or
-// coefficient gradients (media gradients): targets in ascending cell order,
-// each summing its contributions in (statement, point, term) order from a
-// buffer the level's points fill first, so results do not depend on
-// scheduling.  Device arrays; start[t]..start[t+1] are target t's entries.
+// Fixed-order accumulation of one level's transposes: targets in ascending
+// cell order, each summing its contributions in (statement, point, term)
+// order from a buffer the level's points fill first, so results do not
+// depend on scheduling.  Device arrays; start[t]..start[t+1] are target t's
+// entries.
 struct AdjGather {
     int ntgt = 0;
     const int32_t* start = nullptr;   // ntgt + 1
-    const int32_t* key = nullptr;     // target field*2 + level, or media id
+    const int32_t* key = nullptr;     // target field*2 + level
     const int64_t* off = nullptr;     // target linear offset
     const int32_t* src = nullptr;     // per entry: slot in the buffer
-    const int32_t* cinfo = nullptr;   // per entry (transposes): pool index, or -(1 + descriptor*BC_MAXT + term)
+    const int32_t* cinfo = nullptr;   // per entry: pool index, or -(1 + descriptor*BC_MAXT + term)
     const int32_t* boff = nullptr;    // per descriptor: its first buffer slot
     void* buf = nullptr;              // the level's slots (compute type)
 };
 hipError_t launch_adj_bc(const Geom& g, int real_bytes, void* fields, void* bar, const BcDesc* desc, const void* pool,
                          void* gpool, const BcLevel& lv, const AdjGather& ga, int t0, int t1, hipStream_t s);
+// media gradients through derivative program q of term k of descriptor d
+// (one launch per program, in a fixed order)
 hipError_t launch_adj_coef_media(const Geom& g, int real_bytes, const void* media, void* grad, const BcDesc* desc,
-                                 const OpesciExprOp* code, const OpesciBcDTerm* dterms, const void* gpool,
-                                 const BcLevel& lv, const AdjGather& ga, hipStream_t s);
+                                 const OpesciExprOp* code, const OpesciBcDTerm* dterms, const void* gpool, int d,
+                                 int npts, int k, int q, hipStream_t s);
 // receiver residuals, the receivers of one cell (group g: order[start[g] ..
 // start[g+1])) summed in receiver order
 hipError_t launch_adj_receivers(const Geom& g, int real_bytes, void* fields, void* bar, int t1, int ngroups,

@@ -103,11 +103,11 @@ struct OpesciContext {
     int grad_every = 0;             // checkpoint interval of the last forward pass (0: none pending)
     bool grad_keep_t0 = false;
     ZCoef* d_zcoef = nullptr;       // their coefficient block (device)
-    // adjoint: fixed-order gathers per boundary level (transposes of phases
-    // 2-4, media maps of every phase) and the receivers grouped by cell
+    // adjoint: fixed-order gathers of the boundary transposes per level
+    // (phases 2-4) and the receivers grouped by cell
     std::vector<OpesciBcDTerm> h_dterms;
     std::vector<int32_t> h_rec;
-    std::vector<AdjGather> gat_bc[5], gat_md[5];
+    std::vector<AdjGather> gat_bc[5];
     char* d_gather = nullptr;       // their index arrays, one allocation
     void* d_abuf = nullptr;         // the per-level slot buffer (compute type)
     int nrgroups = 0;
@@ -1137,11 +1137,10 @@ int opesci_ctx_configure(OpesciContext* ctx, int32_t kernel_variant, int32_t til
 // ---------------------------------------------------------------- adjoint
 // Index arrays of the fixed-order gathers (AdjGather): per level, every
 // scattered addition of the transposes (target: the source cell of a term,
-// value: coefficient x the point's adjoint value) and of the media maps
-// (target: the media cell of a derivative program, value: its slot),
-// generated in (statement, point, term[, program]) order and stably sorted by
-// target, so each target's sum runs in that order on every call.  Receivers
-// sharing a cell are grouped the same way, in receiver order.
+// value: coefficient x the point's adjoint value), generated in (statement,
+// point, term) order and stably sorted by target, so each target's sum runs
+// in that order on every call.  Receivers sharing a cell are grouped the
+// same way, in receiver order.
 struct GatherEntry {
     int32_t key;
     int32_t src;
@@ -1176,9 +1175,27 @@ static void finish_level(std::vector<GatherEntry>& es, GatherLevel& out) {
     const uint64_t span = (uint64_t)(omax - omin) + 1;
     std::vector<uint64_t> sk(n);
     if ((unsigned __int128)((uint64_t)kmax + 1) * span < ((unsigned __int128)1 << (64 - ibits))) {
-        for (size_t i = 0; i < n; ++i)
+        // least-significant-digit radix sort of the packed keys, 11-bit
+        // digits over the bits the keys use
+        uint64_t kmaxv = 0;
+        for (size_t i = 0; i < n; ++i) {
             sk[i] = (((uint64_t)es[i].key * span + (uint64_t)(es[i].off - omin)) << ibits) | i;
-        std::sort(sk.begin(), sk.end());
+            kmaxv = std::max(kmaxv, sk[i]);
+        }
+        std::vector<uint64_t> tmp(n);
+        std::vector<size_t> cnt(2048);
+        for (int sh = 0; sh < 64 && (kmaxv >> sh) != 0; sh += 11) {
+            std::fill(cnt.begin(), cnt.end(), 0);
+            for (size_t i = 0; i < n; ++i) ++cnt[(sk[i] >> sh) & 2047];
+            size_t acc = 0;
+            for (size_t& c : cnt) {
+                const size_t v = c;
+                c = acc;
+                acc += v;
+            }
+            for (size_t i = 0; i < n; ++i) tmp[cnt[(sk[i] >> sh) & 2047]++] = sk[i];
+            sk.swap(tmp);
+        }
         for (uint64_t& v : sk) v &= ((uint64_t)1 << ibits) - 1;
     } else {
         for (size_t i = 0; i < n; ++i) sk[i] = i;
@@ -1230,71 +1247,25 @@ static void gather_bc_level(const OpesciContext* c, const BcLevel& lv, int32_t* 
     finish_level(es, out);
 }
 
-// media maps of one level: target = the media cell a derivative program
-// differentiates, value = its slot (point p of descriptor d: boff[d] + p * n
-// + j, n the descriptor's programs)
-static void gather_md_level(const OpesciContext* c, const BcLevel& lv, int32_t* boff, GatherLevel& out) {
-    const Geom& g = c->g;
-    std::vector<GatherEntry> es;
-    int64_t slots = 0;
-    for (int di = lv.first; di < lv.first + lv.count; ++di) {
-        const BcDesc& d = c->h_desc[di];
-        boff[di] = (int32_t)slots;
-        int n = 0;
-        for (int k = 0; k < d.nterms; ++k)
-            if (d.t[k].cslot >= 0) n += d.t[k].ndterm;
-        if (n == 0) continue;
-        for (int p = 0; p < d.npts; ++p) {
-            const int i0 = d.lo0 + p / d.n1, i1 = d.lo1 + p % d.n1;
-            const int ax = d.axis == 0 ? 0 : i0;
-            const int ay = d.axis == 0 ? i0 : (d.axis == 1 ? 0 : i1);
-            const int az = d.axis == 2 ? 0 : i1;
-            int j = 0;
-            for (int k = 0; k < d.nterms; ++k) {
-                if (d.t[k].cslot < 0) continue;
-                for (int q = d.t[k].dterm0; q < d.t[k].dterm0 + d.t[k].ndterm; ++q, ++j) {
-                    const OpesciBcDTerm& T = c->h_dterms[q];
-                    es.push_back({T.media, (int32_t)(slots + (int64_t)p * n + j),
-                                  g.at(ax + T.off[0], ay + T.off[1], az + T.off[2]), 0});
-                }
-            }
-        }
-        slots += (int64_t)d.npts * n;
-        if (slots > INT32_MAX || es.size() > (size_t)INT32_MAX) {
-            out.err = 1;
-            return;
-        }
-    }
-    out.slots = slots;
-    finish_level(es, out);
-}
-
 static int build_adj_gathers(OpesciContext* c) {
     const Geom& g = c->g;
     const size_t rb = c->real_bytes;
     const int nd = (int)c->h_desc.size();
-    std::vector<int32_t> boff_bc(nd, 0), boff_md(nd, 0);
-    // one task per (phase, level, kind), run on a few host threads
+    std::vector<int32_t> boff_bc(nd, 0);
+    // one task per level of the stepping phases, run on a few host threads
     struct Task {
-        int ph, L, md;
+        int ph, L;
     };
     std::vector<Task> tasks;
-    for (int ph = 0; ph < 5; ++ph)
-        for (int L = 0; L < (int)c->levels[ph].size(); ++L) {
-            if (ph >= 2) tasks.push_back({ph, L, 0});
-            tasks.push_back({ph, L, 1});
-        }
+    for (int ph = 2; ph < 5; ++ph)
+        for (int L = 0; L < (int)c->levels[ph].size(); ++L) tasks.push_back({ph, L});
     std::vector<GatherLevel> res(tasks.size());
     {
         std::atomic<size_t> next{0};
         auto work = [&]() {
             for (size_t t; (t = next.fetch_add(1)) < tasks.size();) {
                 const Task& k = tasks[t];
-                const BcLevel& lv = c->levels[k.ph][k.L];
-                if (k.md)
-                    gather_md_level(c, lv, boff_md.data(), res[t]);
-                else
-                    gather_bc_level(c, lv, boff_bc.data(), res[t]);
+                gather_bc_level(c, c->levels[k.ph][k.L], boff_bc.data(), res[t]);
             }
         };
         const int nt = (int)std::min<size_t>(tasks.size(), std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
@@ -1310,13 +1281,13 @@ static int build_adj_gathers(OpesciContext* c) {
         size_t e0, s0, t0;   // first entry, first start, first target
         int ntgt;
     };
-    std::vector<Span> sp_bc[5], sp_md[5];
+    std::vector<Span> sp_bc[5];
     for (int ph = 0; ph < 2; ++ph) sp_bc[ph].assign(c->levels[ph].size(), Span{0, 0, 0, 0});
     int64_t maxslots = 0;
     for (size_t t = 0; t < tasks.size(); ++t) {
         GatherLevel& r = res[t];
         if (r.err) return fail(OPESCI_EUNSUPPORTED, "boundary level too large for the adjoint");
-        (tasks[t].md ? sp_md : sp_bc)[tasks[t].ph].push_back(Span{src.size(), start.size(), key.size(), (int)r.key.size()});
+        sp_bc[tasks[t].ph].push_back(Span{src.size(), start.size(), key.size(), (int)r.key.size()});
         maxslots = std::max(maxslots, r.slots);
         start.insert(start.end(), r.start.begin(), r.start.end());
         key.insert(key.end(), r.key.begin(), r.key.end());
@@ -1350,8 +1321,7 @@ static int build_adj_gathers(OpesciContext* c) {
     };
     const size_t o_start = sec(start.size() * 4), o_key = sec(key.size() * 4), o_off = sec(off.size() * 8),
                  o_src = sec(src.size() * 4), o_ci = sec(cinfo.size() * 4), o_bbc = sec((size_t)nd * 4),
-                 o_bmd = sec((size_t)nd * 4), o_rs = sec(rstart.size() * 4), o_rc = sec(rcell.size() * 8),
-                 o_ro = sec(rorder.size() * 4);
+                 o_rs = sec(rstart.size() * 4), o_rc = sec(rcell.size() * 8), o_ro = sec(rorder.size() * 4);
     OPESCI_HIP_TRY(hipMalloc((void**)&c->d_gather, std::max(bytes, (size_t)256)));
     std::vector<char> blob(bytes);
     auto put = [&](size_t o, const void* p, size_t n) {
@@ -1363,7 +1333,6 @@ static int build_adj_gathers(OpesciContext* c) {
     put(o_src, src.data(), src.size() * 4);
     put(o_ci, cinfo.data(), cinfo.size() * 4);
     put(o_bbc, boff_bc.data(), (size_t)nd * 4);
-    put(o_bmd, boff_md.data(), (size_t)nd * 4);
     put(o_rs, rstart.data(), rstart.size() * 4);
     put(o_rc, rcell.data(), rcell.size() * 8);
     put(o_ro, rorder.data(), rorder.size() * 4);
@@ -1388,10 +1357,7 @@ static int build_adj_gathers(OpesciContext* c) {
             out.push_back(a);
         }
     };
-    for (int ph = 0; ph < 5; ++ph) {
-        views(sp_bc[ph], o_bbc, c->gat_bc[ph]);
-        views(sp_md[ph], o_bmd, c->gat_md[ph]);
-    }
+    for (int ph = 0; ph < 5; ++ph) views(sp_bc[ph], o_bbc, c->gat_bc[ph]);
     c->rg_start = (const int32_t*)(base + o_rs);
     c->rg_cell = (const int64_t*)(base + o_rc);
     c->rg_order = (const int32_t*)(base + o_ro);
@@ -1760,12 +1726,20 @@ int opesci_ctx_gradient_reverse(OpesciContext* c) {
             if (int e = step_adjoint(c, n, fp, log)) return e;
         }
     }
-    // media-dependent boundary coefficients -> media cells
-    for (int ph = 0; ph < 5; ++ph)
-        for (size_t L = 0; L < c->levels[ph].size(); ++L)
-            OPESCI_HIP_TRY(launch_adj_coef_media(c->g, c->real_bytes, c->mptr, c->gptr, c->d_desc, c->d_code,
-                                                 c->d_dterms, c->d_gpool, c->levels[ph][L], c->gat_md[ph][L],
-                                                 c->stream));
+    // media-dependent boundary coefficients -> media cells, one launch per
+    // (statement, term, derivative program) of the stepping phases (the
+    // only ones whose coefficient gradients the transposes accumulate)
+    for (int ph = 2; ph < 5; ++ph)
+        for (const BcLevel& lv : c->levels[ph])
+            for (int di = lv.first; di < lv.first + lv.count; ++di) {
+                const BcDesc& d = c->h_desc[di];
+                for (int k = 0; k < d.nterms; ++k)
+                    if (d.t[k].cslot >= 0)
+                        for (int q = d.t[k].dterm0; q < d.t[k].dterm0 + d.t[k].ndterm; ++q)
+                            OPESCI_HIP_TRY(launch_adj_coef_media(c->g, c->real_bytes, c->mptr, c->gptr, c->d_desc,
+                                                                 c->d_code, c->d_dterms, c->d_gpool, di, d.npts, k, q,
+                                                                 c->stream));
+            }
     OPESCI_HIP_TRY(hipStreamSynchronize(c->stream));
     OPESCI_HIP_TRY(hipGetLastError());
     return 0;
