@@ -15,7 +15,9 @@
 #include <chrono>
 #include <cmath>
 #include <memory>
+#include <new>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -907,84 +909,119 @@ double interior_points(const OpesciContext* c) {
 
 }  // namespace
 
+// C++ exceptions (host allocation, thread start) never cross the C ABI:
+// every entry point returns them as error codes.
+template <typename F>
+int guarded(F&& f) {
+    try {
+        return f();
+    } catch (const std::bad_alloc&) {
+        return fail(OPESCI_ENOMEM, "host allocation failed");
+    } catch (const std::system_error& e) {
+        return fail(OPESCI_ENOMEM, std::string("host resources: ") + e.what());
+    } catch (const std::exception& e) {
+        return fail(OPESCI_EUNSUPPORTED, e.what());
+    }
+}
+
 extern "C" {
 
 const char* opesci_last_error(void) { return g_error.c_str(); }
 int opesci_abi_version(void) { return OPESCI_ABI_VERSION; }
 
 int opesci_device_info(char* name, int32_t name_len, char* arch, int32_t arch_len) {
-    if (name && name_len > 0) name[0] = 0;
-    if (arch && arch_len > 0) arch[0] = 0;
-    int n = 0;
-    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(OPESCI_EHIP, "no HIP device available");
-    hipDeviceProp_t p;
-    OPESCI_HIP_TRY(hipGetDeviceProperties(&p, 0));
-    if (name && name_len > 0) {
-        strncpy(name, p.name, name_len - 1);
-        name[name_len - 1] = 0;
-    }
-    if (arch && arch_len > 0) {
-        strncpy(arch, p.gcnArchName, arch_len - 1);
-        arch[arch_len - 1] = 0;
-    }
-    return 0;
+    return guarded([&]() -> int {
+        if (name && name_len > 0) name[0] = 0;
+        if (arch && arch_len > 0) arch[0] = 0;
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(OPESCI_EHIP, "no HIP device available");
+        hipDeviceProp_t p;
+        OPESCI_HIP_TRY(hipGetDeviceProperties(&p, 0));
+        if (name && name_len > 0) {
+            strncpy(name, p.name, name_len - 1);
+            name[name_len - 1] = 0;
+        }
+        if (arch && arch_len > 0) {
+            strncpy(arch, p.gcnArchName, arch_len - 1);
+            arch[arch_len - 1] = 0;
+        }
+        return 0;
+    });
 }
 
 int opesci_ctx_create(const OpesciModelDesc* desc, OpesciContext** out) {
-    if (!out) return fail(OPESCI_EINVAL, "null output handle");
-    *out = nullptr;
-    return create(desc, out);
+    return guarded([&]() -> int {
+        if (!out) return fail(OPESCI_EINVAL, "null output handle");
+        *out = nullptr;
+        return create(desc, out);
+    });
 }
 
 int opesci_ctx_destroy(OpesciContext* ctx) {
-    if (ctx == g_default) g_default = nullptr;
-    destroy(ctx);
-    return 0;
+    return guarded([&]() -> int {
+        if (ctx == g_default) g_default = nullptr;
+        destroy(ctx);
+        return 0;
+    });
 }
 
 int opesci_ctx_init(OpesciContext* ctx) {
-    if (!ctx) return fail(OPESCI_EINVAL, "null context");
-    return ctx_init(ctx);
+    return guarded([&]() -> int {
+        if (!ctx) return fail(OPESCI_EINVAL, "null context");
+        return ctx_init(ctx);
+    });
 }
 
 int opesci_ctx_run(OpesciContext* ctx, int32_t nsteps, double* seconds) {
-    if (!ctx) return fail(OPESCI_EINVAL, "null context");
-    return ctx_run(ctx, nsteps, seconds);
+    return guarded([&]() -> int {
+        if (!ctx) return fail(OPESCI_EINVAL, "null context");
+        return ctx_run(ctx, nsteps, seconds);
+    });
 }
 
 int opesci_ctx_l2(OpesciContext* ctx, double out[OPESCI_NFIELDS]) {
-    if (!ctx || !out) return fail(OPESCI_EINVAL, "null argument");
-    return ctx_l2(ctx, out);
+    return guarded([&]() -> int {
+        if (!ctx || !out) return fail(OPESCI_EINVAL, "null argument");
+        return ctx_l2(ctx, out);
+    });
 }
 
 int opesci_ctx_l2_noalloc(OpesciContext* ctx, double* d_out, void* stream) {
-    if (!ctx || !d_out) return fail(OPESCI_EINVAL, "null argument");
-    if (!ctx->initialised) return fail(OPESCI_ESTATE, "no state: call opesci_ctx_init first");
-    int dev = -1;
-    OPESCI_HIP_TRY(hipGetDevice(&dev));
-    if (dev != ctx->device) return fail(OPESCI_ESTATE, "the context's device must be the current device");
-    double t[NF];
-    for (int f = 0; f < NF; ++f) t[f] = ctx->desc.dt * ctx->steps + (time_stag(f) ? ctx->desc.dt / 2.0 : 0.0);
-    const hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
-    OPESCI_HIP_TRY(launch_l2_noalloc(ctx->g, ctx->real_bytes, ctx->storage, ctx->fptr, ctx->mptr, ctx->sol,
-                                     ctx->desc.spacing, ctx->steps % 2, t, d_out, s));
-    return 0;
+    return guarded([&]() -> int {
+        if (!ctx || !d_out) return fail(OPESCI_EINVAL, "null argument");
+        if (!ctx->initialised) return fail(OPESCI_ESTATE, "no state: call opesci_ctx_init first");
+        int dev = -1;
+        OPESCI_HIP_TRY(hipGetDevice(&dev));
+        if (dev != ctx->device) return fail(OPESCI_ESTATE, "the context's device must be the current device");
+        double t[NF];
+        for (int f = 0; f < NF; ++f) t[f] = ctx->desc.dt * ctx->steps + (time_stag(f) ? ctx->desc.dt / 2.0 : 0.0);
+        const hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+        OPESCI_HIP_TRY(launch_l2_noalloc(ctx->g, ctx->real_bytes, ctx->storage, ctx->fptr, ctx->mptr, ctx->sol,
+                                         ctx->desc.spacing, ctx->steps % 2, t, d_out, s));
+        return 0;
+    });
 }
 
 int opesci_ctx_download(OpesciContext* ctx, int32_t field, int32_t level, void* host) {
-    if (!ctx) return fail(OPESCI_EINVAL, "null context");
-    return ctx_download(ctx, field, level, host);
+    return guarded([&]() -> int {
+        if (!ctx) return fail(OPESCI_EINVAL, "null context");
+        return ctx_download(ctx, field, level, host);
+    });
 }
 
 int opesci_ctx_upload(OpesciContext* ctx, int32_t field, int32_t level, const void* host) {
-    if (!ctx) return fail(OPESCI_EINVAL, "null context");
-    return ctx_upload(ctx, field, level, host);
+    return guarded([&]() -> int {
+        if (!ctx) return fail(OPESCI_EINVAL, "null context");
+        return ctx_upload(ctx, field, level, host);
+    });
 }
 
 int opesci_ctx_steps(OpesciContext* ctx, int32_t* steps) {
-    if (!ctx || !steps) return fail(OPESCI_EINVAL, "null argument");
-    *steps = ctx->steps;
-    return 0;
+    return guarded([&]() -> int {
+        if (!ctx || !steps) return fail(OPESCI_EINVAL, "null argument");
+        *steps = ctx->steps;
+        return 0;
+    });
 }
 
 }  // extern "C"
@@ -1013,125 +1050,135 @@ static int receivers_as(OpesciContext* ctx, T* host, int64_t capacity) {
 extern "C" {
 
 int opesci_ctx_receivers(OpesciContext* ctx, float* host, int64_t capacity) {
-    return receivers_as(ctx, host, capacity);
+    return guarded([&]() -> int {
+        return receivers_as(ctx, host, capacity);
+    });
 }
 
 int opesci_ctx_receivers_f64(OpesciContext* ctx, double* host, int64_t capacity) {
-    return receivers_as(ctx, host, capacity);
+    return guarded([&]() -> int {
+        return receivers_as(ctx, host, capacity);
+    });
 }
 
 int opesci_ctx_kernel_times(OpesciContext* ctx, int32_t nsteps, double out_ms[3]) {
-    if (!ctx || !out_ms || nsteps <= 0) return fail(OPESCI_EINVAL, "bad arguments");
-    if (!ctx->initialised) return fail(OPESCI_ESTATE, "opesci_ctx_init must run first");
-    std::vector<hipEvent_t> ev((size_t)nsteps * 5 + 1);
-    for (auto& e : ev) OPESCI_HIP_TRY(hipEventCreate(&e));
-    int rc = 0;
-    const bool fused = fused_step(ctx);
-    for (int s = 0; s < nsteps && !rc && fused; ++s) {
-        // fused kernel | frame stress | stress BCs | frame velocity | velocity BCs, sources
-        const int t0 = ctx->steps % 2, t1 = 1 - t0;
-        hipEvent_t* e = &ev[(size_t)s * 5];
-        OPESCI_HIP_TRY(hipEventRecord(e[0], ctx->stream));
-        OPESCI_HIP_TRY(launch_step_fused(ctx->g, ctx->storage, ctx->fptr, ctx->coefs(), t0, t1, ctx->zf, ctx->fbox, 0,
-                                         ctx->stream));
-        OPESCI_HIP_TRY(hipEventRecord(e[1], ctx->stream));
-        OPESCI_HIP_TRY(launch_stress(ctx->g, ctx->real_bytes, ctx->storage, ctx->het, ctx->fptr, ctx->mptr,
-                                     ctx->coefs(), t0, t1, ctx->cfg, ctx->stream, &ctx->zf, ctx->d_pool, &ctx->frame));
-        OPESCI_HIP_TRY(hipEventRecord(e[2], ctx->stream));
-        rc = run_bc_phase(ctx, 2, t0, t1);
-        OPESCI_HIP_TRY(hipEventRecord(e[3], ctx->stream));
-        OPESCI_HIP_TRY(launch_velocity(ctx->g, ctx->real_bytes, ctx->storage, ctx->het, ctx->fptr, ctx->mptr,
-                                       ctx->coefs(), t0, t1, ctx->cfg, ctx->stream, &ctx->frame));
-        OPESCI_HIP_TRY(hipEventRecord(e[4], ctx->stream));
-        if (!rc) rc = run_bc_phase(ctx, 3, t0, t1);
-        if (!rc && (ctx->nrec > 0 || ctx->src_ntr))
-            OPESCI_HIP_TRY(launch_record_source(ctx->g, ctx->real_bytes, ctx->storage, ctx->fptr, t1, ctx->nrec,
-                                                ctx->d_rec, ctx->d_trace, ctx->src_ntr, ctx->src_idx,
-                                                ctx->d_wavelet, ctx->d_step, ctx->desc.ntsteps, ctx->stream));
-        OPESCI_HIP_TRY(hipEventRecord(e[5], ctx->stream));
-        ++ctx->steps;
-    }
-    for (int s = 0; s < nsteps && !rc && !fused; ++s) {
-        const int t0 = ctx->steps % 2, t1 = 1 - t0;
-        hipEvent_t* e = &ev[(size_t)s * 5];
-        OPESCI_HIP_TRY(hipEventRecord(e[0], ctx->stream));
-        const bool zin = zfaces_in_sweep(ctx);
-        OPESCI_HIP_TRY(launch_stress(ctx->g, ctx->real_bytes, ctx->storage, ctx->het, ctx->fptr, ctx->mptr,
-                                     ctx->coefs(), t0, t1, ctx->cfg, ctx->stream, zin ? &ctx->zf : nullptr,
-                                     ctx->d_pool));
-        OPESCI_HIP_TRY(hipEventRecord(e[1], ctx->stream));
-        if (!zin) rc = run_bc_phase(ctx, 4, t0, t1);
-        if (!rc) rc = run_bc_phase(ctx, 2, t0, t1);
-        OPESCI_HIP_TRY(hipEventRecord(e[2], ctx->stream));
-        OPESCI_HIP_TRY(launch_velocity(ctx->g, ctx->real_bytes, ctx->storage, ctx->het, ctx->fptr, ctx->mptr,
-                                       ctx->coefs(), t0, t1, ctx->cfg, ctx->stream));
-        OPESCI_HIP_TRY(hipEventRecord(e[3], ctx->stream));
-        if (!rc) rc = run_bc_phase(ctx, 3, t0, t1);
-        if (!rc && (ctx->nrec > 0 || ctx->src_ntr))
-            OPESCI_HIP_TRY(launch_record_source(ctx->g, ctx->real_bytes, ctx->storage, ctx->fptr, t1, ctx->nrec,
-                                                ctx->d_rec, ctx->d_trace, ctx->src_ntr, ctx->src_idx,
-                                                ctx->d_wavelet, ctx->d_step, ctx->desc.ntsteps, ctx->stream));
-        OPESCI_HIP_TRY(hipEventRecord(e[4], ctx->stream));
-        ++ctx->steps;
-    }
-    OPESCI_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    double acc[3] = {0, 0, 0};
-    for (int s = 0; s < nsteps && fused; ++s) {
-        hipEvent_t* e = &ev[(size_t)s * 5];
-        float a = 0, b = 0, c2 = 0, d2 = 0, e2 = 0;
-        OPESCI_HIP_TRY(hipEventElapsedTime(&a, e[0], e[1]));
-        OPESCI_HIP_TRY(hipEventElapsedTime(&b, e[1], e[2]));
-        OPESCI_HIP_TRY(hipEventElapsedTime(&c2, e[2], e[3]));
-        OPESCI_HIP_TRY(hipEventElapsedTime(&d2, e[3], e[4]));
-        OPESCI_HIP_TRY(hipEventElapsedTime(&e2, e[4], e[5]));
-        acc[0] += a;
-        acc[1] += b + d2;
-        acc[2] += c2 + e2;
-    }
-    for (int s = 0; s < nsteps && !fused; ++s) {
-        hipEvent_t* e = &ev[(size_t)s * 5];
-        float a = 0, b = 0, c2 = 0, d2 = 0;
-        OPESCI_HIP_TRY(hipEventElapsedTime(&a, e[0], e[1]));
-        OPESCI_HIP_TRY(hipEventElapsedTime(&b, e[2], e[3]));
-        OPESCI_HIP_TRY(hipEventElapsedTime(&c2, e[1], e[2]));
-        OPESCI_HIP_TRY(hipEventElapsedTime(&d2, e[3], e[4]));
-        acc[0] += a;
-        acc[1] += b;
-        acc[2] += c2 + d2;
-    }
-    for (auto& e : ev) (void)hipEventDestroy(e);
-    for (int i = 0; i < 3; ++i) out_ms[i] = acc[i] / nsteps;
-    return rc;
+    return guarded([&]() -> int {
+        if (!ctx || !out_ms || nsteps <= 0) return fail(OPESCI_EINVAL, "bad arguments");
+        if (!ctx->initialised) return fail(OPESCI_ESTATE, "opesci_ctx_init must run first");
+        std::vector<hipEvent_t> ev((size_t)nsteps * 5 + 1);
+        for (auto& e : ev) OPESCI_HIP_TRY(hipEventCreate(&e));
+        int rc = 0;
+        const bool fused = fused_step(ctx);
+        for (int s = 0; s < nsteps && !rc && fused; ++s) {
+            // fused kernel | frame stress | stress BCs | frame velocity | velocity BCs, sources
+            const int t0 = ctx->steps % 2, t1 = 1 - t0;
+            hipEvent_t* e = &ev[(size_t)s * 5];
+            OPESCI_HIP_TRY(hipEventRecord(e[0], ctx->stream));
+            OPESCI_HIP_TRY(launch_step_fused(ctx->g, ctx->storage, ctx->fptr, ctx->coefs(), t0, t1, ctx->zf, ctx->fbox, 0,
+                                             ctx->stream));
+            OPESCI_HIP_TRY(hipEventRecord(e[1], ctx->stream));
+            OPESCI_HIP_TRY(launch_stress(ctx->g, ctx->real_bytes, ctx->storage, ctx->het, ctx->fptr, ctx->mptr,
+                                         ctx->coefs(), t0, t1, ctx->cfg, ctx->stream, &ctx->zf, ctx->d_pool, &ctx->frame));
+            OPESCI_HIP_TRY(hipEventRecord(e[2], ctx->stream));
+            rc = run_bc_phase(ctx, 2, t0, t1);
+            OPESCI_HIP_TRY(hipEventRecord(e[3], ctx->stream));
+            OPESCI_HIP_TRY(launch_velocity(ctx->g, ctx->real_bytes, ctx->storage, ctx->het, ctx->fptr, ctx->mptr,
+                                           ctx->coefs(), t0, t1, ctx->cfg, ctx->stream, &ctx->frame));
+            OPESCI_HIP_TRY(hipEventRecord(e[4], ctx->stream));
+            if (!rc) rc = run_bc_phase(ctx, 3, t0, t1);
+            if (!rc && (ctx->nrec > 0 || ctx->src_ntr))
+                OPESCI_HIP_TRY(launch_record_source(ctx->g, ctx->real_bytes, ctx->storage, ctx->fptr, t1, ctx->nrec,
+                                                    ctx->d_rec, ctx->d_trace, ctx->src_ntr, ctx->src_idx,
+                                                    ctx->d_wavelet, ctx->d_step, ctx->desc.ntsteps, ctx->stream));
+            OPESCI_HIP_TRY(hipEventRecord(e[5], ctx->stream));
+            ++ctx->steps;
+        }
+        for (int s = 0; s < nsteps && !rc && !fused; ++s) {
+            const int t0 = ctx->steps % 2, t1 = 1 - t0;
+            hipEvent_t* e = &ev[(size_t)s * 5];
+            OPESCI_HIP_TRY(hipEventRecord(e[0], ctx->stream));
+            const bool zin = zfaces_in_sweep(ctx);
+            OPESCI_HIP_TRY(launch_stress(ctx->g, ctx->real_bytes, ctx->storage, ctx->het, ctx->fptr, ctx->mptr,
+                                         ctx->coefs(), t0, t1, ctx->cfg, ctx->stream, zin ? &ctx->zf : nullptr,
+                                         ctx->d_pool));
+            OPESCI_HIP_TRY(hipEventRecord(e[1], ctx->stream));
+            if (!zin) rc = run_bc_phase(ctx, 4, t0, t1);
+            if (!rc) rc = run_bc_phase(ctx, 2, t0, t1);
+            OPESCI_HIP_TRY(hipEventRecord(e[2], ctx->stream));
+            OPESCI_HIP_TRY(launch_velocity(ctx->g, ctx->real_bytes, ctx->storage, ctx->het, ctx->fptr, ctx->mptr,
+                                           ctx->coefs(), t0, t1, ctx->cfg, ctx->stream));
+            OPESCI_HIP_TRY(hipEventRecord(e[3], ctx->stream));
+            if (!rc) rc = run_bc_phase(ctx, 3, t0, t1);
+            if (!rc && (ctx->nrec > 0 || ctx->src_ntr))
+                OPESCI_HIP_TRY(launch_record_source(ctx->g, ctx->real_bytes, ctx->storage, ctx->fptr, t1, ctx->nrec,
+                                                    ctx->d_rec, ctx->d_trace, ctx->src_ntr, ctx->src_idx,
+                                                    ctx->d_wavelet, ctx->d_step, ctx->desc.ntsteps, ctx->stream));
+            OPESCI_HIP_TRY(hipEventRecord(e[4], ctx->stream));
+            ++ctx->steps;
+        }
+        OPESCI_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        double acc[3] = {0, 0, 0};
+        for (int s = 0; s < nsteps && fused; ++s) {
+            hipEvent_t* e = &ev[(size_t)s * 5];
+            float a = 0, b = 0, c2 = 0, d2 = 0, e2 = 0;
+            OPESCI_HIP_TRY(hipEventElapsedTime(&a, e[0], e[1]));
+            OPESCI_HIP_TRY(hipEventElapsedTime(&b, e[1], e[2]));
+            OPESCI_HIP_TRY(hipEventElapsedTime(&c2, e[2], e[3]));
+            OPESCI_HIP_TRY(hipEventElapsedTime(&d2, e[3], e[4]));
+            OPESCI_HIP_TRY(hipEventElapsedTime(&e2, e[4], e[5]));
+            acc[0] += a;
+            acc[1] += b + d2;
+            acc[2] += c2 + e2;
+        }
+        for (int s = 0; s < nsteps && !fused; ++s) {
+            hipEvent_t* e = &ev[(size_t)s * 5];
+            float a = 0, b = 0, c2 = 0, d2 = 0;
+            OPESCI_HIP_TRY(hipEventElapsedTime(&a, e[0], e[1]));
+            OPESCI_HIP_TRY(hipEventElapsedTime(&b, e[2], e[3]));
+            OPESCI_HIP_TRY(hipEventElapsedTime(&c2, e[1], e[2]));
+            OPESCI_HIP_TRY(hipEventElapsedTime(&d2, e[3], e[4]));
+            acc[0] += a;
+            acc[1] += b;
+            acc[2] += c2 + d2;
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        for (int i = 0; i < 3; ++i) out_ms[i] = acc[i] / nsteps;
+        return rc;
+    });
 }
 
 int opesci_ctx_stats(OpesciContext* ctx, int32_t out[5]) {
-    if (!ctx || !out) return fail(OPESCI_EINVAL, "null argument");
-    int n4 = 0;
-    for (const BcLevel& lv : ctx->levels[4]) n4 += lv.count;
-    const bool zin = zfaces_in_sweep(ctx);
-    out[0] = (int)(ctx->levels[2].size() + ctx->levels[3].size() + (zin ? 0 : ctx->levels[4].size()));
-    out[1] = n4;
-    out[2] = zin ? 0 : (int)ctx->levels[4].size();
-    out[3] = zin ? 1 : 0;
-    out[4] = fused_step(ctx) ? ctx->fused_F : 0;
-    return 0;
+    return guarded([&]() -> int {
+        if (!ctx || !out) return fail(OPESCI_EINVAL, "null argument");
+        int n4 = 0;
+        for (const BcLevel& lv : ctx->levels[4]) n4 += lv.count;
+        const bool zin = zfaces_in_sweep(ctx);
+        out[0] = (int)(ctx->levels[2].size() + ctx->levels[3].size() + (zin ? 0 : ctx->levels[4].size()));
+        out[1] = n4;
+        out[2] = zin ? 0 : (int)ctx->levels[4].size();
+        out[3] = zin ? 1 : 0;
+        out[4] = fused_step(ctx) ? ctx->fused_F : 0;
+        return 0;
+    });
 }
 
 int opesci_ctx_configure(OpesciContext* ctx, int32_t kernel_variant, int32_t tile_y, int32_t x_chunk) {
-    if (!ctx) return fail(OPESCI_EINVAL, "null context");
-    if (kernel_variant < 0 || kernel_variant > 2 || (tile_y != 0 && tile_y != 4 && tile_y != 8 && tile_y != 16) ||
-        x_chunk < 0)
-        return fail(OPESCI_EINVAL, "kernel_variant 0|1|2, tile_y 0|4|8|16, x_chunk >= 0");
-    ctx->cfg = InteriorLaunch{kernel_variant, tile_y, x_chunk};
-    ctx->desc.kernel_variant = kernel_variant;
-    ctx->desc.tile_y = tile_y;
-    ctx->desc.x_chunk = x_chunk;
-    if (ctx->graph) {
-        OPESCI_HIP_TRY(hipStreamSynchronize(ctx->stream));
-        (void)hipGraphExecDestroy(ctx->graph);
-        ctx->graph = nullptr;
-    }
-    return 0;
+    return guarded([&]() -> int {
+        if (!ctx) return fail(OPESCI_EINVAL, "null context");
+        if (kernel_variant < 0 || kernel_variant > 2 || (tile_y != 0 && tile_y != 4 && tile_y != 8 && tile_y != 16) ||
+            x_chunk < 0)
+            return fail(OPESCI_EINVAL, "kernel_variant 0|1|2, tile_y 0|4|8|16, x_chunk >= 0");
+        ctx->cfg = InteriorLaunch{kernel_variant, tile_y, x_chunk};
+        ctx->desc.kernel_variant = kernel_variant;
+        ctx->desc.tile_y = tile_y;
+        ctx->desc.x_chunk = x_chunk;
+        if (ctx->graph) {
+            OPESCI_HIP_TRY(hipStreamSynchronize(ctx->stream));
+            (void)hipGraphExecDestroy(ctx->graph);
+            ctx->graph = nullptr;
+        }
+        return 0;
+    });
 }
 
 // ---------------------------------------------------------------- adjoint
@@ -1556,234 +1603,246 @@ static bool steps_keep_t0(const OpesciContext* c) {
 // opesci_ctx_gradient_reverse.
 int opesci_ctx_misfit_forward(OpesciContext* c, const float* observed, int64_t count, int32_t every,
                               double* misfit) {
-    if (!c || !observed || every <= 0) return fail(OPESCI_EINVAL, "bad arguments");
-    if (!c->het || c->storage != OPESCI_STORE_NATIVE)
-        return fail(OPESCI_EUNSUPPORTED, "gradients need the read path (media arrays) and native storage");
-    if (c->nrec <= 0) return fail(OPESCI_EINVAL, "gradients need receivers");
-    const int N = c->desc.ntsteps;
-    if (count != (int64_t)N * c->nrec * 3) return fail(OPESCI_EINVAL, "observed must hold ntsteps * nrec * 3 values");
-    c->grad_every = 0;
-    // a failure part way leaves a partial state: stepping needs
-    // opesci_ctx_init again
-    struct InitOnFailure {
-        OpesciContext* c;
-        bool ok = false;
-        ~InitOnFailure() {
-            if (!ok) {
-                c->initialised = false;
-                c->steps = 0;
+    return guarded([&]() -> int {
+        if (!c || !observed || every <= 0) return fail(OPESCI_EINVAL, "bad arguments");
+        if (!c->het || c->storage != OPESCI_STORE_NATIVE)
+            return fail(OPESCI_EUNSUPPORTED, "gradients need the read path (media arrays) and native storage");
+        if (c->nrec <= 0) return fail(OPESCI_EINVAL, "gradients need receivers");
+        const int N = c->desc.ntsteps;
+        if (count != (int64_t)N * c->nrec * 3) return fail(OPESCI_EINVAL, "observed must hold ntsteps * nrec * 3 values");
+        c->grad_every = 0;
+        // a failure part way leaves a partial state: stepping needs
+        // opesci_ctx_init again
+        struct InitOnFailure {
+            OpesciContext* c;
+            bool ok = false;
+            ~InitOnFailure() {
+                if (!ok) {
+                    c->initialised = false;
+                    c->steps = 0;
+                }
             }
+        } guard{c};
+        const int nck = (N + every - 1) / every;
+        const size_t state = (size_t)2 * NF * c->field_bytes;
+        // Segment scheme.  keep_t0: the segment's forward pass logs every step and
+        // writes its level t1 (nine arrays) into a slot of its own; the reverse
+        // step runs on pointers into those slots (level t0 = the previous step's
+        // slot, or the checkpoint).
+        // Otherwise (and with the simple launch variant 1, which selects the
+        // reference forms throughout): full states before each step, and the step
+        // recomputed with its log before its adjoint.
+        const bool keep_t0 = steps_keep_t0(c) && c->cfg.variant != 1;
+        const size_t slot = keep_t0 ? (size_t)NF * c->field_bytes : state;
+        if (int e = adj_alloc(c, std::max(nck, 1), slot * every, keep_t0 ? every : 1)) return e;
+        if (!c->d_obs) OPESCI_HIP_TRY(hipMalloc((void**)&c->d_obs, (size_t)count * sizeof(float)));
+        OPESCI_HIP_TRY(hipMemcpyAsync(c->d_obs, observed, (size_t)count * sizeof(float), hipMemcpyDefault, c->stream));
+        std::vector<float> obs_host((size_t)count);
+        OPESCI_HIP_TRY(hipMemcpyAsync(obs_host.data(), c->d_obs, (size_t)count * sizeof(float), hipMemcpyDeviceToHost,
+                                      c->stream));
+        // forward with checkpoints
+        if (int e = ctx_init(c)) return e;
+        for (int n = 0; n < N; ++n) {
+            if (n % every == 0)
+                if (int e = state_copy(c, c->d_ckpt + (size_t)(n / every) * state, true)) return e;
+            if (int e = step_once(c, n % 2)) return e;
         }
-    } guard{c};
-    const int nck = (N + every - 1) / every;
-    const size_t state = (size_t)2 * NF * c->field_bytes;
-    // Segment scheme.  keep_t0: the segment's forward pass logs every step and
-    // writes its level t1 (nine arrays) into a slot of its own; the reverse
-    // step runs on pointers into those slots (level t0 = the previous step's
-    // slot, or the checkpoint).
-    // Otherwise (and with the simple launch variant 1, which selects the
-    // reference forms throughout): full states before each step, and the step
-    // recomputed with its log before its adjoint.
-    const bool keep_t0 = steps_keep_t0(c) && c->cfg.variant != 1;
-    const size_t slot = keep_t0 ? (size_t)NF * c->field_bytes : state;
-    if (int e = adj_alloc(c, std::max(nck, 1), slot * every, keep_t0 ? every : 1)) return e;
-    if (!c->d_obs) OPESCI_HIP_TRY(hipMalloc((void**)&c->d_obs, (size_t)count * sizeof(float)));
-    OPESCI_HIP_TRY(hipMemcpyAsync(c->d_obs, observed, (size_t)count * sizeof(float), hipMemcpyDefault, c->stream));
-    std::vector<float> obs_host((size_t)count);
-    OPESCI_HIP_TRY(hipMemcpyAsync(obs_host.data(), c->d_obs, (size_t)count * sizeof(float), hipMemcpyDeviceToHost,
-                                  c->stream));
-    // forward with checkpoints
-    if (int e = ctx_init(c)) return e;
-    for (int n = 0; n < N; ++n) {
-        if (n % every == 0)
-            if (int e = state_copy(c, c->d_ckpt + (size_t)(n / every) * state, true)) return e;
-        if (int e = step_once(c, n % 2)) return e;
-    }
-    c->steps = N;
-    // misfit from the recorded U, V, W traces
-    std::vector<double> tr((size_t)N * c->nrec * 4);
-    if (int e = receivers_as(c, tr.data(), (int64_t)tr.size())) return e;
-    double J = 0;
-    for (int64_t i = 0; i < (int64_t)N * c->nrec; ++i)
-        for (int k = 0; k < 3; ++k) {
-            const double r = tr[(size_t)i * 4 + k] - (double)obs_host[(size_t)i * 3 + k];
-            J += 0.5 * r * r;
-        }
-    if (misfit) *misfit = J;
-    c->grad_every = every;
-    c->grad_keep_t0 = keep_t0;
-    guard.ok = true;
-    return 0;
+        c->steps = N;
+        // misfit from the recorded U, V, W traces
+        std::vector<double> tr((size_t)N * c->nrec * 4);
+        if (int e = receivers_as(c, tr.data(), (int64_t)tr.size())) return e;
+        double J = 0;
+        for (int64_t i = 0; i < (int64_t)N * c->nrec; ++i)
+            for (int k = 0; k < 3; ++k) {
+                const double r = tr[(size_t)i * 4 + k] - (double)obs_host[(size_t)i * 3 + k];
+                J += 0.5 * r * r;
+            }
+        if (misfit) *misfit = J;
+        c->grad_every = every;
+        c->grad_keep_t0 = keep_t0;
+        guard.ok = true;
+        return 0;
+    });
 }
 
 // Reverse sweep after opesci_ctx_misfit_forward: media gradients into the
 // context's gradient arrays.  Leaves a recomputed early state in the fields.
 int opesci_ctx_gradient_reverse(OpesciContext* c) {
-    if (!c) return fail(OPESCI_EINVAL, "null context");
-    if (c->grad_every <= 0) return fail(OPESCI_ESTATE, "no forward pass: call opesci_ctx_misfit_forward first");
-    // the fields end up holding a recomputed state (or, after a failure, a
-    // partial one): on every return, stepping needs opesci_ctx_init again,
-    // and a new forward pass precedes the next reverse sweep
-    struct NeedsInit {
-        OpesciContext* c;
-        ~NeedsInit() {
-            c->initialised = false;
-            c->steps = 0;
-            c->grad_every = 0;
-        }
-    } needs_init{c};
-    const int N = c->desc.ntsteps, every = c->grad_every;
-    const int nck = (N + every - 1) / every;
-    const size_t state = (size_t)2 * NF * c->field_bytes;
-    const bool keep_t0 = c->grad_keep_t0;
-    const size_t slot = keep_t0 ? (size_t)NF * c->field_bytes : state;
-    // reverse sweep, segment by segment
-    for (int f = 0; f < NF; ++f)
-        for (int l = 0; l < 2; ++l) OPESCI_HIP_TRY(launch_zero(c->abase[f][l], c->field_bytes, c->stream));
-    const size_t mbytes = (size_t)c->g.nx * c->g.sx * c->real_bytes;
-    for (int i = 0; i < OPESCI_M_COUNT; ++i) OPESCI_HIP_TRY(launch_zero(c->gbase[i], mbytes, c->stream));
-    if (c->d_gpool)
-        OPESCI_HIP_TRY(launch_zero(c->d_gpool, ((size_t)c->pool_elems * c->real_bytes + 15) & ~size_t(15), c->stream));
-    const size_t rb = c->real_bytes, zo = (size_t)c->g.zoff * rb;
-    const int* si = c->src_idx;
-    char* srclog = static_cast<char*>(c->d_srclog);
-    // keep_t0: slot k of a segment holds level t1 of step k0 + k, written in
-    // place by the logged forward (the step reads level t0 from the slot
-    // before it, or from the checkpoint).  Cells no step writes hold a
-    // constant per level, so a slot is filled once with the whole level of
-    // its parity (slot_par; once per gradient when `every` is even).  Cells
-    // a boundary statement reads at level t1 before a later statement of the
-    // step writes them hold the value of two steps before, copied in from
-    // the slot two back (or the checkpoint) ahead of the step.
-    std::vector<int> slot_par(keep_t0 ? every : 0, -1);
-    const size_t fb = c->field_bytes;
-    for (int seg = nck - 1; seg >= 0; --seg) {
-        const int k0 = seg * every, k1 = std::min(N, k0 + every);
-        char* ck = c->d_ckpt + (size_t)seg * state;
-        OPESCI_HIP_TRY(launch_set_step(c->d_step, k0, c->stream));
-        if (!keep_t0) {
-            if (int e = state_copy(c, ck, false)) return e;
+    return guarded([&]() -> int {
+        if (!c) return fail(OPESCI_EINVAL, "null context");
+        if (c->grad_every <= 0) return fail(OPESCI_ESTATE, "no forward pass: call opesci_ctx_misfit_forward first");
+        // the fields end up holding a recomputed state (or, after a failure, a
+        // partial one): on every return, stepping needs opesci_ctx_init again,
+        // and a new forward pass precedes the next reverse sweep
+        struct NeedsInit {
+            OpesciContext* c;
+            ~NeedsInit() {
+                c->initialised = false;
+                c->steps = 0;
+                c->grad_every = 0;
+            }
+        } needs_init{c};
+        const int N = c->desc.ntsteps, every = c->grad_every;
+        const int nck = (N + every - 1) / every;
+        const size_t state = (size_t)2 * NF * c->field_bytes;
+        const bool keep_t0 = c->grad_keep_t0;
+        const size_t slot = keep_t0 ? (size_t)NF * c->field_bytes : state;
+        // reverse sweep, segment by segment
+        for (int f = 0; f < NF; ++f)
+            for (int l = 0; l < 2; ++l) OPESCI_HIP_TRY(launch_zero(c->abase[f][l], c->field_bytes, c->stream));
+        const size_t mbytes = (size_t)c->g.nx * c->g.sx * c->real_bytes;
+        for (int i = 0; i < OPESCI_M_COUNT; ++i) OPESCI_HIP_TRY(launch_zero(c->gbase[i], mbytes, c->stream));
+        if (c->d_gpool)
+            OPESCI_HIP_TRY(launch_zero(c->d_gpool, ((size_t)c->pool_elems * c->real_bytes + 15) & ~size_t(15), c->stream));
+        const size_t rb = c->real_bytes, zo = (size_t)c->g.zoff * rb;
+        const int* si = c->src_idx;
+        char* srclog = static_cast<char*>(c->d_srclog);
+        // keep_t0: slot k of a segment holds level t1 of step k0 + k, written in
+        // place by the logged forward (the step reads level t0 from the slot
+        // before it, or from the checkpoint).  Cells no step writes hold a
+        // constant per level, so a slot is filled once with the whole level of
+        // its parity (slot_par; once per gradient when `every` is even).  Cells
+        // a boundary statement reads at level t1 before a later statement of the
+        // step writes them hold the value of two steps before, copied in from
+        // the slot two back (or the checkpoint) ahead of the step.
+        std::vector<int> slot_par(keep_t0 ? every : 0, -1);
+        const size_t fb = c->field_bytes;
+        for (int seg = nck - 1; seg >= 0; --seg) {
+            const int k0 = seg * every, k1 = std::min(N, k0 + every);
+            char* ck = c->d_ckpt + (size_t)seg * state;
+            OPESCI_HIP_TRY(launch_set_step(c->d_step, k0, c->stream));
+            if (!keep_t0) {
+                if (int e = state_copy(c, ck, false)) return e;
+                for (int n = k0; n < k1; ++n) {
+                    if (int e = state_copy(c, c->d_seg + (size_t)(n - k0) * state, true)) return e;
+                    if (int e = step_once(c, n % 2)) return e;
+                }
+                for (int n = k1 - 1; n >= k0; --n) {
+                    if (int e = state_copy(c, c->d_seg + (size_t)(n - k0) * state, false)) return e;
+                    if (int e = step_logged(c, c->fptr, n % 2, static_cast<char*>(c->d_log))) return e;
+                    if (int e = step_adjoint(c, n, c->fptr, static_cast<char*>(c->d_log))) return e;
+                }
+                continue;
+            }
+            // level pointers of step n: t1 in slot n - k0, t0 in the slot before
+            // (or the checkpoint)
+            auto slot_image = [&](int n, void* fp[NF][2]) {
+                const int k = n - k0, t0 = n % 2, t1 = 1 - t0;
+                for (int f = 0; f < NF; ++f) {
+                    fp[f][t1] = c->d_seg + (size_t)k * slot + (size_t)f * fb + zo;
+                    fp[f][t0] = k > 0 ? c->d_seg + (size_t)(k - 1) * slot + (size_t)f * fb + zo
+                                      : ck + (size_t)(2 * f + t0) * fb + zo;
+                }
+            };
             for (int n = k0; n < k1; ++n) {
-                if (int e = state_copy(c, c->d_seg + (size_t)(n - k0) * state, true)) return e;
-                if (int e = step_once(c, n % 2)) return e;
+                const int k = n - k0, t0 = n % 2, t1 = 1 - t0;
+                char* sk = c->d_seg + (size_t)k * slot;
+                if (slot_par[k] != t1) {
+                    for (int f = 0; f < NF; ++f)
+                        OPESCI_HIP_TRY(hipMemcpyAsync(sk + (size_t)f * fb, ck + (size_t)(2 * f + t1) * fb, fb,
+                                                      hipMemcpyDeviceToDevice, c->stream));
+                    slot_par[k] = t1;
+                }
+                void* fp[NF][2];
+                slot_image(n, fp);
+                void* fs[NF][2];
+                for (int f = 0; f < NF; ++f)
+                    fs[f][0] = fs[f][1] = k >= 2 ? c->d_seg + (size_t)(k - 2) * slot + (size_t)f * fb + zo
+                                                 : ck + (size_t)(2 * f + t1) * fb + zo;
+                OPESCI_HIP_TRY(launch_copy_cells(c->real_bytes, fp, t1, fs, t1, c->d_scells, c->n_scells, c->stream));
+                char* log = static_cast<char*>(c->d_log) + (size_t)k * c->log_step * rb;
+                char* sl = srclog + (size_t)6 * k * rb;
+                if (int e = step_logged(c, fp, t0, log)) return e;
+                if (c->src_ntr)
+                    OPESCI_HIP_TRY(launch_src_cells(c->g, c->real_bytes, fp, t1, si[0], si[1], si[2], sl, 0, c->stream));
+                // source and step counter only (the traces are already recorded)
+                OPESCI_HIP_TRY(launch_record_source(c->g, c->real_bytes, c->storage, fp, t1, 0, c->d_rec, c->d_trace,
+                                                    c->src_ntr, c->src_idx, c->d_wavelet, c->d_step, c->desc.ntsteps,
+                                                    c->stream));
+                if (c->src_ntr)
+                    OPESCI_HIP_TRY(launch_src_cells(c->g, c->real_bytes, fp, t1, si[0], si[1], si[2], sl + 3 * rb, 0,
+                                                    c->stream));
             }
             for (int n = k1 - 1; n >= k0; --n) {
-                if (int e = state_copy(c, c->d_seg + (size_t)(n - k0) * state, false)) return e;
-                if (int e = step_logged(c, c->fptr, n % 2, static_cast<char*>(c->d_log))) return e;
-                if (int e = step_adjoint(c, n, c->fptr, static_cast<char*>(c->d_log))) return e;
+                const int t1 = 1 - n % 2;
+                void* fp[NF][2];
+                slot_image(n, fp);
+                // the slots hold the injected source (what the next step read);
+                // the adjoint of step n reads the state before the injection
+                if (c->src_ntr)
+                    OPESCI_HIP_TRY(launch_src_cells(c->g, c->real_bytes, fp, t1, si[0], si[1], si[2],
+                                                    srclog + (size_t)6 * (n - k0) * rb, 1, c->stream));
+                char* log = static_cast<char*>(c->d_log) + (size_t)(n - k0) * c->log_step * rb;
+                if (int e = step_adjoint(c, n, fp, log)) return e;
             }
-            continue;
         }
-        // level pointers of step n: t1 in slot n - k0, t0 in the slot before
-        // (or the checkpoint)
-        auto slot_image = [&](int n, void* fp[NF][2]) {
-            const int k = n - k0, t0 = n % 2, t1 = 1 - t0;
-            for (int f = 0; f < NF; ++f) {
-                fp[f][t1] = c->d_seg + (size_t)k * slot + (size_t)f * fb + zo;
-                fp[f][t0] = k > 0 ? c->d_seg + (size_t)(k - 1) * slot + (size_t)f * fb + zo
-                                  : ck + (size_t)(2 * f + t0) * fb + zo;
-            }
-        };
-        for (int n = k0; n < k1; ++n) {
-            const int k = n - k0, t0 = n % 2, t1 = 1 - t0;
-            char* sk = c->d_seg + (size_t)k * slot;
-            if (slot_par[k] != t1) {
-                for (int f = 0; f < NF; ++f)
-                    OPESCI_HIP_TRY(hipMemcpyAsync(sk + (size_t)f * fb, ck + (size_t)(2 * f + t1) * fb, fb,
-                                                  hipMemcpyDeviceToDevice, c->stream));
-                slot_par[k] = t1;
-            }
-            void* fp[NF][2];
-            slot_image(n, fp);
-            void* fs[NF][2];
-            for (int f = 0; f < NF; ++f)
-                fs[f][0] = fs[f][1] = k >= 2 ? c->d_seg + (size_t)(k - 2) * slot + (size_t)f * fb + zo
-                                             : ck + (size_t)(2 * f + t1) * fb + zo;
-            OPESCI_HIP_TRY(launch_copy_cells(c->real_bytes, fp, t1, fs, t1, c->d_scells, c->n_scells, c->stream));
-            char* log = static_cast<char*>(c->d_log) + (size_t)k * c->log_step * rb;
-            char* sl = srclog + (size_t)6 * k * rb;
-            if (int e = step_logged(c, fp, t0, log)) return e;
-            if (c->src_ntr)
-                OPESCI_HIP_TRY(launch_src_cells(c->g, c->real_bytes, fp, t1, si[0], si[1], si[2], sl, 0, c->stream));
-            // source and step counter only (the traces are already recorded)
-            OPESCI_HIP_TRY(launch_record_source(c->g, c->real_bytes, c->storage, fp, t1, 0, c->d_rec, c->d_trace,
-                                                c->src_ntr, c->src_idx, c->d_wavelet, c->d_step, c->desc.ntsteps,
-                                                c->stream));
-            if (c->src_ntr)
-                OPESCI_HIP_TRY(launch_src_cells(c->g, c->real_bytes, fp, t1, si[0], si[1], si[2], sl + 3 * rb, 0,
-                                                c->stream));
-        }
-        for (int n = k1 - 1; n >= k0; --n) {
-            const int t1 = 1 - n % 2;
-            void* fp[NF][2];
-            slot_image(n, fp);
-            // the slots hold the injected source (what the next step read);
-            // the adjoint of step n reads the state before the injection
-            if (c->src_ntr)
-                OPESCI_HIP_TRY(launch_src_cells(c->g, c->real_bytes, fp, t1, si[0], si[1], si[2],
-                                                srclog + (size_t)6 * (n - k0) * rb, 1, c->stream));
-            char* log = static_cast<char*>(c->d_log) + (size_t)(n - k0) * c->log_step * rb;
-            if (int e = step_adjoint(c, n, fp, log)) return e;
-        }
-    }
-    // media-dependent boundary coefficients -> media cells, one launch per
-    // (statement, term, derivative program) of the stepping phases (the
-    // only ones whose coefficient gradients the transposes accumulate)
-    for (int ph = 2; ph < 5; ++ph)
-        for (const BcLevel& lv : c->levels[ph])
-            for (int di = lv.first; di < lv.first + lv.count; ++di) {
-                const BcDesc& d = c->h_desc[di];
-                for (int k = 0; k < d.nterms; ++k)
-                    if (d.t[k].cslot >= 0)
-                        for (int q = d.t[k].dterm0; q < d.t[k].dterm0 + d.t[k].ndterm; ++q)
-                            OPESCI_HIP_TRY(launch_adj_coef_media(c->g, c->real_bytes, c->mptr, c->gptr, c->d_desc,
-                                                                 c->d_code, c->d_dterms, c->d_gpool, di, d.npts, k, q,
-                                                                 c->stream));
-            }
-    OPESCI_HIP_TRY(hipStreamSynchronize(c->stream));
-    OPESCI_HIP_TRY(hipGetLastError());
-    return 0;
+        // media-dependent boundary coefficients -> media cells, one launch per
+        // (statement, term, derivative program) of the stepping phases (the
+        // only ones whose coefficient gradients the transposes accumulate)
+        for (int ph = 2; ph < 5; ++ph)
+            for (const BcLevel& lv : c->levels[ph])
+                for (int di = lv.first; di < lv.first + lv.count; ++di) {
+                    const BcDesc& d = c->h_desc[di];
+                    for (int k = 0; k < d.nterms; ++k)
+                        if (d.t[k].cslot >= 0)
+                            for (int q = d.t[k].dterm0; q < d.t[k].dterm0 + d.t[k].ndterm; ++q)
+                                OPESCI_HIP_TRY(launch_adj_coef_media(c->g, c->real_bytes, c->mptr, c->gptr, c->d_desc,
+                                                                     c->d_code, c->d_dterms, c->d_gpool, di, d.npts, k, q,
+                                                                     c->stream));
+                }
+        OPESCI_HIP_TRY(hipStreamSynchronize(c->stream));
+        OPESCI_HIP_TRY(hipGetLastError());
+        return 0;
+    });
 }
 
 int opesci_ctx_gradient(OpesciContext* c, const float* observed, int64_t count, int32_t every, double* misfit) {
-    if (int e = opesci_ctx_misfit_forward(c, observed, count, every, misfit)) return e;
-    return opesci_ctx_gradient_reverse(c);
+    return guarded([&]() -> int {
+        if (int e = opesci_ctx_misfit_forward(c, observed, count, every, misfit)) return e;
+        return opesci_ctx_gradient_reverse(c);
+    });
 }
 
 int opesci_ctx_download_gradient(OpesciContext* c, int32_t media, void* host) {
-    if (!c || !host || media < 0 || media >= OPESCI_M_COUNT) return fail(OPESCI_EINVAL, "bad arguments");
-    if (!c->gbase[media]) return fail(OPESCI_ESTATE, "no gradient: call opesci_ctx_gradient first");
-    const size_t dense = (size_t)c->g.nx * c->g.ny * c->g.nz;
-    OPESCI_HIP_TRY(launch_unpack(c->g, c->real_bytes, OPESCI_STORE_NATIVE, c->gptr[media], c->d_stage, c->stream));
-    OPESCI_HIP_TRY(hipMemcpyAsync(host, c->d_stage, dense * c->real_bytes, hipMemcpyDefault, c->stream));
-    OPESCI_HIP_TRY(hipStreamSynchronize(c->stream));
-    return 0;
+    return guarded([&]() -> int {
+        if (!c || !host || media < 0 || media >= OPESCI_M_COUNT) return fail(OPESCI_EINVAL, "bad arguments");
+        if (!c->gbase[media]) return fail(OPESCI_ESTATE, "no gradient: call opesci_ctx_gradient first");
+        const size_t dense = (size_t)c->g.nx * c->g.ny * c->g.nz;
+        OPESCI_HIP_TRY(launch_unpack(c->g, c->real_bytes, OPESCI_STORE_NATIVE, c->gptr[media], c->d_stage, c->stream));
+        OPESCI_HIP_TRY(hipMemcpyAsync(host, c->d_stage, dense * c->real_bytes, hipMemcpyDefault, c->stream));
+        OPESCI_HIP_TRY(hipStreamSynchronize(c->stream));
+        return 0;
+    });
 }
 
 int opesci_ctx_download_media(OpesciContext* c, int32_t media, void* host) {
-    if (!c || !host || media < 0 || media >= OPESCI_M_COUNT) return fail(OPESCI_EINVAL, "bad arguments");
-    if (!c->het || !c->mptr[media]) return fail(OPESCI_EUNSUPPORTED, "media arrays exist on the read path only");
-    const size_t dense = (size_t)c->g.nx * c->g.ny * c->g.nz;
-    OPESCI_HIP_TRY(launch_unpack(c->g, c->real_bytes, OPESCI_STORE_NATIVE, c->mptr[media], c->d_stage, c->stream));
-    OPESCI_HIP_TRY(hipMemcpyAsync(host, c->d_stage, dense * c->real_bytes, hipMemcpyDefault, c->stream));
-    OPESCI_HIP_TRY(hipStreamSynchronize(c->stream));
-    return 0;
+    return guarded([&]() -> int {
+        if (!c || !host || media < 0 || media >= OPESCI_M_COUNT) return fail(OPESCI_EINVAL, "bad arguments");
+        if (!c->het || !c->mptr[media]) return fail(OPESCI_EUNSUPPORTED, "media arrays exist on the read path only");
+        const size_t dense = (size_t)c->g.nx * c->g.ny * c->g.nz;
+        OPESCI_HIP_TRY(launch_unpack(c->g, c->real_bytes, OPESCI_STORE_NATIVE, c->mptr[media], c->d_stage, c->stream));
+        OPESCI_HIP_TRY(hipMemcpyAsync(host, c->d_stage, dense * c->real_bytes, hipMemcpyDefault, c->stream));
+        OPESCI_HIP_TRY(hipStreamSynchronize(c->stream));
+        return 0;
+    });
 }
 
 int opesci_ctx_upload_media(OpesciContext* c, int32_t media, const void* host) {
-    if (!c || !host || media < 0 || media >= OPESCI_M_COUNT) return fail(OPESCI_EINVAL, "bad arguments");
-    if (!c->het || !c->mptr[media]) return fail(OPESCI_EUNSUPPORTED, "media arrays exist on the read path only");
-    c->grad_every = 0;
-    const size_t dense = (size_t)c->g.nx * c->g.ny * c->g.nz;
-    OPESCI_HIP_TRY(hipMemcpyAsync(c->d_stage, host, dense * c->real_bytes, hipMemcpyDefault, c->stream));
-    OPESCI_HIP_TRY(launch_pack_media(c->g, c->real_bytes, c->d_stage, c->mptr[media], c->stream));
-    if (c->d_pool)
-        for (int ph = 0; ph < 5; ++ph)
-            for (const BcLevel& lv : c->levels[ph])
-                OPESCI_HIP_TRY(launch_bc_coef(c->g, c->real_bytes, c->mptr, c->d_desc, c->d_code, c->d_pool, lv.first,
-                                              lv.count, lv.maxpts, c->stream));
-    OPESCI_HIP_TRY(hipStreamSynchronize(c->stream));
-    return 0;
+    return guarded([&]() -> int {
+        if (!c || !host || media < 0 || media >= OPESCI_M_COUNT) return fail(OPESCI_EINVAL, "bad arguments");
+        if (!c->het || !c->mptr[media]) return fail(OPESCI_EUNSUPPORTED, "media arrays exist on the read path only");
+        c->grad_every = 0;
+        const size_t dense = (size_t)c->g.nx * c->g.ny * c->g.nz;
+        OPESCI_HIP_TRY(hipMemcpyAsync(c->d_stage, host, dense * c->real_bytes, hipMemcpyDefault, c->stream));
+        OPESCI_HIP_TRY(launch_pack_media(c->g, c->real_bytes, c->d_stage, c->mptr[media], c->stream));
+        if (c->d_pool)
+            for (int ph = 0; ph < 5; ++ph)
+                for (const BcLevel& lv : c->levels[ph])
+                    OPESCI_HIP_TRY(launch_bc_coef(c->g, c->real_bytes, c->mptr, c->d_desc, c->d_code, c->d_pool, lv.first,
+                                                  lv.count, lv.maxpts, c->stream));
+        OPESCI_HIP_TRY(hipStreamSynchronize(c->stream));
+        return 0;
+    });
 }
 
 // ---------------------------------------------------------------- batches
@@ -1839,179 +1898,193 @@ static int batch_graph(OpesciBatch* b) {
 }
 
 int opesci_batch_create(OpesciContext** ctxs, int32_t n, OpesciBatch** out) {
-    if (!ctxs || !out || n <= 0) return fail(OPESCI_EINVAL, "batch needs at least one context");
-    *out = nullptr;
-    for (int i = 0; i < n; ++i) {
-        if (!ctxs[i]) return fail(OPESCI_EINVAL, "null context in batch");
-        for (int j = 0; j < i; ++j)
-            if (ctxs[j] == ctxs[i]) return fail(OPESCI_EINVAL, "context listed twice in batch");
-    }
-    OpesciBatch* b = new OpesciBatch();
-    b->ctx.assign(ctxs, ctxs + n);
-    b->ev.assign(n + 1, nullptr);
-    hipError_t he = hipStreamCreateWithFlags(&b->origin, hipStreamNonBlocking);
-    for (int i = 0; i <= n && he == hipSuccess; ++i) he = hipEventCreateWithFlags(&b->ev[i], hipEventDisableTiming);
-    if (he != hipSuccess) {
-        batch_free(b);
-        return fail(OPESCI_EHIP, std::string("batch setup: ") + hipGetErrorString(he));
-    }
-    *out = b;
-    return 0;
+    return guarded([&]() -> int {
+        if (!ctxs || !out || n <= 0) return fail(OPESCI_EINVAL, "batch needs at least one context");
+        *out = nullptr;
+        for (int i = 0; i < n; ++i) {
+            if (!ctxs[i]) return fail(OPESCI_EINVAL, "null context in batch");
+            for (int j = 0; j < i; ++j)
+                if (ctxs[j] == ctxs[i]) return fail(OPESCI_EINVAL, "context listed twice in batch");
+        }
+        OpesciBatch* b = new OpesciBatch();
+        b->ctx.assign(ctxs, ctxs + n);
+        b->ev.assign(n + 1, nullptr);
+        hipError_t he = hipStreamCreateWithFlags(&b->origin, hipStreamNonBlocking);
+        for (int i = 0; i <= n && he == hipSuccess; ++i) he = hipEventCreateWithFlags(&b->ev[i], hipEventDisableTiming);
+        if (he != hipSuccess) {
+            batch_free(b);
+            return fail(OPESCI_EHIP, std::string("batch setup: ") + hipGetErrorString(he));
+        }
+        *out = b;
+        return 0;
+    });
 }
 
 int opesci_batch_destroy(OpesciBatch* b) {
-    batch_free(b);
-    return 0;
+    return guarded([&]() -> int {
+        batch_free(b);
+        return 0;
+    });
 }
 
 int opesci_batch_run(OpesciBatch* b, int32_t nsteps, double* seconds) {
-    if (!b) return fail(OPESCI_EINVAL, "null batch");
-    if (nsteps < 0) return fail(OPESCI_EINVAL, "negative step count");
-    const int s0 = b->ctx[0]->steps;
-    for (OpesciContext* c : b->ctx) {
-        if (!c->initialised) return fail(OPESCI_ESTATE, "every context of the batch must be initialised");
-        if (c->steps != s0) return fail(OPESCI_ESTATE, "contexts of a batch must be at the same step");
-    }
-    // the graph holds an even/odd step pair: bring an odd start to even first
-    int left = nsteps;
-    double extra = 0;
-    if (s0 % 2 == 1 && left > 0) {
+    return guarded([&]() -> int {
+        if (!b) return fail(OPESCI_EINVAL, "null batch");
+        if (nsteps < 0) return fail(OPESCI_EINVAL, "negative step count");
+        const int s0 = b->ctx[0]->steps;
         for (OpesciContext* c : b->ctx) {
-            double s = 0;
-            if (int e = ctx_run(c, 1, &s)) return e;
-            extra += s;
+            if (!c->initialised) return fail(OPESCI_ESTATE, "every context of the batch must be initialised");
+            if (c->steps != s0) return fail(OPESCI_ESTATE, "contexts of a batch must be at the same step");
         }
-        --left;
-    }
-    if (left >= 2)
-        if (int e = batch_graph(b)) return e;
-    hipEvent_t e0, e1;
-    OPESCI_HIP_TRY(hipEventCreate(&e0));
-    OPESCI_HIP_TRY(hipEventCreate(&e1));
-    for (OpesciContext* c : b->ctx) OPESCI_HIP_TRY(hipStreamSynchronize(c->stream));
-    OPESCI_HIP_TRY(hipEventRecord(e0, b->origin));
-    int rc = 0;
-    const int reps = left / 2;
-    for (int r = 0; r < reps && !rc; ++r) {
-        hipError_t he = hipGraphLaunch(b->graph, b->origin);
-        if (he != hipSuccess) rc = fail(OPESCI_EHIP, std::string("hipGraphLaunch: ") + hipGetErrorString(he));
-    }
-    OPESCI_HIP_TRY(hipEventRecord(e1, b->origin));
-    OPESCI_HIP_TRY(hipEventSynchronize(e1));
-    float ms = 0;
-    OPESCI_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (rc) return rc;
-    for (OpesciContext* c : b->ctx) c->steps += 2 * reps;
-    if (left % 2 == 1)
-        for (OpesciContext* c : b->ctx) {
-            double s = 0;
-            if (int e = ctx_run(c, 1, &s)) return e;
-            extra += s;
+        // the graph holds an even/odd step pair: bring an odd start to even first
+        int left = nsteps;
+        double extra = 0;
+        if (s0 % 2 == 1 && left > 0) {
+            for (OpesciContext* c : b->ctx) {
+                double s = 0;
+                if (int e = ctx_run(c, 1, &s)) return e;
+                extra += s;
+            }
+            --left;
         }
-    if (seconds) *seconds = ms * 1e-3 + extra;
-    return 0;
+        if (left >= 2)
+            if (int e = batch_graph(b)) return e;
+        hipEvent_t e0, e1;
+        OPESCI_HIP_TRY(hipEventCreate(&e0));
+        OPESCI_HIP_TRY(hipEventCreate(&e1));
+        for (OpesciContext* c : b->ctx) OPESCI_HIP_TRY(hipStreamSynchronize(c->stream));
+        OPESCI_HIP_TRY(hipEventRecord(e0, b->origin));
+        int rc = 0;
+        const int reps = left / 2;
+        for (int r = 0; r < reps && !rc; ++r) {
+            hipError_t he = hipGraphLaunch(b->graph, b->origin);
+            if (he != hipSuccess) rc = fail(OPESCI_EHIP, std::string("hipGraphLaunch: ") + hipGetErrorString(he));
+        }
+        OPESCI_HIP_TRY(hipEventRecord(e1, b->origin));
+        OPESCI_HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0;
+        OPESCI_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (rc) return rc;
+        for (OpesciContext* c : b->ctx) c->steps += 2 * reps;
+        if (left % 2 == 1)
+            for (OpesciContext* c : b->ctx) {
+                double s = 0;
+                if (int e = ctx_run(c, 1, &s)) return e;
+                extra += s;
+            }
+        if (seconds) *seconds = ms * 1e-3 + extra;
+        return 0;
+    });
 }
 
 // ---------------------------------------------------------------- reference-named ABI
 int opesci_model_load(const OpesciModelDesc* desc) {
-    if (int e = validate(desc)) return e;
-    if (g_default) {
-        destroy(g_default);
-        g_default = nullptr;
-    }
-    // keep a deep copy: the generated TU owned its tables for the process lifetime
-    const size_t dense = (size_t)desc->dims[0] * desc->dims[1] * desc->dims[2];
-    const size_t sz_st = (size_t)desc->nstmts * sizeof(OpesciBcStmt), sz_te = (size_t)desc->nterms * sizeof(OpesciBcTerm);
-    const size_t sz_co = (size_t)desc->ncode * sizeof(OpesciExprOp), sz_so = (size_t)desc->nsol * sizeof(OpesciExprOp);
-    const size_t sz_me = desc->media_arrays ? 3 * dense * sizeof(float) : 0;
-    const size_t sz_w = (size_t)std::max(desc->ntsteps, 1) * desc->src_on * sizeof(float);
-    const size_t sz_r = (size_t)desc->nrec * 3 * sizeof(int32_t);
-    const size_t sz_dt = (size_t)desc->ndterms * sizeof(OpesciBcDTerm);
-    g_loaded_blob.assign(sz_st + sz_te + sz_co + sz_so + sz_me + sz_w + sz_r + sz_dt + 72, 0);
-    char* p = g_loaded_blob.data();
-    g_loaded = *desc;
-    auto take = [&](const void* src, size_t n) -> void* {
-        if (!n) return nullptr;
-        void* dst = p;
-        memcpy(dst, src, n);
-        p += (n + 7) & ~size_t(7);
-        return dst;
-    };
-    g_loaded.stmts = (const OpesciBcStmt*)take(desc->stmts, sz_st);
-    g_loaded.terms = (const OpesciBcTerm*)take(desc->terms, sz_te);
-    g_loaded.bc_code = (const OpesciExprOp*)take(desc->bc_code, sz_co);
-    g_loaded.sol_code = (const OpesciExprOp*)take(desc->sol_code, sz_so);
-    if (desc->media_arrays) {
-        g_loaded.rho = (const float*)take(desc->rho, dense * sizeof(float));
-        g_loaded.vp = (const float*)take(desc->vp, dense * sizeof(float));
-        g_loaded.vs = (const float*)take(desc->vs, dense * sizeof(float));
-    }
-    g_loaded.src_wavelet = desc->src_on ? (const float*)take(desc->src_wavelet, sz_w) : nullptr;
-    g_loaded.rec_idx = desc->nrec ? (const int32_t*)take(desc->rec_idx, sz_r) : nullptr;
-    g_loaded.dterms = desc->ndterms ? (const OpesciBcDTerm*)take(desc->dterms, sz_dt) : nullptr;
-    g_have_loaded = true;
-    return 0;
+    return guarded([&]() -> int {
+        if (int e = validate(desc)) return e;
+        if (g_default) {
+            destroy(g_default);
+            g_default = nullptr;
+        }
+        // keep a deep copy: the generated TU owned its tables for the process lifetime
+        const size_t dense = (size_t)desc->dims[0] * desc->dims[1] * desc->dims[2];
+        const size_t sz_st = (size_t)desc->nstmts * sizeof(OpesciBcStmt), sz_te = (size_t)desc->nterms * sizeof(OpesciBcTerm);
+        const size_t sz_co = (size_t)desc->ncode * sizeof(OpesciExprOp), sz_so = (size_t)desc->nsol * sizeof(OpesciExprOp);
+        const size_t sz_me = desc->media_arrays ? 3 * dense * sizeof(float) : 0;
+        const size_t sz_w = (size_t)std::max(desc->ntsteps, 1) * desc->src_on * sizeof(float);
+        const size_t sz_r = (size_t)desc->nrec * 3 * sizeof(int32_t);
+        const size_t sz_dt = (size_t)desc->ndterms * sizeof(OpesciBcDTerm);
+        g_loaded_blob.assign(sz_st + sz_te + sz_co + sz_so + sz_me + sz_w + sz_r + sz_dt + 72, 0);
+        char* p = g_loaded_blob.data();
+        g_loaded = *desc;
+        auto take = [&](const void* src, size_t n) -> void* {
+            if (!n) return nullptr;
+            void* dst = p;
+            memcpy(dst, src, n);
+            p += (n + 7) & ~size_t(7);
+            return dst;
+        };
+        g_loaded.stmts = (const OpesciBcStmt*)take(desc->stmts, sz_st);
+        g_loaded.terms = (const OpesciBcTerm*)take(desc->terms, sz_te);
+        g_loaded.bc_code = (const OpesciExprOp*)take(desc->bc_code, sz_co);
+        g_loaded.sol_code = (const OpesciExprOp*)take(desc->sol_code, sz_so);
+        if (desc->media_arrays) {
+            g_loaded.rho = (const float*)take(desc->rho, dense * sizeof(float));
+            g_loaded.vp = (const float*)take(desc->vp, dense * sizeof(float));
+            g_loaded.vs = (const float*)take(desc->vs, dense * sizeof(float));
+        }
+        g_loaded.src_wavelet = desc->src_on ? (const float*)take(desc->src_wavelet, sz_w) : nullptr;
+        g_loaded.rec_idx = desc->nrec ? (const int32_t*)take(desc->rec_idx, sz_r) : nullptr;
+        g_loaded.dterms = desc->ndterms ? (const OpesciBcDTerm*)take(desc->dterms, sz_dt) : nullptr;
+        g_have_loaded = true;
+        return 0;
+    });
 }
 
 int opesci_execute(OpesciGrid* grid, OpesciProfiling* profiling) {
-    if (!grid) return fail(OPESCI_EINVAL, "null grid");
-    if (!g_have_loaded) return fail(OPESCI_ESTATE, "no model loaded: call opesci_model_load first");
-    if (g_default) {
-        destroy(g_default);
-        g_default = nullptr;
-    }
-    if (int e = create(&g_loaded, &g_default)) return e;
-    OpesciContext* c = g_default;
-    if (int e = ctx_init(c)) return e;
-    double secs = 0;
-    if (int e = ctx_run(c, c->desc.ntsteps, &secs)) return e;
-    const size_t dense = (size_t)c->g.nx * c->g.ny * c->g.nz;
-    void** slots = (void**)grid;
-    for (int f = 0; f < NF; ++f) {
-        void* h = nullptr;
-        if (posix_memalign(&h, 4096, 2 * dense * c->real_bytes) != 0 || !h)
-            return fail(OPESCI_ENOMEM, "host allocation for the result fields failed");
-        for (int l = 0; l < 2; ++l)
-            if (int e = ctx_download(c, f, l, (char*)h + (size_t)l * dense * c->real_bytes)) {
-                free(h);
-                return e;
-            }
-        slots[f] = h;
-    }
-    if (profiling) {
-        profiling->g_rtime = (float)secs;
-        profiling->g_ptime = (float)secs;
-        profiling->g_mflops =
-            secs > 0 ? (float)(flops_per_point(c) * interior_points(c) * c->desc.ntsteps / secs * 1e-6) : 0.f;
-    }
-    return 0;
+    return guarded([&]() -> int {
+        if (!grid) return fail(OPESCI_EINVAL, "null grid");
+        if (!g_have_loaded) return fail(OPESCI_ESTATE, "no model loaded: call opesci_model_load first");
+        if (g_default) {
+            destroy(g_default);
+            g_default = nullptr;
+        }
+        if (int e = create(&g_loaded, &g_default)) return e;
+        OpesciContext* c = g_default;
+        if (int e = ctx_init(c)) return e;
+        double secs = 0;
+        if (int e = ctx_run(c, c->desc.ntsteps, &secs)) return e;
+        const size_t dense = (size_t)c->g.nx * c->g.ny * c->g.nz;
+        void** slots = (void**)grid;
+        for (int f = 0; f < NF; ++f) {
+            void* h = nullptr;
+            if (posix_memalign(&h, 4096, 2 * dense * c->real_bytes) != 0 || !h)
+                return fail(OPESCI_ENOMEM, "host allocation for the result fields failed");
+            for (int l = 0; l < 2; ++l)
+                if (int e = ctx_download(c, f, l, (char*)h + (size_t)l * dense * c->real_bytes)) {
+                    free(h);
+                    return e;
+                }
+            slots[f] = h;
+        }
+        if (profiling) {
+            profiling->g_rtime = (float)secs;
+            profiling->g_ptime = (float)secs;
+            profiling->g_mflops =
+                secs > 0 ? (float)(flops_per_point(c) * interior_points(c) * c->desc.ntsteps / secs * 1e-6) : 0.f;
+        }
+        return 0;
+    });
 }
 
 int opesci_convergence(OpesciGrid* grid, void* conv) {
-    if (!grid || !conv) return fail(OPESCI_EINVAL, "null argument");
-    if (!g_default || !g_default->initialised)
-        return fail(OPESCI_ESTATE, "convergence before execute: call opesci_execute first");
-    double l2[NF];
-    if (int e = ctx_l2(g_default, l2)) return e;
-    for (int f = 0; f < NF; ++f) {
-        if (g_default->real_bytes == 8)
-            ((double*)conv)[f] = l2[f];
-        else
-            ((float*)conv)[f] = (float)l2[f];
-    }
-    return 0;
+    return guarded([&]() -> int {
+        if (!grid || !conv) return fail(OPESCI_EINVAL, "null argument");
+        if (!g_default || !g_default->initialised)
+            return fail(OPESCI_ESTATE, "convergence before execute: call opesci_execute first");
+        double l2[NF];
+        if (int e = ctx_l2(g_default, l2)) return e;
+        for (int f = 0; f < NF; ++f) {
+            if (g_default->real_bytes == 8)
+                ((double*)conv)[f] = l2[f];
+            else
+                ((float*)conv)[f] = (float)l2[f];
+        }
+        return 0;
+    });
 }
 
 int opesci_free(OpesciGrid* grid) {
-    if (!grid) return fail(OPESCI_EINVAL, "null grid");
-    void** slots = (void**)grid;
-    for (int f = 0; f < NF; ++f) {
-        free(slots[f]);
-        slots[f] = nullptr;
-    }
-    return 0;
+    return guarded([&]() -> int {
+        if (!grid) return fail(OPESCI_EINVAL, "null grid");
+        void** slots = (void**)grid;
+        for (int f = 0; f < NF; ++f) {
+            free(slots[f]);
+            slots[f] = nullptr;
+        }
+        return 0;
+    });
 }
 
 }  // extern "C"
