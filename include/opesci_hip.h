@@ -248,7 +248,9 @@ int opesci_ctx_l2(OpesciContext* ctx, double out[OPESCI_NFIELDS]);
  * sums are grid-stride, so any grid size fits); `d_out` is device memory
  * for 9 doubles; the work is enqueued on `stream` (a hipStream_t; null =
  * the context's stream) with no host synchronisation.  The context's device
- * must be current.  Calls that overlap on one device race on the array.
+ * must be current.  Calls that overlap on one device race on the array, and
+ * calls that overlap on one context also on the context's tables of the
+ * analytic solution (allocated with the context, refilled by every L2).
  * Same block count and summation order as opesci_ctx_l2. */
 int opesci_ctx_l2_noalloc(OpesciContext* ctx, double* d_out, void* stream);
 /* copy one field, one time level (0/1, or -1 = latest) to/from a dense host array real_t[dim1][dim2][dim3] */
