@@ -50,6 +50,25 @@ def test_fields_match_plan_interpreter(order, double):
         assert max_rel(fields[f], ref[f], scale) < tol, (FIELDS[f], max_rel(fields[f], ref[f], scale))
 
 
+@pytest.mark.parametrize("order,double", [(4, True), (4, False), (8, True)])
+def test_large_boundary_levels_match_plan_interpreter(order, double):
+    """93^3 interior (face planes of 97^2 and 101^2 points, boundary
+    launches of many blocks per statement, which the smaller grids above
+    never reach): every field after 6 steps against the plan interpreter
+    (FP64 1e-12, FP32 2e-6 of the field maximum)."""
+    g, kw = eigen_model(93, order, 6, double=double)
+    plan = g.lower()
+    fields, _ = gpu_run(plan)
+    vm = PlanVM(plan)
+    vm.init()
+    vm.run()
+    ref = vm.latest()
+    scale = np.abs(ref[0]).max()
+    tol = 1e-12 if double else 2e-6
+    for f in range(9):
+        assert max_rel(fields[f], ref[f], scale) < tol, (FIELDS[f], max_rel(fields[f], ref[f], scale))
+
+
 @pytest.mark.parametrize("order,double", [(2, True), (4, True), (4, False), (8, True), (8, False)])
 def test_l2_matches_oracle(order, double):
     """FP64: 1e-10 relative.  FP32: two FP32 roundings of one scheme differ
