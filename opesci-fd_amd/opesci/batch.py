@@ -20,9 +20,10 @@ from . import backend
 __all__ = ['Batch', 'autotune', 'CONFIGS']
 
 # (kernel_variant, tile_y, x_chunk): the default sweep (two z points per lane
-# for FP32 arithmetic) with its default rows, 8 or 4 rows per tile, and x
-# chunks of 8, 16 or 32 planes beside the automatic choice; the
-# one-point-per-lane sweep with 16 or 8 rows, automatic or 16-plane chunks
+# for FP32 arithmetic, 128-point z tiles) with its default rows, 8 or 4 rows
+# per tile, and x chunks of 8, 16 or 32 planes beside the automatic choice;
+# the one-point-per-lane sweep (64-point z tiles) with 16 or 8 rows,
+# automatic or 16-plane chunks
 CONFIGS = ((0, 0, 0), (0, 8, 0), (0, 4, 0), (0, 0, 8), (0, 0, 16), (0, 0, 32), (0, 4, 16),
            (2, 16, 0), (2, 8, 0), (2, 8, 16))
 
