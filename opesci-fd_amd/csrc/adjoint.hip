@@ -455,29 +455,17 @@ __global__ __launch_bounds__(256) void bc_log_kernel(Geom g, FieldPtrs<R> F, con
     bc_point<R, R>(g, F, D, pool, p, i0, i1, t0, t1);
 }
 
-template <typename R>
-__global__ __launch_bounds__(256) void bc_undo_kernel(Geom g, FieldPtrs<R> F, const BcDesc* __restrict__ desc,
-                                                      int first, int maxpts, const R* __restrict__ log, int t0,
-                                                      int t1) {
-    const BcDesc* D = desc + first + blockIdx.y;
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= D->npts) return;
-    const int n1 = D->n1, i0 = D->lo0 + p / n1, i1 = D->lo1 + p % n1;
-    const int o0 = D->axis == 0 ? 1 : 0, o1 = D->axis == 2 ? 1 : 2;
-    const int64_t so0 = o0 == 0 ? g.sx : g.sy, so1 = o1 == 1 ? g.sy : 1;
-    const int64_t dst = (int64_t)i0 * so0 + (int64_t)i1 * so1 + D->disp;
-    const int fl = D->fl;
-    F.f[fl >> 1][(fl & 1) ? t1 : t0][dst] = log[(int64_t)blockIdx.y * maxpts + p];
-}
-
 // Transposed statements of one level, first pass: each point's adjoint
 // value a = bar[dst] into the level's buffer, bar[dst] = 0, and the
-// coefficient gradients into gpool (one slot per point and term).
+// coefficient gradients into gpool (one slot per point and term); then the
+// forward value the level overwrote at dst is restored from the undo log
+// (the pass reads forward values at the level's sources only, which no
+// statement of the level writes).
 template <typename R>
 __global__ __launch_bounds__(256) void adj_bc_kernel(Geom g, FieldPtrs<R> F, AdjPtrs<R> B,
                                                      const BcDesc* __restrict__ desc, const R* __restrict__ pool,
                                                      R* gpool, int first, const int32_t* __restrict__ boff, R* buf,
-                                                     int t0, int t1) {
+                                                     const R* __restrict__ log, int maxpts, int t0, int t1) {
     const int d = first + blockIdx.y;
     const BcDesc* D = desc + d;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -491,13 +479,14 @@ __global__ __launch_bounds__(256) void adj_bc_kernel(Geom g, FieldPtrs<R> F, Adj
     const R a = *bd;
     *bd = 0;
     buf[boff[d] + p] = a;
-    if (a == (R)0) return;
-    for (int k = 0; k < D->nterms; ++k)
-        if (D->t[k].cslot >= 0) {
-            const int sfl = D->t[k].fl;
-            const int64_t ci = D->cpool + (int64_t)D->t[k].cslot * D->npts + p;
-            gpool[ci] += a * F.f[sfl >> 1][(sfl & 1) ? t1 : t0][dst + D->t[k].disp];
-        }
+    if (a != (R)0)
+        for (int k = 0; k < D->nterms; ++k)
+            if (D->t[k].cslot >= 0) {
+                const int sfl = D->t[k].fl;
+                const int64_t ci = D->cpool + (int64_t)D->t[k].cslot * D->npts + p;
+                gpool[ci] += a * F.f[sfl >> 1][(sfl & 1) ? t1 : t0][dst + D->t[k].disp];
+            }
+    F.f[fl >> 1][(fl & 1) ? t1 : t0][dst] = log[(int64_t)blockIdx.y * maxpts + p];
 }
 
 // Second pass: bar[src] += c_k a over the source cell's contributions, in
@@ -763,34 +752,29 @@ hipError_t launch_adj_velocity(const Geom& g, int real_bytes, void* fields, void
 
 template <typename R>
 static hipError_t bc_log_t(const Geom& g, void* fields, const BcDesc* desc, const void* pool, const BcLevel& lv,
-                           void* log, int t0, int t1, int undo, hipStream_t s) {
+                           void* log, int t0, int t1, hipStream_t s) {
     if (lv.count <= 0 || lv.maxpts <= 0) return hipSuccess;
     dim3 grid((lv.maxpts + 255) / 256, lv.count);
-    const FieldPtrs<R>& F = *static_cast<const FieldPtrs<R>*>(fields);
-    if (undo)
-        hipLaunchKernelGGL((bc_undo_kernel<R>), grid, dim3(256), 0, s, g, F, desc, lv.first, lv.maxpts,
-                           static_cast<const R*>(log), t0, t1);
-    else
-        hipLaunchKernelGGL((bc_log_kernel<R>), grid, dim3(256), 0, s, g, F, desc, static_cast<const R*>(pool),
-                           lv.first, lv.maxpts, static_cast<R*>(log), t0, t1);
+    hipLaunchKernelGGL((bc_log_kernel<R>), grid, dim3(256), 0, s, g, *static_cast<const FieldPtrs<R>*>(fields), desc,
+                       static_cast<const R*>(pool), lv.first, lv.maxpts, static_cast<R*>(log), t0, t1);
     return hipGetLastError();
 }
 
 hipError_t launch_bc_logged(const Geom& g, int real_bytes, void* fields, const BcDesc* desc, const void* pool,
-                            const BcLevel& lv, void* log, int t0, int t1, int undo, hipStream_t s) {
-    return real_bytes == 8 ? bc_log_t<double>(g, fields, desc, pool, lv, log, t0, t1, undo, s)
-                           : bc_log_t<float>(g, fields, desc, pool, lv, log, t0, t1, undo, s);
+                            const BcLevel& lv, void* log, int t0, int t1, hipStream_t s) {
+    return real_bytes == 8 ? bc_log_t<double>(g, fields, desc, pool, lv, log, t0, t1, s)
+                           : bc_log_t<float>(g, fields, desc, pool, lv, log, t0, t1, s);
 }
 
 template <typename R>
 static hipError_t adj_bc_t(const Geom& g, void* fields, void* bar, const BcDesc* desc, const void* pool, void* gpool,
-                           const BcLevel& lv, const AdjGather& ga, int t0, int t1, hipStream_t s) {
+                           const BcLevel& lv, const AdjGather& ga, const void* log, int t0, int t1, hipStream_t s) {
     if (lv.count <= 0 || lv.maxpts <= 0) return hipSuccess;
     dim3 grid((lv.maxpts + 255) / 256, lv.count);
     const AdjPtrs<R>& B = *static_cast<const AdjPtrs<R>*>(bar);
     hipLaunchKernelGGL((adj_bc_kernel<R>), grid, dim3(256), 0, s, g, *static_cast<const FieldPtrs<R>*>(fields), B,
                        desc, static_cast<const R*>(pool), static_cast<R*>(gpool), lv.first, ga.boff,
-                       static_cast<R*>(ga.buf), t0, t1);
+                       static_cast<R*>(ga.buf), static_cast<const R*>(log), lv.maxpts, t0, t1);
     if (hipError_t e = hipGetLastError()) return e;
     if (ga.ntgt <= 0) return hipSuccess;
     hipLaunchKernelGGL((adj_bc_gather_kernel<R>), dim3((ga.ntgt + 255) / 256), dim3(256), 0, s, B, desc,
@@ -799,9 +783,10 @@ static hipError_t adj_bc_t(const Geom& g, void* fields, void* bar, const BcDesc*
 }
 
 hipError_t launch_adj_bc(const Geom& g, int real_bytes, void* fields, void* bar, const BcDesc* desc, const void* pool,
-                         void* gpool, const BcLevel& lv, const AdjGather& ga, int t0, int t1, hipStream_t s) {
-    return real_bytes == 8 ? adj_bc_t<double>(g, fields, bar, desc, pool, gpool, lv, ga, t0, t1, s)
-                           : adj_bc_t<float>(g, fields, bar, desc, pool, gpool, lv, ga, t0, t1, s);
+                         void* gpool, const BcLevel& lv, const AdjGather& ga, const void* log, int t0, int t1,
+                         hipStream_t s) {
+    return real_bytes == 8 ? adj_bc_t<double>(g, fields, bar, desc, pool, gpool, lv, ga, log, t0, t1, s)
+                           : adj_bc_t<float>(g, fields, bar, desc, pool, gpool, lv, ga, log, t0, t1, s);
 }
 
 template <typename R>

@@ -323,8 +323,10 @@ hipError_t launch_adj_stress(const Geom& g, int real_bytes, void* fields, void* 
 hipError_t launch_adj_velocity(const Geom& g, int real_bytes, void* fields, void* bar, const void* media, void* grad,
                                const void* coefs, void* const* scratch, const AdjCells& cells,
                                const InteriorLaunch& cfg, int t0, int t1, hipStream_t s);
+// one boundary level, each overwritten value first saved into `log`
+// (descriptor-major, lv.maxpts slots per descriptor); launch_adj_bc restores them
 hipError_t launch_bc_logged(const Geom& g, int real_bytes, void* fields, const BcDesc* desc, const void* pool,
-                            const BcLevel& lv, void* log, int t0, int t1, int undo, hipStream_t s);
+                            const BcLevel& lv, void* log, int t0, int t1, hipStream_t s);
 // Fixed-order accumulation of one level's transposes: targets in ascending
 // cell order, each summing its contributions in (statement, point, term)
 // order from a buffer the level's points fill first, so results do not
@@ -340,8 +342,11 @@ struct AdjGather {
     const int32_t* boff = nullptr;    // per descriptor: its first buffer slot
     void* buf = nullptr;              // the level's slots (compute type)
 };
+// the level's transposes; its first pass also restores the forward values
+// the level overwrote from `log` (the level's undo log, as bc_log wrote it)
 hipError_t launch_adj_bc(const Geom& g, int real_bytes, void* fields, void* bar, const BcDesc* desc, const void* pool,
-                         void* gpool, const BcLevel& lv, const AdjGather& ga, int t0, int t1, hipStream_t s);
+                         void* gpool, const BcLevel& lv, const AdjGather& ga, const void* log, int t0, int t1,
+                         hipStream_t s);
 // media gradients through derivative program q of term k of descriptor d
 // (one launch per program, in a fixed order)
 hipError_t launch_adj_coef_media(const Geom& g, int real_bytes, const void* media, void* grad, const BcDesc* desc,

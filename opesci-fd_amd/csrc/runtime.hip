@@ -1549,7 +1549,7 @@ static int step_logged(OpesciContext* c, void* fp, int t0, char* log) {
     auto levels = [&](int ph) -> int {
         for (size_t L = 0; L < c->levels[ph].size(); ++L)
             OPESCI_HIP_TRY(launch_bc_logged(c->g, c->real_bytes, fp, c->d_desc, c->d_pool, c->levels[ph][L],
-                                            log + c->log_off[ph][L] * c->real_bytes, t0, t1, 0, c->stream));
+                                            log + c->log_off[ph][L] * c->real_bytes, t0, t1, c->stream));
         return 0;
     };
     OPESCI_HIP_TRY(launch_stress(c->g, c->real_bytes, c->storage, c->het, fp, c->mptr, c->coefs(), t0, t1, c->cfg,
@@ -1571,10 +1571,10 @@ static int step_adjoint(OpesciContext* c, int n, void* fp, char* log) {
     auto levels = [&](int ph) -> int {
         for (int L = (int)c->levels[ph].size() - 1; L >= 0; --L) {
             const BcLevel& lv = c->levels[ph][L];
+            // transposes, with the level's forward values restored from its log
             OPESCI_HIP_TRY(launch_adj_bc(c->g, c->real_bytes, fp, c->aptr, c->d_desc, c->d_pool, c->d_gpool, lv,
-                                         c->gat_bc[ph][L], t0, t1, c->stream));
-            OPESCI_HIP_TRY(launch_bc_logged(c->g, c->real_bytes, fp, c->d_desc, c->d_pool, lv,
-                                            log + c->log_off[ph][L] * c->real_bytes, t0, t1, 1, c->stream));
+                                         c->gat_bc[ph][L], log + c->log_off[ph][L] * c->real_bytes, t0, t1,
+                                         c->stream));
         }
         return 0;
     };
